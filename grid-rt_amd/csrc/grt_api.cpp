@@ -1,0 +1,850 @@
+// grt_api.cpp -- C ABI of libgrt (include/grt/grt.h): contexts, scenes, the
+// grid build driver and the tracing entry points.  Device memory is owned by
+// opaque handles and freed on destroy (the reference's cudaMalloc'd globals
+// were never freed, RayTracing.cu:19-22,106-170).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "grt/grt.h"
+#include "grt_internal.h"
+#include "grt_launch.h"
+
+using namespace grt;
+
+namespace {
+thread_local std::string g_err;
+}
+
+namespace grt {
+
+int set_error(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess)
+        return GRT_OK;
+    return set_error(e == hipErrorOutOfMemory ? GRT_NO_MEMORY : GRT_DEVICE_ERROR, "%s: %s", what,
+                     hipGetErrorString(e));
+}
+
+int DevMem::reserve(grt_context_t* ctx, size_t nbytes, bool zero)
+{
+    if (nbytes <= bytes)
+        return GRT_OK;
+    release(ctx);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GRT_NO_MEMORY, "hipMalloc(%zu bytes) failed: %s", nbytes,
+                         hipGetErrorString(e));
+    }
+    ptr = p;
+    bytes = nbytes;
+    ctx->bytes_used += nbytes;
+    if (zero) {
+        e = hipMemsetAsync(ptr, 0, nbytes, ctx->stream);
+        if (e != hipSuccess)
+            return hip_check(e, "hipMemsetAsync");
+    }
+    return GRT_OK;
+}
+
+void DevMem::release(grt_context_t* ctx)
+{
+    if (ptr) {
+        (void)hipFree(ptr);
+        ctx->bytes_used -= bytes;
+    }
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int scene_pack(grt_scene_t* sc)
+{
+    return hip_check(launch_pack(sc->verts.as<float>(), sc->n_tris, sc->tri.as<float4>(),
+                                 sc->ctx->stream),
+                     "pack");
+}
+
+// Ensure the cell-side buffers hold `cells` cells.  counts must stay zero
+// outside a build, so a fresh allocation is cleared.
+int grid_reserve_cells(grt_scene_t* sc, uint32_t cells)
+{
+    if (cells <= sc->cells_cap && sc->counts.ptr)
+        return GRT_OK;
+    const uint64_t cap = ((uint64_t)cells + cells / 8 + kScanTile) / kScanTile * kScanTile;
+    if (cap > 0xffffffffull)
+        return set_error(GRT_CAPACITY, "grid of %u cells is too large", cells);
+    int rc;
+    if ((rc = sc->counts.reserve(sc->ctx, cap * 4, true)) != GRT_OK)
+        return rc;
+    if ((rc = sc->offsets.reserve(sc->ctx, (cap + 1) * 4, false)) != GRT_OK)
+        return rc;
+    if ((rc = sc->block_sums.reserve(sc->ctx, (cap / kScanTile + 1) * 4, false)) != GRT_OK)
+        return rc;
+    sc->cells_cap = (uint32_t)cap;
+    return GRT_OK;
+}
+
+int grid_reserve_refs(grt_scene_t* sc, uint32_t refs)
+{
+    if (refs <= sc->refs_cap && sc->refs.ptr)
+        return GRT_OK;
+    uint64_t cap = (uint64_t)refs + refs / 8 + 1024;
+    if (cap > 0xfffffff0ull)
+        cap = 0xfffffff0ull;
+    int rc;
+    if ((rc = sc->refs.reserve(sc->ctx, cap * 4, false)) != GRT_OK)
+        return rc;
+    const uint64_t list_cap = cap / (kSortSmallCells + 1) + 1;
+    if ((rc = sc->sort_list.reserve(sc->ctx, list_cap * 4, false)) != GRT_OK)
+        return rc;
+    sc->refs_cap = (uint32_t)cap;
+    sc->list_cap = (uint32_t)list_cap;
+    return GRT_OK;
+}
+
+static int fetch_frame(grt_scene_t* sc, DevFrame* host)
+{
+    hipError_t e = hipMemcpyAsync(host, sc->frame.ptr, sizeof(DevFrame), hipMemcpyDeviceToHost,
+                                  sc->ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(sc->ctx->stream);
+    return hip_check(e, "reading the grid frame");
+}
+
+static void frame_to_info(const DevFrame& f, grt_grid_info* out)
+{
+    for (int d = 0; d < 3; ++d) {
+        out->bmin[d] = f.bmin[d];
+        out->bmax[d] = f.bmax[d];
+        out->cell[d] = f.cell[d];
+        out->inv_cell[d] = f.inv[d];
+        out->n[d] = f.n[d];
+    }
+    out->m = f.m;
+    out->r = f.r;
+}
+
+// Count -> scan -> scatter -> sort, all on the stream.
+static int enqueue_build_passes(grt_scene_t* sc, uint32_t refs_cap)
+{
+    hipStream_t s = sc->ctx->stream;
+    DevFrame* fr = sc->frame.as<DevFrame>();
+    const float* verts = sc->verts.as<float>();
+    int rc;
+    if ((rc = hip_check(launch_grid_count(fr, verts, sc->n_tris, sc->counts.as<uint32_t>(),
+                                          sc->big_list.as<uint32_t>(),
+                                          sc->small_counters.as<uint32_t>(), s),
+                        "grid count")) != GRT_OK)
+        return rc;
+    return hip_check(launch_grid_scan(fr, sc->counts.as<uint32_t>(), sc->offsets.as<uint32_t>(),
+                                      sc->block_sums.as<uint32_t>(), sc->cells_cap, refs_cap, s),
+                     "grid scan");
+}
+
+static int enqueue_scatter_sort(grt_scene_t* sc)
+{
+    hipStream_t s = sc->ctx->stream;
+    DevFrame* fr = sc->frame.as<DevFrame>();
+    int rc;
+    if ((rc = hip_check(launch_grid_scatter(fr, sc->verts.as<float>(), sc->n_tris,
+                                            sc->counts.as<uint32_t>(), sc->offsets.as<uint32_t>(),
+                                            sc->refs.as<uint32_t>(), sc->big_list.as<uint32_t>(),
+                                            sc->small_counters.as<uint32_t>(), s),
+                        "grid scatter")) != GRT_OK)
+        return rc;
+    return hip_check(launch_grid_sort(fr, sc->offsets.as<uint32_t>(), sc->refs.as<uint32_t>(),
+                                      nullptr, sc->sort_list.as<uint32_t>(), sc->list_cap,
+                                      sc->small_counters.as<uint32_t>() + 1, sc->cells_cap, s),
+                     "grid sort");
+}
+
+int scene_common_buffers(grt_scene_t* sc)
+{
+    int rc;
+    if ((rc = sc->frame.reserve(sc->ctx, sizeof(DevFrame), true)) != GRT_OK)
+        return rc;
+    if ((rc = sc->partial.reserve(sc->ctx, 6 * kBBoxMaxBlocks * sizeof(float), false)) != GRT_OK)
+        return rc;
+    if ((rc = sc->small_counters.reserve(sc->ctx, 64, true)) != GRT_OK)
+        return rc;
+    return GRT_OK;
+}
+
+}  // namespace grt
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* grt_last_error(void) { return g_err.c_str(); }
+
+int grt_init(int device, grt_context* out)
+{
+    if (!out)
+        return set_error(GRT_INVALID_PARAM, "grt_init: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        return set_error(GRT_DEVICE_ERROR, "grt_init: no HIP device (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= count)
+        return set_error(GRT_INVALID_PARAM, "grt_init: device %d out of range [0,%d)", device,
+                         count);
+    if ((e = hipSetDevice(device)) != hipSuccess)
+        return hip_check(e, "hipSetDevice");
+    grt_context_t* ctx = new (std::nothrow) grt_context_t();
+    if (!ctx)
+        return set_error(GRT_NO_MEMORY, "grt_init: out of host memory");
+    ctx->device = device;
+    if ((e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking)) != hipSuccess) {
+        delete ctx;
+        return hip_check(e, "hipStreamCreate");
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return GRT_OK;
+}
+
+int grt_shutdown(grt_context ctx)
+{
+    if (!ctx)
+        return set_error(GRT_INVALID_PARAM, "grt_shutdown: NULL context");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (grt_scene_t* sc : std::vector<grt_scene_t*>(ctx->scenes.begin(), ctx->scenes.end()))
+        grt_scene_destroy(sc);
+    if (ctx->own_stream)
+        (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return GRT_OK;
+}
+
+int grt_set_stream(grt_context ctx, void* hip_stream)
+{
+    if (!ctx)
+        return set_error(GRT_INVALID_PARAM, "grt_set_stream: NULL context");
+    ctx->stream = (hipStream_t)hip_stream;
+    return GRT_OK;
+}
+
+int grt_synchronize(grt_context ctx)
+{
+    if (!ctx)
+        return set_error(GRT_INVALID_PARAM, "grt_synchronize: NULL context");
+    return hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int grt_memory_used(grt_context ctx, uint64_t* bytes)
+{
+    if (!ctx || !bytes)
+        return set_error(GRT_INVALID_PARAM, "grt_memory_used: NULL argument");
+    *bytes = ctx->bytes_used;
+    return GRT_OK;
+}
+
+int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any)
+{
+    if (!ctx)
+        return set_error(GRT_INVALID_PARAM, "grt_set_block_size: NULL context");
+    auto ok = [](uint32_t b) { return b >= 64 && b <= 256 && (b % 64) == 0; };
+    if (!ok(closest) || !ok(any))
+        return set_error(GRT_INVALID_PARAM, "block sizes must be 64..256 in steps of 64");
+    ctx->block_closest = closest;
+    ctx->block_any = any;
+    return GRT_OK;
+}
+
+int grt_scene_create(grt_context ctx, grt_scene* out)
+{
+    if (!ctx || !out)
+        return set_error(GRT_INVALID_PARAM, "grt_scene_create: NULL argument");
+    grt_scene_t* sc = new (std::nothrow) grt_scene_t();
+    if (!sc)
+        return set_error(GRT_NO_MEMORY, "grt_scene_create: out of host memory");
+    sc->ctx = ctx;
+    int rc = scene_common_buffers(sc);
+    if (rc != GRT_OK) {
+        grt_scene_destroy(sc);
+        return rc;
+    }
+    ctx->scenes.insert(sc);
+    *out = sc;
+    return GRT_OK;
+}
+
+int grt_scene_destroy(grt_scene sc)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_scene_destroy: NULL scene");
+    grt_context_t* ctx = sc->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevMem* m : sc->all_buffers())
+        m->release(ctx);
+    ctx->scenes.erase(sc);
+    delete sc;
+    return GRT_OK;
+}
+
+static int scene_alloc_geometry(grt_scene_t* sc, uint32_t n_tris)
+{
+    int rc;
+    const size_t b9 = (size_t)n_tris * 9 * sizeof(float);
+    if ((rc = sc->verts.reserve(sc->ctx, b9, false)) != GRT_OK)
+        return rc;
+    if ((rc = sc->normals.reserve(sc->ctx, b9, false)) != GRT_OK)
+        return rc;
+    if ((rc = sc->tri.reserve(sc->ctx, (size_t)n_tris * 48, false)) != GRT_OK)
+        return rc;
+    if ((rc = sc->big_list.reserve(sc->ctx, (size_t)n_tris * 4, false)) != GRT_OK)
+        return rc;
+    return GRT_OK;
+}
+
+int grt_scene_upload_triangles(grt_scene sc, const float* verts_host, uint32_t n_tris)
+{
+    if (!sc || (!verts_host && n_tris))
+        return set_error(GRT_INVALID_PARAM, "grt_scene_upload_triangles: NULL argument");
+    if (n_tris == 0 || n_tris > (1u << 28))
+        return set_error(GRT_INVALID_PARAM, "grt_scene_upload_triangles: n_tris=%u out of range",
+                         n_tris);
+    int rc = scene_alloc_geometry(sc, n_tris);
+    if (rc != GRT_OK)
+        return rc;
+    const size_t b9 = (size_t)n_tris * 9 * sizeof(float);
+    hipError_t e = hipMemcpyAsync(sc->verts.ptr, verts_host, b9, hipMemcpyHostToDevice,
+                                  sc->ctx->stream);
+    if (e != hipSuccess)
+        return hip_check(e, "uploading vertices");
+    if (sc->n_tris != n_tris || !sc->has_normals) {
+        // Until normals are uploaded, use zero normals (secondary rays need them).
+        if ((e = hipMemsetAsync(sc->normals.ptr, 0, b9, sc->ctx->stream)) != hipSuccess)
+            return hip_check(e, "clearing normals");
+        sc->has_normals = false;
+    }
+    sc->n_tris = n_tris;
+    sc->grid_valid = false;
+    sc->rest_valid = false;
+    if ((rc = scene_pack(sc)) != GRT_OK)
+        return rc;
+    return hip_check(hipStreamSynchronize(sc->ctx->stream), "upload");
+}
+
+int grt_scene_upload_normals(grt_scene sc, const float* normals_host, uint32_t n_tris)
+{
+    if (!sc || !normals_host)
+        return set_error(GRT_INVALID_PARAM, "grt_scene_upload_normals: NULL argument");
+    if (n_tris != sc->n_tris)
+        return set_error(GRT_INVALID_PARAM,
+                         "grt_scene_upload_normals: %u normals for %u triangles", n_tris,
+                         sc->n_tris);
+    hipError_t e = hipMemcpyAsync(sc->normals.ptr, normals_host, (size_t)n_tris * 36,
+                                  hipMemcpyHostToDevice, sc->ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(sc->ctx->stream);
+    sc->has_normals = true;
+    sc->rest_valid = false;
+    return hip_check(e, "uploading normals");
+}
+
+int grt_scene_set_triangles_device(grt_scene sc, const float* verts_dev, const float* normals_dev,
+                                   uint32_t n_tris)
+{
+    if (!sc || !verts_dev || n_tris == 0 || n_tris > (1u << 28))
+        return set_error(GRT_INVALID_PARAM, "grt_scene_set_triangles_device: bad argument");
+    int rc = scene_alloc_geometry(sc, n_tris);
+    if (rc != GRT_OK)
+        return rc;
+    const size_t b9 = (size_t)n_tris * 36;
+    hipError_t e = hipMemcpyAsync(sc->verts.ptr, verts_dev, b9, hipMemcpyDeviceToDevice,
+                                  sc->ctx->stream);
+    if (e == hipSuccess) {
+        if (normals_dev)
+            e = hipMemcpyAsync(sc->normals.ptr, normals_dev, b9, hipMemcpyDeviceToDevice,
+                               sc->ctx->stream);
+        else
+            e = hipMemsetAsync(sc->normals.ptr, 0, b9, sc->ctx->stream);
+    }
+    if (e != hipSuccess)
+        return hip_check(e, "device triangle copy");
+    sc->n_tris = n_tris;
+    sc->has_normals = normals_dev != nullptr;
+    sc->grid_valid = false;
+    sc->rest_valid = false;
+    return scene_pack(sc);
+}
+
+int grt_scene_triangle_count(grt_scene sc, uint32_t* n)
+{
+    if (!sc || !n)
+        return set_error(GRT_INVALID_PARAM, "grt_scene_triangle_count: NULL argument");
+    *n = sc->n_tris;
+    return GRT_OK;
+}
+
+int grt_scene_device_pointers(grt_scene sc, float** verts_dev, float** normals_dev)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_scene_device_pointers: NULL scene");
+    if (verts_dev)
+        *verts_dev = sc->verts.as<float>();
+    if (normals_dev)
+        *normals_dev = sc->normals.as<float>();
+    return GRT_OK;
+}
+
+int grt_scene_download_geometry(grt_scene sc, float* verts_host, float* normals_host)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_scene_download_geometry: NULL scene");
+    const size_t b9 = (size_t)sc->n_tris * 36;
+    hipStream_t s = sc->ctx->stream;
+    hipError_t e = hipSuccess;
+    if (verts_host && b9)
+        e = hipMemcpyAsync(verts_host, sc->verts.ptr, b9, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && normals_host && b9)
+        e = hipMemcpyAsync(normals_host, sc->normals.ptr, b9, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    return hip_check(e, "geometry download");
+}
+
+int grt_scene_transform_clusters(grt_scene sc, const float* xf_host, uint32_t n_clusters,
+                                 uint32_t tris_per_cluster, uint32_t n_cluster_tris)
+{
+    if (!sc || !xf_host || tris_per_cluster == 0)
+        return set_error(GRT_INVALID_PARAM, "grt_scene_transform_clusters: bad argument");
+    if (sc->n_tris == 0)
+        return set_error(GRT_NOT_READY, "grt_scene_transform_clusters: empty scene");
+    if (n_cluster_tris > sc->n_tris ||
+        (uint64_t)n_clusters * tris_per_cluster < (uint64_t)n_cluster_tris)
+        return set_error(GRT_INVALID_PARAM,
+                         "grt_scene_transform_clusters: %u clusters x %u do not cover %u triangles",
+                         n_clusters, tris_per_cluster, n_cluster_tris);
+    int rc;
+    hipStream_t s = sc->ctx->stream;
+    const size_t b9 = (size_t)sc->n_tris * 36;
+    if (!sc->rest_valid) {
+        if ((rc = sc->rest_verts.reserve(sc->ctx, b9, false)) != GRT_OK)
+            return rc;
+        if ((rc = sc->rest_normals.reserve(sc->ctx, b9, false)) != GRT_OK)
+            return rc;
+        hipError_t e = hipMemcpyAsync(sc->rest_verts.ptr, sc->verts.ptr, b9,
+                                      hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sc->rest_normals.ptr, sc->normals.ptr, b9, hipMemcpyDeviceToDevice,
+                               s);
+        if (e != hipSuccess)
+            return hip_check(e, "saving the rest pose");
+        sc->rest_valid = true;
+    }
+    const size_t xb = (size_t)n_clusters * 12 * sizeof(float);
+    if ((rc = sc->xf.reserve(sc->ctx, xb, false)) != GRT_OK)
+        return rc;
+    // Stage through pinned memory so the copy is truly asynchronous.
+    if (sc->xf_host_bytes < xb) {
+        if (sc->xf_host)
+            (void)hipHostFree(sc->xf_host);
+        sc->xf_host = nullptr;
+        sc->xf_host_bytes = 0;
+        if (hipHostMalloc(&sc->xf_host, xb, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(GRT_NO_MEMORY, "pinned staging of %zu bytes failed", xb);
+        }
+        sc->xf_host_bytes = xb;
+    }
+    // The previous transform's copy must have left the staging buffer.
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return hip_check(e, "transform staging");
+    std::memcpy(sc->xf_host, xf_host, xb);
+    e = hipMemcpyAsync(sc->xf.ptr, sc->xf_host, xb, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess)
+        return hip_check(e, "uploading transforms");
+    sc->grid_valid = false;
+    return hip_check(launch_transform_pack(sc->n_tris, tris_per_cluster, n_cluster_tris,
+                                           sc->xf.as<float>(), sc->rest_verts.as<float>(),
+                                           sc->rest_normals.as<float>(), sc->verts.as<float>(),
+                                           sc->normals.as<float>(), sc->tri.as<float4>(), s),
+                     "transform");
+}
+
+int grt_grid_build(grt_scene sc, float lambda)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_build: NULL scene");
+    if (sc->n_tris == 0)
+        return set_error(GRT_NOT_READY, "grt_grid_build: empty scene");
+    if (!(lambda > 0.0f) || !std::isfinite(lambda))
+        return set_error(GRT_INVALID_PARAM, "grt_grid_build: lambda must be > 0");
+    hipStream_t s = sc->ctx->stream;
+    int rc;
+    DevFrame host;
+    if ((rc = hip_check(launch_frame_from_verts(sc->verts.as<float>(), sc->n_tris, lambda,
+                                                0xffffffffu, sc->partial.as<float>(),
+                                                sc->frame.as<DevFrame>(), s),
+                        "grid frame")) != GRT_OK)
+        return rc;
+    if ((rc = fetch_frame(sc, &host)) != GRT_OK)
+        return rc;
+    if (host.status & kStatusFrameInvalid)
+        return set_error(GRT_CAPACITY, "grid frame invalid (cells %u x %u x %u)", host.n[0],
+                         host.n[1], host.n[2]);
+    if ((rc = grid_reserve_cells(sc, host.m)) != GRT_OK)
+        return rc;
+    if ((rc = enqueue_build_passes(sc, 0xffffffffu)) != GRT_OK)
+        return rc;
+    if ((rc = fetch_frame(sc, &host)) != GRT_OK)
+        return rc;
+    if (host.status != 0)
+        return set_error(GRT_CAPACITY, "grid build status 0x%x", host.status);
+    if ((rc = grid_reserve_refs(sc, host.r > 0 ? host.r : 1)) != GRT_OK)
+        return rc;
+    if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
+        return rc;
+    if ((rc = fetch_frame(sc, &host)) != GRT_OK)
+        return rc;
+    if (host.status != 0)
+        return set_error(GRT_CAPACITY, "grid build status 0x%x", host.status);
+    sc->info_host = host;
+    sc->lambda = lambda;
+    sc->grid_valid = true;
+    return GRT_OK;
+}
+
+int grt_grid_reserve(grt_scene sc, uint32_t cells, uint32_t refs)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_reserve: NULL scene");
+    int rc;
+    if ((rc = grid_reserve_cells(sc, cells)) != GRT_OK)
+        return rc;
+    return grid_reserve_refs(sc, refs);
+}
+
+int grt_grid_rebuild_async(grt_scene sc, float lambda)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_rebuild_async: NULL scene");
+    if (sc->n_tris == 0)
+        return set_error(GRT_NOT_READY, "grt_grid_rebuild_async: empty scene");
+    if (!sc->counts.ptr || !sc->refs.ptr)
+        return set_error(GRT_NOT_READY,
+                         "grt_grid_rebuild_async: call grt_grid_build or grt_grid_reserve first");
+    if (!(lambda > 0.0f) || !std::isfinite(lambda))
+        return set_error(GRT_INVALID_PARAM, "grt_grid_rebuild_async: lambda must be > 0");
+    hipStream_t s = sc->ctx->stream;
+    int rc;
+    if ((rc = hip_check(launch_frame_from_verts(sc->verts.as<float>(), sc->n_tris, lambda,
+                                                sc->cells_cap, sc->partial.as<float>(),
+                                                sc->frame.as<DevFrame>(), s),
+                        "grid frame")) != GRT_OK)
+        return rc;
+    if ((rc = enqueue_build_passes(sc, sc->refs_cap)) != GRT_OK)
+        return rc;
+    if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
+        return rc;
+    sc->lambda = lambda;
+    sc->grid_valid = true;
+    sc->info_stale = true;
+    return GRT_OK;
+}
+
+int grt_grid_check(grt_scene sc)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_check: NULL scene");
+    if (!sc->grid_valid)
+        return set_error(GRT_NOT_READY, "grt_grid_check: no grid");
+    DevFrame host;
+    int rc = fetch_frame(sc, &host);
+    if (rc != GRT_OK)
+        return rc;
+    sc->info_host = host;
+    sc->info_stale = false;
+    if (host.status & kStatusRefsOver) {
+        // The scatter did not run, so the counts were not returned to zero.
+        (void)launch_fill_u32(sc->counts.as<uint32_t>(), 0u, sc->cells_cap, sc->ctx->stream);
+        (void)hipStreamSynchronize(sc->ctx->stream);
+    }
+    if (host.status != 0) {
+        sc->grid_valid = false;
+        return set_error(GRT_CAPACITY, "grid rebuild exceeded capacity (status 0x%x)",
+                         host.status);
+    }
+    return GRT_OK;
+}
+
+int grt_grid_upload(grt_scene sc, const grt_grid_info* info, const uint32_t* offsets_host,
+                    const uint32_t* refs_host)
+{
+    if (!sc || !info || !offsets_host)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_upload: NULL argument");
+    uint64_t m = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (info->n[d] < 1)
+            return set_error(GRT_INVALID_PARAM, "grt_grid_upload: n[%d]=%d", d, info->n[d]);
+        m *= (uint64_t)info->n[d];
+    }
+    if (m > (1ull << 30))
+        return set_error(GRT_INVALID_PARAM, "grt_grid_upload: too many cells");
+    const uint32_t r = offsets_host[m];
+    if (r && !refs_host)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_upload: refs is NULL");
+    if (offsets_host[0] != 0)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_upload: offsets[0] must be 0");
+    for (uint64_t c = 0; c < m; ++c)
+        if (offsets_host[c + 1] < offsets_host[c])
+            return set_error(GRT_INVALID_PARAM, "grt_grid_upload: offsets not monotone at %llu",
+                             (unsigned long long)c);
+    for (uint32_t k = 0; k < r; ++k)
+        if (refs_host[k] >= sc->n_tris)
+            return set_error(GRT_INVALID_PARAM, "grt_grid_upload: ref %u = %u >= %u triangles", k,
+                             refs_host[k], sc->n_tris);
+    int rc;
+    if ((rc = grid_reserve_cells(sc, (uint32_t)m)) != GRT_OK)
+        return rc;
+    if ((rc = grid_reserve_refs(sc, r > 0 ? r : 1)) != GRT_OK)
+        return rc;
+    DevFrame f;
+    std::memset(&f, 0, sizeof(f));
+    for (int d = 0; d < 3; ++d) {
+        f.bmin[d] = info->bmin[d];
+        f.bmax[d] = info->bmax[d];
+        f.cell[d] = info->cell[d];
+        f.inv[d] = info->inv_cell[d];
+        f.n[d] = info->n[d];
+    }
+    f.m = (uint32_t)m;
+    f.r = r;
+    hipStream_t s = sc->ctx->stream;
+    hipError_t e = hipMemcpyAsync(sc->frame.ptr, &f, sizeof(f), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sc->offsets.ptr, offsets_host, (m + 1) * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && r)
+        e = hipMemcpyAsync(sc->refs.ptr, refs_host, (size_t)r * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return hip_check(e, "grid upload");
+    sc->info_host = f;
+    sc->grid_valid = true;
+    sc->info_stale = false;
+    return GRT_OK;
+}
+
+int grt_grid_info_get(grt_scene sc, grt_grid_info* out)
+{
+    if (!sc || !out)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_info_get: NULL argument");
+    if (!sc->grid_valid)
+        return set_error(GRT_NOT_READY, "grt_grid_info_get: no grid");
+    if (sc->info_stale) {
+        int rc = grt_grid_check(sc);
+        if (rc != GRT_OK)
+            return rc;
+    }
+    frame_to_info(sc->info_host, out);
+    return GRT_OK;
+}
+
+int grt_grid_download(grt_scene sc, uint32_t* offsets_host, uint32_t* refs_host)
+{
+    grt_grid_info info;
+    int rc = grt_grid_info_get(sc, &info);
+    if (rc != GRT_OK)
+        return rc;
+    hipStream_t s = sc->ctx->stream;
+    hipError_t e = hipSuccess;
+    if (offsets_host)
+        e = hipMemcpyAsync(offsets_host, sc->offsets.ptr, ((size_t)info.m + 1) * 4,
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && refs_host && info.r)
+        e = hipMemcpyAsync(refs_host, sc->refs.ptr, (size_t)info.r * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    return hip_check(e, "grid download");
+}
+
+static bool rays_ok(const grt_rays* r)
+{
+    return r && r->ox && r->oy && r->oz && r->dx && r->dy && r->dz && r->tmin && r->tmax;
+}
+static bool hits_ok(const grt_hits* h) { return h && h->id && h->t && h->u && h->v; }
+
+static RaySoA to_soa(const grt_rays* r)
+{
+    return RaySoA{r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->tmin, r->tmax};
+}
+static RaySoAOut to_soa_out(const grt_rays* r)
+{
+    return RaySoAOut{r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->tmin, r->tmax};
+}
+static HitSoAIn to_hits_in(const grt_hits* h, uint32_t n_tris)
+{
+    return HitSoAIn{h->id, h->t, h->u, h->v, n_tris};
+}
+
+static int trace_ready(grt_scene sc, const char* who)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "%s: NULL scene", who);
+    if (sc->n_tris == 0 || !sc->grid_valid)
+        return set_error(GRT_NOT_READY, "%s: scene has no grid", who);
+    return GRT_OK;
+}
+
+static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint32_t* counters)
+{
+    TraceArgs a;
+    a.fr = sc->frame.as<DevFrame>();
+    a.offsets = sc->offsets.as<uint32_t>();
+    a.refs = sc->refs.as<uint32_t>();
+    a.tri = sc->tri.as<float4>();
+    a.n = n;
+    a.image_width = image_width;
+    a.counters = counters;
+    return a;
+}
+
+int grt_trace_closest(grt_scene sc, const grt_rays* rays, const grt_hits* hits, uint32_t n,
+                      uint32_t image_width, uint32_t* counters_dev)
+{
+    int rc = trace_ready(sc, "grt_trace_closest");
+    if (rc != GRT_OK)
+        return rc;
+    if (!rays_ok(rays) || !hits_ok(hits))
+        return set_error(GRT_INVALID_PARAM, "grt_trace_closest: NULL ray or hit buffer");
+    const TraceArgs a = trace_args(sc, n, image_width, counters_dev);
+    return hip_check(launch_trace_closest(a, to_soa(rays),
+                                          HitSoA{hits->id, hits->t, hits->u, hits->v},
+                                          sc->ctx->block_closest, sc->ctx->stream),
+                     "trace_closest");
+}
+
+int grt_trace_any(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev, uint32_t n,
+                  uint32_t image_width, uint32_t* counters_dev)
+{
+    int rc = trace_ready(sc, "grt_trace_any");
+    if (rc != GRT_OK)
+        return rc;
+    if (!rays_ok(rays) || !occluded_dev)
+        return set_error(GRT_INVALID_PARAM, "grt_trace_any: NULL ray or output buffer");
+    const TraceArgs a = trace_args(sc, n, image_width, counters_dev);
+    return hip_check(launch_trace_any(a, to_soa(rays), occluded_dev, sc->ctx->block_any,
+                                      sc->ctx->stream),
+                     "trace_any");
+}
+
+int grt_camera_lookat(const float eye[3], const float center[3], const float up[3],
+                      float fovy_deg, float znear, uint32_t width, uint32_t height,
+                      grt_camera* out)
+{
+    if (!eye || !center || !up || !out || width == 0 || height == 0)
+        return set_error(GRT_INVALID_PARAM, "grt_camera_lookat: bad argument");
+    camera_lookat(eye, center, up, fovy_deg, znear, width, height, out);
+    return GRT_OK;
+}
+
+int grt_gen_primary(grt_context ctx, const grt_camera* cam, const grt_rays* out)
+{
+    if (!ctx || !cam || !rays_ok(out))
+        return set_error(GRT_INVALID_PARAM, "grt_gen_primary: bad argument");
+    if ((uint64_t)cam->width * cam->height > 0xffffffffull)
+        return set_error(GRT_INVALID_PARAM, "grt_gen_primary: image too large");
+    DevCamera c;
+    std::memcpy(c.pos, cam->pos, sizeof(c.pos));
+    std::memcpy(c.base, cam->base, sizeof(c.base));
+    std::memcpy(c.nu, cam->nu, sizeof(c.nu));
+    std::memcpy(c.nv, cam->nv, sizeof(c.nv));
+    c.width = cam->width;
+    c.height = cam->height;
+    return hip_check(launch_gen_primary(c, to_soa_out(out), ctx->stream), "gen_primary");
+}
+
+int grt_gen_shadow(grt_scene sc, const grt_rays* in, const grt_hits* hits, uint32_t n,
+                   const float light[3], float offset, const grt_rays* out)
+{
+    if (!sc || !rays_ok(in) || !hits_ok(hits) || !rays_ok(out) || !light)
+        return set_error(GRT_INVALID_PARAM, "grt_gen_shadow: bad argument");
+    if (sc->n_tris == 0)
+        return set_error(GRT_NOT_READY, "grt_gen_shadow: empty scene");
+    return hip_check(launch_gen_shadow(n, to_soa(in), to_hits_in(hits, sc->n_tris), sc->normals.as<float>(),
+                                       light[0], light[1], light[2], offset, to_soa_out(out),
+                                       sc->ctx->stream),
+                     "gen_shadow");
+}
+
+int grt_gen_reflection(grt_scene sc, const grt_rays* in, const grt_hits* hits, uint32_t n,
+                       float offset, const grt_rays* out)
+{
+    if (!sc || !rays_ok(in) || !hits_ok(hits) || !rays_ok(out))
+        return set_error(GRT_INVALID_PARAM, "grt_gen_reflection: bad argument");
+    if (sc->n_tris == 0)
+        return set_error(GRT_NOT_READY, "grt_gen_reflection: empty scene");
+    return hip_check(launch_gen_reflection(n, to_soa(in), to_hits_in(hits, sc->n_tris),
+                                           sc->normals.as<float>(), offset, to_soa_out(out),
+                                           sc->ctx->stream),
+                     "gen_reflection");
+}
+
+int grt_gen_ao(grt_scene sc, const grt_rays* in, const grt_hits* hits, uint32_t n, uint32_t spp,
+               uint64_t seed, float offset, float ao_tmax, const grt_rays* out)
+{
+    if (!sc || !rays_ok(in) || !hits_ok(hits) || !rays_ok(out) || spp == 0)
+        return set_error(GRT_INVALID_PARAM, "grt_gen_ao: bad argument");
+    if ((uint64_t)n * spp > 0xffffffffull)
+        return set_error(GRT_INVALID_PARAM, "grt_gen_ao: too many rays");
+    if (sc->n_tris == 0)
+        return set_error(GRT_NOT_READY, "grt_gen_ao: empty scene");
+    return hip_check(launch_gen_ao(n, spp, seed, to_soa(in), to_hits_in(hits, sc->n_tris),
+                                   sc->normals.as<float>(), offset, ao_tmax, to_soa_out(out),
+                                   sc->ctx->stream),
+                     "gen_ao");
+}
+
+int grt_shade_headlight(grt_scene sc, const grt_rays* rays, const grt_hits* hits, uint32_t n,
+                        uint32_t* rgba_dev)
+{
+    if (!sc || !rays_ok(rays) || !hits_ok(hits) || !rgba_dev)
+        return set_error(GRT_INVALID_PARAM, "grt_shade_headlight: bad argument");
+    return hip_check(launch_shade_headlight(n, to_soa(rays), to_hits_in(hits, sc->n_tris),
+                                            sc->normals.as<float>(), rgba_dev, sc->ctx->stream),
+                     "shade_headlight");
+}
+
+int grt_debug_scan(grt_context ctx, const uint32_t* in_dev, uint32_t* out_dev, uint32_t n)
+{
+    if (!ctx || !in_dev || !out_dev || n == 0)
+        return set_error(GRT_INVALID_PARAM, "grt_debug_scan: bad argument");
+    DevMem bs;
+    int rc = bs.reserve(ctx, ((size_t)n / kScanTile + 2) * 4, false);
+    if (rc != GRT_OK)
+        return rc;
+    hipError_t e = launch_scan_u32(in_dev, out_dev, n, bs.as<uint32_t>(), nullptr, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    bs.release(ctx);
+    return hip_check(e, "debug scan");
+}
+
+}  // extern "C"

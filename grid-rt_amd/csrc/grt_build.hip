@@ -1,0 +1,820 @@
+// grt_build.hip -- per-frame uniform-grid construction on gfx950.
+//
+// Pipeline (one stream, no host round trip needed once buffers are sized):
+//   1. k_bbox_partial + k_frame : scene box -> padded grid frame
+//        (Aabb.cpp:11-91, UniformGridAccStructBuilder.cpp:73-102,
+//         UniformGridAccStruct.cpp:44-63)
+//   2. k_grid_pass<false>       : per-cell reference counts
+//        (UniformGridAccStructBuilder.cpp:108-145; countCellVP.glsl:31-58)
+//   3. k_scan_*                 : exclusive scan of the counts (DPP wave scan,
+//        LDS block scan, reduce-then-scan across blocks)
+//   4. k_grid_pass<true>        : scatter triangle ids into their cells
+//        (gridVP_sem_GS.glsl:56-106)
+//   5. k_sort_small/large       : every cell's ids ascending, the order the
+//        reference's push_back loop produces (UniformGridAccStructBuilder.cpp:141)
+//
+// Triangles spanning more than kSmallCells cells (ground planes) are put on a
+// list and expanded by whole blocks, so one thread never loops over 10^5 cells.
+#include "grt_launch.h"
+
+namespace grt {
+
+constexpr uint32_t kSmallCells = 16;
+constexpr uint32_t kSortTile = 16384;   // LDS tile of the per-cell sort (64 KB)
+
+// ---------------------------------------------------------------------------
+// Wave64 / block primitives
+// ---------------------------------------------------------------------------
+
+// Inclusive prefix sum over a full wave64 with DPP: row_shr 1,2,4,8 inside each
+// row of 16 lanes, then row_bcast15 (rows 1,3) and row_bcast31 (rows 2,3).
+// All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
+// Exclusive block scan (BLOCK threads, BLOCK/64 waves); lds needs BLOCK/64+1
+// words.  Returns the exclusive prefix; `total` receives the block sum.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total)
+{
+    constexpr int NW = BLOCK / 64;
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63)
+        lds[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        const uint32_t w = lane < NW ? lds[lane] : 0u;
+        const uint32_t wi = wave_incl_scan(w);
+        if (lane < NW)
+            lds[lane] = wi - w;
+        if (lane == NW - 1)
+            lds[NW] = wi;
+    }
+    __syncthreads();
+    const uint32_t res = inc - v + lds[wid];
+    total = lds[NW];
+    __syncthreads();
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// 1. Frame
+// ---------------------------------------------------------------------------
+
+// Cube root of the specification: binary64 Newton iteration with basic IEEE
+// operations only, so host and device agree bit for bit (the reference's
+// powf(x, 1.0f/3.0f), UniformGridAccStructBuilder.cpp:93, is not reproducible
+// across math libraries).
+__device__ float spec_cbrtf(float xf)
+{
+    double x = (double)xf;
+    if (!(x > 0.0) || __builtin_isinf(x))
+        return xf;
+    int e;
+    double m = frexp(x, &e);
+    const int q = (e >= 0) ? (e / 3) : -((-e + 2) / 3);
+    const int r = e - 3 * q;
+    m = ldexp(m, r);
+    double y = 1.0;
+#pragma unroll 1
+    for (int i = 0; i < 8; ++i)
+        y = (2.0 * y + m / (y * y)) / 3.0;
+    return (float)ldexp(y, q);
+}
+
+// Aabb::scaleBy(0.01f) + resolution heuristic + setResolution.
+__device__ void frame_from_box(const float mn[3], const float mx[3], uint32_t n_tris,
+                               float lambda, uint32_t cells_cap, DevFrame* fr)
+{
+    uint32_t status = 0;
+    float diag[3];
+    for (int d = 0; d < 3; ++d) {
+        const float s = (mn[d] == mx[d]) ? 0.01f : fabsf(mx[d] - mn[d]) * 0.01f;
+        fr->bmin[d] = mn[d] - s;
+        fr->bmax[d] = mx[d] + s;
+        diag[d] = fr->bmax[d] - fr->bmin[d];
+    }
+    const float vol = (diag[0] * diag[1]) * diag[2];
+    const float x = (lambda * (float)n_tris) / vol;
+    const float fac = spec_cbrtf(x);
+    uint64_t m = 1;
+    for (int d = 0; d < 3; ++d) {
+        const float c = ceilf(diag[d] * fac);
+        int32_t n = 1;
+        if (!(c < 1048576.0f))
+            status |= kStatusFrameInvalid;
+        else
+            n = (int32_t)c;
+        if (n < 1)
+            n = 1;
+        fr->n[d] = n;
+        m *= (uint64_t)n;
+    }
+    if (n_tris == 0 || !(lambda > 0.0f) || m > (1ull << 30))
+        status |= kStatusFrameInvalid;
+    if (m > (uint64_t)cells_cap)
+        status |= kStatusCellsOver;
+    fr->m = (uint32_t)(m > 0xffffffffull ? 0xffffffffull : m);
+    for (int d = 0; d < 3; ++d) {
+        fr->cell[d] = diag[d] / (float)fr->n[d];
+        fr->inv[d] = 1.0f / fr->cell[d];
+    }
+    fr->r = 0;
+    fr->status = status;
+}
+
+__global__ __launch_bounds__(256) void k_bbox_partial(const float* __restrict__ verts,
+                                                      uint32_t n_tris, float* __restrict__ partial)
+{
+    __shared__ float lds[6][4];
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_tris;
+         t += gridDim.x * blockDim.x) {
+        const float* v = verts + 9ull * t;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float x = v[3 * k + d];
+                mn[d] = x < mn[d] ? x : mn[d];
+                mx[d] = x > mx[d] ? x : mx[d];
+            }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float a = __shfl_xor(mn[d], off);
+            const float b = __shfl_xor(mx[d], off);
+            mn[d] = a < mn[d] ? a : mn[d];
+            mx[d] = b > mx[d] ? b : mx[d];
+        }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) {
+            lds[d][wid] = mn[d];
+            lds[3 + d][wid] = mx[d];
+        }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        float r = lds[d][0];
+        for (int w = 1; w < 4; ++w) {
+            const float x = lds[d][w];
+            r = d < 3 ? (x < r ? x : r) : (x > r ? x : r);
+        }
+        partial[6 * blockIdx.x + d] = r;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_frame(const float* __restrict__ partial, uint32_t nparts,
+                                              uint32_t n_tris, float lambda, uint32_t cells_cap,
+                                              DevFrame* __restrict__ fr)
+{
+    if (threadIdx.x != 0)
+        return;
+    float mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = partial[d];
+        mx[d] = partial[3 + d];
+    }
+    for (uint32_t p = 1; p < nparts; ++p)
+        for (int d = 0; d < 3; ++d) {
+            const float a = partial[6 * p + d], b = partial[6 * p + 3 + d];
+            mn[d] = a < mn[d] ? a : mn[d];
+            mx[d] = b > mx[d] ? b : mx[d];
+        }
+    frame_from_box(mn, mx, n_tris, lambda, cells_cap, fr);
+}
+
+__global__ void k_frame_box(float x0, float y0, float z0, float x1, float y1, float z1,
+                            uint32_t n_tris, float lambda, uint32_t cells_cap, DevFrame* fr)
+{
+    if (threadIdx.x != 0)
+        return;
+    const float mn[3] = {x0, y0, z0}, mx[3] = {x1, y1, z1};
+    frame_from_box(mn, mx, n_tris, lambda, cells_cap, fr);
+}
+
+// ---------------------------------------------------------------------------
+// 2/4. Count and scatter passes
+// ---------------------------------------------------------------------------
+struct CellBox {
+    int32_t lo[3];
+    int32_t w[3];
+};
+
+__device__ __forceinline__ CellBox tri_cells(const DevFrame* fr, const float* v)
+{
+    CellBox b;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float a = v[d], c1 = v[3 + d], c2 = v[6 + d];
+        float mn = a, mx = a;
+        mn = c1 < mn ? c1 : mn;
+        mx = c1 > mx ? c1 : mx;
+        mn = c2 < mn ? c2 : mn;
+        mx = c2 > mx ? c2 : mx;
+        const int32_t lo = vox(mn, fr->bmin[d], fr->inv[d], fr->n[d]);
+        const int32_t hi = vox(mx, fr->bmin[d], fr->inv[d], fr->n[d]);
+        b.lo[d] = lo;
+        b.w[d] = hi - lo + 1;
+    }
+    return b;
+}
+
+template <bool SCATTER>
+__device__ __forceinline__ void cell_op(uint32_t c, uint32_t t, uint32_t* counts,
+                                        const uint32_t* offsets, uint32_t* refs)
+{
+    if (SCATTER) {
+        const uint32_t slot = atomicSub(&counts[c], 1u) - 1u;
+        refs[offsets[c] + slot] = t;
+    } else {
+        atomicAdd(&counts[c], 1u);
+    }
+}
+
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ fr,
+                                                   const float* __restrict__ verts, uint32_t n_tris,
+                                                   uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   uint32_t* __restrict__ refs,
+                                                   uint32_t* __restrict__ big_list,
+                                                   uint32_t* __restrict__ big_count)
+{
+    const uint32_t bad = SCATTER ? (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)
+                                 : (kStatusFrameInvalid | kStatusCellsOver);
+    if (fr->status & bad)
+        return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tris)
+        return;
+    const CellBox b = tri_cells(fr, verts + 9ull * t);
+    const uint32_t k = (uint32_t)b.w[0] * (uint32_t)b.w[1] * (uint32_t)b.w[2];
+    if (k > kSmallCells) {
+        if (!SCATTER)
+            big_list[atomicAdd(big_count, 1u)] = t;
+        return;
+    }
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    for (int32_t z = 0; z < b.w[2]; ++z)
+        for (int32_t y = 0; y < b.w[1]; ++y) {
+            const uint32_t row = (uint32_t)b.lo[0] +
+                                 nx * ((uint32_t)(b.lo[1] + y) + ny * (uint32_t)(b.lo[2] + z));
+            for (int32_t x = 0; x < b.w[0]; ++x)
+                cell_op<SCATTER>(row + (uint32_t)x, t, counts, offsets, refs);
+        }
+}
+
+// Triangles on the big list: one block per triangle, threads stride its cells.
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void k_grid_pass_big(const DevFrame* __restrict__ fr,
+                                                       const float* __restrict__ verts,
+                                                       uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ offsets,
+                                                       uint32_t* __restrict__ refs,
+                                                       const uint32_t* __restrict__ big_list,
+                                                       const uint32_t* __restrict__ big_count)
+{
+    const uint32_t bad = SCATTER ? (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)
+                                 : (kStatusFrameInvalid | kStatusCellsOver);
+    if (fr->status & bad)
+        return;
+    const uint32_t nb = *big_count;
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    for (uint32_t i = blockIdx.x; i < nb; i += gridDim.x) {
+        const uint32_t t = big_list[i];
+        const CellBox b = tri_cells(fr, verts + 9ull * t);
+        const uint32_t wx = (uint32_t)b.w[0], wy = (uint32_t)b.w[1];
+        const uint32_t k = wx * wy * (uint32_t)b.w[2];
+        for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
+            const uint32_t x = j % wx;
+            const uint32_t yz = j / wx;
+            const uint32_t y = yz % wy;
+            const uint32_t z = yz / wy;
+            const uint32_t c = ((uint32_t)b.lo[0] + x) +
+                               nx * (((uint32_t)b.lo[1] + y) + ny * ((uint32_t)b.lo[2] + z));
+            cell_op<SCATTER>(c, t, counts, offsets, refs);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. Exclusive scan (reduce -> scan block sums -> apply)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool scan_len(const DevFrame* fr, uint32_t n_host, uint32_t& n)
+{
+    if (fr) {
+        if (fr->status & (kStatusFrameInvalid | kStatusCellsOver))
+            return false;
+        n = fr->m;
+    } else {
+        n = n_host;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void load16(const uint32_t* __restrict__ in, uint32_t i0, uint32_t n,
+                                       uint32_t v[16])
+{
+    if (i0 + 16 <= n) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + i0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 w = p[q];
+            v[4 * q] = w.x;
+            v[4 * q + 1] = w.y;
+            v[4 * q + 2] = w.z;
+            v[4 * q + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            v[k] = (i0 + k < n) ? in[i0 + k] : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in,
+                                                     uint32_t n_host, const DevFrame* fr,
+                                                     uint32_t* __restrict__ bsum)
+{
+    __shared__ uint32_t lds[8];
+    uint32_t n;
+    if (!scan_len(fr, n_host, n))
+        return;
+    const uint32_t base = blockIdx.x * kScanTile;
+    if (base >= n)
+        return;
+    uint32_t v[16];
+    load16(in, base + threadIdx.x * 16u, n, v);
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        s += v[k];
+    uint32_t total;
+    block_excl_scan<256>(s, lds, total);
+    if (threadIdx.x == 0)
+        bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* __restrict__ bsum, uint32_t n_host,
+                                                      DevFrame* fr, uint32_t* __restrict__ out,
+                                                      uint32_t refs_cap)
+{
+    __shared__ uint32_t lds[17];
+    uint32_t n;
+    if (!scan_len(fr, n_host, n))
+        return;
+    const uint32_t nb = (n + kScanTile - 1) / kScanTile;
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<1024>(v, lds, tot);
+        if (i < nb)
+            bsum[i] = (uint32_t)carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        out[n] = (uint32_t)carry;
+        if (fr) {
+            fr->r = (uint32_t)carry;
+            if (carry > (uint64_t)refs_cap)
+                fr->status |= kStatusRefsOver;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n_host,
+                                                    const DevFrame* fr,
+                                                    const uint32_t* __restrict__ bsum,
+                                                    uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t lds[8];
+    uint32_t n;
+    if (!scan_len(fr, n_host, n))
+        return;
+    const uint32_t base = blockIdx.x * kScanTile;
+    if (base >= n)
+        return;
+    const uint32_t i0 = base + threadIdx.x * 16u;
+    uint32_t v[16];
+    load16(in, i0, n, v);
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        s += v[k];
+    uint32_t total;
+    uint32_t p = block_excl_scan<256>(s, lds, total) + bsum[blockIdx.x];
+    uint32_t o[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        o[k] = p;
+        p += v[k];
+    }
+    if (i0 + 16 <= n) {
+        uint4* q = reinterpret_cast<uint4*>(out + i0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            q[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (i0 + k < n)
+                out[i0 + k] = o[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 5. Per-cell sort
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
+{
+    const uint32_t lo = a < b ? a : b;
+    const uint32_t hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+// Cells of 2..8 references: Batcher odd-even merge network in registers.
+// Larger cells go to `list` for k_sort_large.
+__global__ __launch_bounds__(256) void k_sort_small(const DevFrame* __restrict__ fr,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ refs,
+                                                    uint32_t* __restrict__ list, uint32_t list_cap,
+                                                    uint32_t* __restrict__ list_count)
+{
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= fr->m)
+        return;
+    const uint32_t b = offsets[c];
+    const uint32_t cnt = offsets[c + 1] - b;
+    if (cnt <= 1)
+        return;
+    if (cnt > kSortSmallCells) {
+        const uint32_t slot = atomicAdd(list_count, 1u);
+        if (slot < list_cap)
+            list[slot] = c;
+        return;
+    }
+    uint32_t a[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+        a[k] = k < cnt ? refs[b + k] : 0xffffffffu;
+    bool sorted = true;
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+        sorted = sorted && (a[k - 1] <= a[k]);
+    if (sorted)
+        return;
+    cswap(a[0], a[1]); cswap(a[2], a[3]); cswap(a[4], a[5]); cswap(a[6], a[7]);
+    cswap(a[0], a[2]); cswap(a[1], a[3]); cswap(a[4], a[6]); cswap(a[5], a[7]);
+    cswap(a[1], a[2]); cswap(a[5], a[6]);
+    cswap(a[0], a[4]); cswap(a[1], a[5]); cswap(a[2], a[6]); cswap(a[3], a[7]);
+    cswap(a[2], a[4]); cswap(a[3], a[5]);
+    cswap(a[1], a[2]); cswap(a[3], a[4]); cswap(a[5], a[6]);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+        if (k < cnt)
+            refs[b + k] = a[k];
+}
+
+// Bitonic sort written with ascending comparators only (the first comparator
+// of each merge mirrors the block: i <-> i^(k-1)), so the virtual padding up to
+// a power of two is +inf at the top and never needs storage: a comparator whose
+// upper index is >= n is a no-op.
+__device__ __forceinline__ void cex(uint32_t* a, uint32_t i, uint32_t j)
+{
+    const uint32_t x = a[i], y = a[j];
+    if (y < x) {
+        a[i] = y;
+        a[j] = x;
+    }
+}
+
+// Full sort of lds[0..n), n <= kSortTile, by the calling block.
+__device__ void lds_sort(uint32_t* lds, uint32_t n)
+{
+    uint32_t p = 2;
+    while (p < n)
+        p <<= 1;
+    for (uint32_t k = 2; k <= p; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < (p >> 1); i += blockDim.x) {
+                const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const uint32_t hi = (j == (k >> 1)) ? (lo ^ (k - 1)) : (lo | j);
+                const uint32_t a = lo < hi ? lo : hi, b = lo < hi ? hi : lo;
+                if (b < n)
+                    cex(lds, a, b);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Half-cleaners j = jmax .. 1 on lds[0..len) where the tile starts at global
+// index `g0` of a cell of n elements (comparators mirror nothing here).
+__device__ void lds_half_cleaners(uint32_t* lds, uint32_t len, uint32_t jmax)
+{
+    for (uint32_t j = jmax; j > 0; j >>= 1) {
+        for (uint32_t i = threadIdx.x; i < (kSortTile >> 1); i += blockDim.x) {
+            const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+            const uint32_t hi = lo | j;
+            if (hi < len)
+                cex(lds, lo, hi);
+        }
+        __syncthreads();
+    }
+}
+
+// Global comparator pass over refs[b..b+n) by one block.
+__device__ void global_pass(uint32_t* __restrict__ a, uint32_t n, uint32_t p, uint32_t k,
+                            uint32_t j)
+{
+    for (uint32_t i = threadIdx.x; i < (p >> 1); i += blockDim.x) {
+        const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+        const uint32_t hi = (j == (k >> 1)) ? (lo ^ (k - 1)) : (lo | j);
+        const uint32_t x = lo < hi ? lo : hi, y = lo < hi ? hi : lo;
+        if (y < n)
+            cex(a, x, y);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__ fr,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ refs,
+                                                    const uint32_t* __restrict__ list,
+                                                    uint32_t list_cap,
+                                                    const uint32_t* __restrict__ list_count)
+{
+    __shared__ uint32_t lds[kSortTile];
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    uint32_t cnt = *list_count;
+    cnt = cnt < list_cap ? cnt : list_cap;
+    for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const uint32_t c = list[li];
+        const uint32_t b = offsets[c];
+        const uint32_t n = offsets[c + 1] - b;
+        uint32_t* seg = refs + b;
+        if (n <= kSortTile) {
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+                lds[i] = seg[i];
+            __syncthreads();
+            lds_sort(lds, n);
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+                seg[i] = lds[i];
+            __syncthreads();
+            continue;
+        }
+        // n > kSortTile: sort each tile in LDS, then merge stages with
+        // k > kSortTile run their j >= kSortTile comparators in global memory
+        // and the rest tile by tile in LDS.
+        uint32_t p = kSortTile;
+        while (p < n)
+            p <<= 1;
+        for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
+            const uint32_t len = (n - t0) < kSortTile ? (n - t0) : kSortTile;
+            for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
+                lds[i] = seg[t0 + i];
+            __syncthreads();
+            lds_sort(lds, len);
+            for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
+                seg[t0 + i] = lds[i];
+            __syncthreads();
+        }
+        for (uint32_t k = 2 * kSortTile; k <= p; k <<= 1) {
+            for (uint32_t j = k >> 1; j >= kSortTile; j >>= 1)
+                global_pass(seg, n, p, k, j);
+            for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
+                const uint32_t len = (n - t0) < kSortTile ? (n - t0) : kSortTile;
+                for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
+                    lds[i] = seg[t0 + i];
+                __syncthreads();
+                lds_half_cleaners(lds, len, kSortTile >> 1);
+                for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
+                    seg[t0 + i] = lds[i];
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Triangle packing and per-frame rigid transform
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void pack_one(uint32_t t, const float v[9], float4* __restrict__ tri)
+{
+    const float e1x = v[3] - v[0], e1y = v[4] - v[1], e1z = v[5] - v[2];
+    const float e2x = v[6] - v[0], e2y = v[7] - v[1], e2z = v[8] - v[2];
+    tri[3ull * t + 0] = make_float4(v[0], v[1], v[2], e1x);
+    tri[3ull * t + 1] = make_float4(e1y, e1z, e2x, e2y);
+    tri[3ull * t + 2] = make_float4(e2z, __uint_as_float(t), 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ verts, uint32_t n_tris,
+                                              float4* __restrict__ tri)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tris)
+        return;
+    float v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        v[k] = verts[9ull * t + k];
+    pack_one(t, v, tri);
+}
+
+__global__ __launch_bounds__(256) void k_transform_pack(uint32_t n_tris, uint32_t tpc, uint32_t nct,
+                                                        const float* __restrict__ xf,
+                                                        const float* __restrict__ rest_v,
+                                                        const float* __restrict__ rest_n,
+                                                        float* __restrict__ verts,
+                                                        float* __restrict__ normals,
+                                                        float4* __restrict__ tri)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tris)
+        return;
+    float v[9], nn[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        v[k] = rest_v[9ull * t + k];
+        nn[k] = rest_n[9ull * t + k];
+    }
+    if (t < nct) {
+        const float* m = xf + 12ull * (t / tpc);
+        float mm[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            mm[k] = m[k];
+        float ov[9], on[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                ov[3 * k + r] = __builtin_fmaf(mm[4 * r], v[3 * k],
+                                __builtin_fmaf(mm[4 * r + 1], v[3 * k + 1],
+                                __builtin_fmaf(mm[4 * r + 2], v[3 * k + 2], mm[4 * r + 3])));
+                on[3 * k + r] = __builtin_fmaf(mm[4 * r], nn[3 * k],
+                                __builtin_fmaf(mm[4 * r + 1], nn[3 * k + 1],
+                                mm[4 * r + 2] * nn[3 * k + 2]));
+            }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            v[k] = ov[k];
+            nn[k] = on[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        verts[9ull * t + k] = v[k];
+        normals[9ull * t + k] = nn[k];
+    }
+    pack_one(t, v, tri);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 0x7fffffffu)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g < 1)
+        g = 1;
+    return (uint32_t)(g < cap ? g : cap);
+}
+
+hipError_t launch_frame_from_verts(const float* verts, uint32_t n_tris, float lambda,
+                                   uint32_t cells_cap, float* partial, DevFrame* fr,
+                                   hipStream_t s)
+{
+    const uint32_t nb = grid_for(n_tris, 256, kBBoxMaxBlocks);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, verts, n_tris, partial);
+    hipLaunchKernelGGL(k_frame, dim3(1), dim3(64), 0, s, (const float*)partial, nb, n_tris, lambda,
+                       cells_cap, fr);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame_from_box(const float bmin[3], const float bmax[3], uint32_t n_tris,
+                                 float lambda, uint32_t cells_cap, DevFrame* fr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_frame_box, dim3(1), dim3(64), 0, s, bmin[0], bmin[1], bmin[2], bmax[0],
+                       bmax[1], bmax[2], n_tris, lambda, cells_cap, fr);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_tris,
+                             uint32_t* counts, uint32_t* big_list, uint32_t* big_count,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, big_count, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(k_grid_pass<false>, dim3(grid_for(n_tris, 256)), dim3(256), 0, s, fr, verts,
+                       n_tris, counts, (const uint32_t*)nullptr, (uint32_t*)nullptr, big_list,
+                       big_count);
+    hipLaunchKernelGGL(k_grid_pass_big<false>, dim3(2048), dim3(256), 0, s, fr, verts, counts,
+                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)big_list,
+                       (const uint32_t*)big_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
+                            uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
+                            hipStream_t s)
+{
+    const uint32_t nb = grid_for(cells_cap, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
+                       block_sums);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, 0u, fr, offsets,
+                       refs_cap);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
+                       (const uint32_t*)block_sums, offsets);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* block_sums,
+                           DevFrame* tmp_frame, hipStream_t s)
+{
+    (void)tmp_frame;
+    const uint32_t nb = grid_for(n, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, (const DevFrame*)nullptr,
+                       block_sums);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, n, (DevFrame*)nullptr,
+                       out, 0xffffffffu);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n, (const DevFrame*)nullptr,
+                       (const uint32_t*)block_sums, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t n_tris,
+                               uint32_t* counts, const uint32_t* offsets, uint32_t* refs,
+                               const uint32_t* big_list, const uint32_t* big_count,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(k_grid_pass<true>, dim3(grid_for(n_tris, 256)), dim3(256), 0, s, fr, verts,
+                       n_tris, counts, offsets, refs, (uint32_t*)nullptr, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_grid_pass_big<true>, dim3(2048), dim3(256), 0, s, fr, verts, counts, offsets,
+                       refs, big_list, big_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_t* refs,
+                            uint32_t* scratch, uint32_t* list, uint32_t list_cap,
+                            uint32_t* list_count, uint32_t cells_cap, hipStream_t s)
+{
+    (void)scratch;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, list_count, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(k_sort_small, dim3(grid_for(cells_cap, 256)), dim3(256), 0, s, fr, offsets,
+                       refs, list, list_cap, list_count);
+    hipLaunchKernelGGL(k_sort_large, dim3(1024), dim3(256), 0, s, fr, offsets, refs,
+                       (const uint32_t*)list, list_cap, (const uint32_t*)list_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* verts, uint32_t n_tris, float4* tri, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(n_tris, 256)), dim3(256), 0, s, verts, n_tris, tri);
+    return hipGetLastError();
+}
+
+hipError_t launch_transform_pack(uint32_t n_tris, uint32_t tris_per_cluster,
+                                 uint32_t n_cluster_tris, const float* xf,
+                                 const float* rest_v, const float* rest_n, float* verts,
+                                 float* normals, float4* tri, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_transform_pack, dim3(grid_for(n_tris, 256)), dim3(256), 0, s, n_tris,
+                       tris_per_cluster, n_cluster_tris, xf, rest_v, rest_n, verts, normals, tri);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+}  // namespace grt

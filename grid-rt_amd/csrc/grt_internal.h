@@ -1,0 +1,78 @@
+// grt_internal.h -- host-side state behind the opaque handles of grt.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <set>
+#include <vector>
+
+#include "grt/grt.h"
+#include "grt_common.h"
+
+struct grt_context_t;
+
+namespace grt {
+
+int set_error(int code, const char* fmt, ...);
+int hip_check(hipError_t e, const char* what);
+
+// One device allocation, counted against its context.
+struct DevMem {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(grt_context_t* ctx, size_t nbytes, bool zero);
+    void release(grt_context_t* ctx);
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// PinholeCamera::setLookAt + newFrame on the host (camera.cpp).
+void camera_lookat(const float eye[3], const float center[3], const float up[3], float fovy_deg,
+                   float znear, uint32_t width, uint32_t height, grt_camera* out);
+
+}  // namespace grt
+
+struct grt_scene_t;
+
+struct grt_context_t {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    uint64_t bytes_used = 0;
+    uint32_t block_closest = 128;
+    uint32_t block_any = 128;
+    std::set<grt_scene_t*> scenes;
+};
+
+struct grt_scene_t {
+    grt_context_t* ctx = nullptr;
+    uint32_t n_tris = 0;
+    bool has_normals = false;
+    bool rest_valid = false;
+    // Geometry: 9 floats per triangle (current pose), rest pose, 48-byte records.
+    grt::DevMem verts, normals, rest_verts, rest_normals, tri, xf;
+    void* xf_host = nullptr;
+    size_t xf_host_bytes = 0;
+    // Grid.
+    grt::DevMem frame, partial, counts, offsets, refs, block_sums, big_list, sort_list,
+        small_counters;
+    uint32_t cells_cap = 0, refs_cap = 0, list_cap = 0;
+    bool grid_valid = false;
+    bool info_stale = false;
+    float lambda = 0.0f;
+    grt::DevFrame info_host{};
+
+    std::vector<grt::DevMem*> all_buffers()
+    {
+        return {&verts,  &normals,    &rest_verts, &rest_normals, &tri,       &xf,
+                &frame,  &partial,    &counts,     &offsets,      &refs,      &block_sums,
+                &big_list, &sort_list, &small_counters};
+    }
+    ~grt_scene_t()
+    {
+        if (xf_host)
+            (void)hipHostFree(xf_host);
+    }
+};

@@ -1,0 +1,87 @@
+// grt_launch.h -- host-side launchers for the HIP kernels of libgrt.
+// Every launcher enqueues on `stream` and returns the launch status; none
+// synchronises.  Buffer sizes are checked by the callers in grt_api.cpp.
+#pragma once
+
+#include "grt_common.h"
+
+namespace grt {
+
+// ---- grid build (grt_build.hip) --------------------------------------------
+constexpr uint32_t kScanTile = 4096;    // elements per scan block (256 threads x 16)
+constexpr uint32_t kBBoxMaxBlocks = 1024;
+constexpr uint32_t kSortSmallCells = 8;  // cells up to this size sort in registers
+
+// Frame from the vertices: block partial boxes, then one block computes the
+// frame.  `partial` holds 6*kBBoxMaxBlocks floats.
+hipError_t launch_frame_from_verts(const float* verts, uint32_t n_tris, float lambda,
+                                   uint32_t cells_cap, float* partial, DevFrame* fr,
+                                   hipStream_t s);
+// Frame from an explicit (unpadded) box given on the host.
+hipError_t launch_frame_from_box(const float bmin[3], const float bmax[3], uint32_t n_tris,
+                                 float lambda, uint32_t cells_cap, DevFrame* fr, hipStream_t s);
+
+// Count pass: per-cell counts (atomics) and the list of triangles spanning
+// more than kSmallCells cells.  counts must be zero on entry.
+hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_tris,
+                             uint32_t* counts, uint32_t* big_list, uint32_t* big_count,
+                             hipStream_t s);
+// Exclusive scan counts[0..m) -> offsets[0..m], total also to fr->r; flags
+// kStatusRefsOver when the total exceeds refs_cap.  block_sums holds
+// ceil(cells_cap/kScanTile) entries.
+hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
+                            uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
+                            hipStream_t s);
+// Scatter pass: refs[offsets[c] + k] = t; returns counts to zero.
+hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t n_tris,
+                               uint32_t* counts, const uint32_t* offsets, uint32_t* refs,
+                               const uint32_t* big_list, const uint32_t* big_count,
+                               hipStream_t s);
+// Sort every cell's references ascending; `scratch` holds refs_cap entries,
+// `list` holds list_cap cell indices, `list_count` one counter.
+hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_t* refs,
+                            uint32_t* scratch, uint32_t* list, uint32_t list_cap,
+                            uint32_t* list_count, uint32_t cells_cap, hipStream_t s);
+
+// Pack 9-float triangles into the 48-byte traversal record
+// {v0.xyz, e1.x} {e1.yz, e2.xy} {e2.z, id, 0, 0}.
+hipError_t launch_pack(const float* verts, uint32_t n_tris, float4* tri, hipStream_t s);
+// Rigid per-cluster transform of the rest pose, fused with the pack.
+hipError_t launch_transform_pack(uint32_t n_tris, uint32_t tris_per_cluster,
+                                 uint32_t n_cluster_tris, const float* xf,
+                                 const float* rest_v, const float* rest_n, float* verts,
+                                 float* normals, float4* tri, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+
+// Exclusive scan of an arbitrary array (exposed for tests of the scan).
+hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* block_sums,
+                           DevFrame* tmp_frame, hipStream_t s);
+
+// ---- traversal and ray generation (grt_trace.hip) ----------------------------
+struct TraceArgs {
+    const DevFrame* fr;
+    const uint32_t* offsets;
+    const uint32_t* refs;
+    const float4* tri;
+    uint32_t n;
+    uint32_t image_width;   // 0: lane i traces ray i; else 8x8 pixel tiles per wave
+    uint32_t* counters;     // NULL or 2 per ray {cells, tests}
+};
+hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                                uint32_t block, hipStream_t s);
+hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occluded,
+                            uint32_t block, hipStream_t s);
+hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s);
+hipError_t launch_gen_shadow(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
+                             const float* normals, float lx, float ly, float lz, float offset,
+                             const RaySoAOut& out, hipStream_t s);
+hipError_t launch_gen_reflection(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
+                                 const float* normals, float offset, const RaySoAOut& out,
+                                 hipStream_t s);
+hipError_t launch_gen_ao(uint32_t n, uint32_t spp, uint64_t seed, const RaySoA& in,
+                         const HitSoAIn& hits, const float* normals, float offset,
+                         float ao_tmax, const RaySoAOut& out, hipStream_t s);
+hipError_t launch_shade_headlight(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
+                                  const float* normals, uint32_t* rgba, hipStream_t s);
+
+}  // namespace grt

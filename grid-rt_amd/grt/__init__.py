@@ -1,0 +1,410 @@
+"""grt -- Python bindings (ctypes) over libgrt.so, the gfx950 uniform-grid tracer.
+
+The library's C ABI (include/grt/grt.h) takes plain device pointers; here they
+come from torch tensors on ``cuda:0`` (torch is only plumbing for device memory
+and the stream).  Importing this module never needs a GPU; creating a
+:class:`Context` does, and fails loudly (GrtError) when the HIP library or the
+device is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "libgrt.so")
+
+GRT_OK = 0
+GRT_NO_MEMORY = 1
+GRT_INVALID_PARAM = 3
+GRT_DEVICE_ERROR = 4
+GRT_NOT_READY = 5
+GRT_CAPACITY = 6
+
+RAY_EPSILON = np.float32(2e-4)
+RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
+HIT_FIELDS = ("id", "t", "u", "v")
+
+
+class GrtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"grt error {code}: {msg}")
+        self.code = code
+
+
+class Rays(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_void_p) for f in RAY_FIELDS]
+
+
+class Hits(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_void_p) for f in HIT_FIELDS]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("base", ctypes.c_float * 3),
+                ("nu", ctypes.c_float * 3), ("nv", ctypes.c_float * 3),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+
+
+class GridInfo(ctypes.Structure):
+    _fields_ = [("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3),
+                ("cell", ctypes.c_float * 3), ("inv_cell", ctypes.c_float * 3),
+                ("n", ctypes.c_int32 * 3), ("m", ctypes.c_uint32), ("r", ctypes.c_uint32)]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_F = ctypes.c_float
+_SIGS = {
+    "grt_last_error": (ctypes.c_char_p, []),
+    "grt_init": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "grt_shutdown": (ctypes.c_int, [_P]),
+    "grt_set_stream": (ctypes.c_int, [_P, _P]),
+    "grt_synchronize": (ctypes.c_int, [_P]),
+    "grt_memory_used": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "grt_set_block_size": (ctypes.c_int, [_P, _U32, _U32]),
+    "grt_scene_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "grt_scene_destroy": (ctypes.c_int, [_P]),
+    "grt_scene_upload_triangles": (ctypes.c_int, [_P, _P, _U32]),
+    "grt_scene_upload_normals": (ctypes.c_int, [_P, _P, _U32]),
+    "grt_scene_set_triangles_device": (ctypes.c_int, [_P, _P, _P, _U32]),
+    "grt_scene_triangle_count": (ctypes.c_int, [_P, ctypes.POINTER(_U32)]),
+    "grt_scene_device_pointers": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "grt_scene_transform_clusters": (ctypes.c_int, [_P, _P, _U32, _U32, _U32]),
+    "grt_scene_download_geometry": (ctypes.c_int, [_P, _P, _P]),
+    "grt_grid_build": (ctypes.c_int, [_P, _F]),
+    "grt_grid_rebuild_async": (ctypes.c_int, [_P, _F]),
+    "grt_grid_reserve": (ctypes.c_int, [_P, _U32, _U32]),
+    "grt_grid_check": (ctypes.c_int, [_P]),
+    "grt_grid_upload": (ctypes.c_int, [_P, ctypes.POINTER(GridInfo), _P, _P]),
+    "grt_grid_info_get": (ctypes.c_int, [_P, ctypes.POINTER(GridInfo)]),
+    "grt_grid_download": (ctypes.c_int, [_P, _P, _P]),
+    "grt_trace_closest": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32,
+                                         _U32, _P]),
+    "grt_trace_any": (ctypes.c_int, [_P, ctypes.POINTER(Rays), _P, _U32, _U32, _P]),
+    "grt_camera_lookat": (ctypes.c_int, [_F * 3, _F * 3, _F * 3, _F, _F, _U32, _U32,
+                                         ctypes.POINTER(Camera)]),
+    "grt_gen_primary": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(Rays)]),
+    "grt_gen_shadow": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32,
+                                      _F * 3, _F, ctypes.POINTER(Rays)]),
+    "grt_gen_reflection": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32,
+                                          _F, ctypes.POINTER(Rays)]),
+    "grt_gen_ao": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32, _U32,
+                                  ctypes.c_uint64, _F, _F, ctypes.POINTER(Rays)]),
+    "grt_shade_headlight": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32,
+                                           _P]),
+    "grt_debug_scan": (ctypes.c_int, [_P, _P, _P, _U32]),
+    # scene generators (host only)
+    "grt_gen_spheres": (_U32, [ctypes.c_uint64, _U32, _F, _F, _P, _P]),
+    "grt_gen_sphere_clusters": (_U32, [ctypes.c_uint64, _U32, _U32, _F, _F, _F, _P, _P]),
+    "grt_gen_stadium": (_U32, [ctypes.c_uint64, _U32, _U32, _U32, _P, _P, _P]),
+    "grt_gen_cluster_transforms": (None, [ctypes.c_uint64, _U32, _U32, _P, _F, _P]),
+    "grt_gen_uniform": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64]),
+}
+
+
+def lib():
+    """Load libgrt.so (built in-tree by ``make -C grid-rt_amd``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GrtError(GRT_NOT_READY, f"{LIB_PATH} is missing: build it with "
+                           "`make -C grid-rt_amd` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            if hasattr(L, name):
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != GRT_OK:
+        raise GrtError(rc, lib().grt_last_error().decode(errors="replace"))
+
+
+def _ptr(a) -> int:
+    """Data pointer of a numpy array or torch tensor (0 for None)."""
+    if a is None:
+        return 0
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+# ---------------------------------------------------------------------------
+# Scenes (host generators)
+# ---------------------------------------------------------------------------
+def gen_spheres(seed: int, n_spheres: int, rmin: float = 0.02, rmax: float = 0.08):
+    L = lib()
+    n = L.grt_gen_spheres(seed, n_spheres, rmin, rmax, None, None)
+    v = np.empty((n, 9), np.float32)
+    nn = np.empty((n, 9), np.float32)
+    L.grt_gen_spheres(seed, n_spheres, rmin, rmax, _ptr(v), _ptr(nn))
+    return v, nn
+
+
+def gen_sphere_clusters(seed: int, n_clusters: int, per_cluster: int, sigma: float,
+                        rmin: float, rmax: float):
+    L = lib()
+    n = L.grt_gen_sphere_clusters(seed, n_clusters, per_cluster, sigma, rmin, rmax, None, None)
+    v = np.empty((n, 9), np.float32)
+    nn = np.empty((n, 9), np.float32)
+    L.grt_gen_sphere_clusters(seed, n_clusters, per_cluster, sigma, rmin, rmax, _ptr(v), _ptr(nn))
+    return v, nn
+
+
+def gen_stadium(seed: int, n_clusters: int, tris_per_cluster: int, n_cluster_tris: int):
+    L = lib()
+    n = L.grt_gen_stadium(seed, n_clusters, tris_per_cluster, n_cluster_tris, None, None, None)
+    if n == 0:
+        raise GrtError(GRT_INVALID_PARAM, "gen_stadium: bad cluster sizes")
+    v = np.empty((n, 9), np.float32)
+    nn = np.empty((n, 9), np.float32)
+    c = np.empty((n_clusters, 3), np.float32)
+    L.grt_gen_stadium(seed, n_clusters, tris_per_cluster, n_cluster_tris, _ptr(v), _ptr(nn), _ptr(c))
+    return v, nn, c
+
+
+def gen_cluster_transforms(seed: int, frame: int, centers: np.ndarray, max_translation: float):
+    centers = np.ascontiguousarray(centers, np.float32)
+    xf = np.empty((centers.shape[0], 12), np.float32)
+    lib().grt_gen_cluster_transforms(seed, frame, centers.shape[0], _ptr(centers),
+                                     max_translation, _ptr(xf))
+    return xf
+
+
+def camera_lookat(eye, center, up, fovy_deg: float, width: int, height: int,
+                  znear: float = 1.0) -> Camera:
+    cam = Camera()
+    f3 = ctypes.c_float * 3
+    check(lib().grt_camera_lookat(f3(*eye), f3(*center), f3(*up), fovy_deg, znear, width, height,
+                                  ctypes.byref(cam)))
+    return cam
+
+
+def camera_dict(cam: Camera) -> dict:
+    return {"pos": list(cam.pos), "base": list(cam.base), "nu": list(cam.nu), "nv": list(cam.nv),
+            "width": cam.width, "height": cam.height}
+
+
+# ---------------------------------------------------------------------------
+# Device batches
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch  # noqa: deferred so CPU-only users never need a GPU
+    return torch
+
+
+class RayBatch:
+    """Structure-of-arrays ray buffer on the device (8 float32 tensors)."""
+
+    def __init__(self, fields: dict):
+        self.f = fields
+        self.n = int(fields["ox"].numel())
+
+    @classmethod
+    def empty(cls, n: int, device="cuda:0"):
+        t = _torch()
+        return cls({k: t.empty(n, dtype=t.float32, device=device) for k in RAY_FIELDS})
+
+    @classmethod
+    def from_numpy(cls, d: dict, device="cuda:0"):
+        t = _torch()
+        return cls({k: t.from_numpy(np.ascontiguousarray(d[k], np.float32)).to(device)
+                    for k in RAY_FIELDS})
+
+    def numpy(self) -> dict:
+        return {k: v.cpu().numpy() for k, v in self.f.items()}
+
+    def c(self, offset: int = 0) -> Rays:
+        r = Rays()
+        for k in RAY_FIELDS:
+            setattr(r, k, self.f[k].data_ptr() + 4 * offset)
+        return r
+
+
+class HitBatch:
+    def __init__(self, fields: dict):
+        self.f = fields
+        self.n = int(fields["id"].numel())
+
+    @classmethod
+    def empty(cls, n: int, device="cuda:0"):
+        t = _torch()
+        return cls({"id": t.empty(n, dtype=t.int32, device=device),
+                    "t": t.empty(n, dtype=t.float32, device=device),
+                    "u": t.empty(n, dtype=t.float32, device=device),
+                    "v": t.empty(n, dtype=t.float32, device=device)})
+
+    @classmethod
+    def from_numpy(cls, d: dict, device="cuda:0"):
+        t = _torch()
+        return cls({"id": t.from_numpy(np.ascontiguousarray(d["id"], np.int32)).to(device),
+                    **{k: t.from_numpy(np.ascontiguousarray(d[k], np.float32)).to(device)
+                       for k in ("t", "u", "v")}})
+
+    def numpy(self) -> dict:
+        return {k: v.cpu().numpy() for k, v in self.f.items()}
+
+    def c(self, offset: int = 0) -> Hits:
+        h = Hits()
+        for k in HIT_FIELDS:
+            setattr(h, k, self.f[k].data_ptr() + 4 * offset)
+        return h
+
+
+# ---------------------------------------------------------------------------
+# Context / scene handles
+# ---------------------------------------------------------------------------
+class Context:
+    """One device context; by default enqueues on torch's current stream."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        L = lib()
+        h = _P()
+        check(L.grt_init(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        if use_torch_stream:
+            t = _torch()
+            check(L.grt_set_stream(self.h, t.cuda.current_stream(device).cuda_stream))
+
+    def set_stream(self, stream_handle: int):
+        check(lib().grt_set_stream(self.h, stream_handle))
+
+    def synchronize(self):
+        check(lib().grt_synchronize(self.h))
+
+    def memory_used(self) -> int:
+        v = ctypes.c_uint64()
+        check(lib().grt_memory_used(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_block_size(self, closest: int, any_hit: int):
+        check(lib().grt_set_block_size(self.h, closest, any_hit))
+
+    def gen_primary(self, cam: Camera, out: RayBatch):
+        check(lib().grt_gen_primary(self.h, ctypes.byref(cam), ctypes.byref(out.c())))
+
+    def debug_scan(self, x, out):
+        check(lib().grt_debug_scan(self.h, _ptr(x), _ptr(out), int(x.numel())))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().grt_shutdown(self.h)
+            self.h = None
+
+
+class Scene:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = _P()
+        check(lib().grt_scene_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+        self.n_tris = 0
+
+    def upload(self, verts: np.ndarray, normals: np.ndarray | None = None):
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+        self.n_tris = verts.shape[0]
+        check(lib().grt_scene_upload_triangles(self.h, _ptr(verts), self.n_tris))
+        if normals is not None:
+            normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+            check(lib().grt_scene_upload_normals(self.h, _ptr(normals), normals.shape[0]))
+
+    def transform_clusters(self, xf: np.ndarray, tris_per_cluster: int, n_cluster_tris: int):
+        xf = np.ascontiguousarray(xf, np.float32).reshape(-1, 12)
+        check(lib().grt_scene_transform_clusters(self.h, _ptr(xf), xf.shape[0], tris_per_cluster,
+                                                 n_cluster_tris))
+
+    def download_geometry(self):
+        v = np.empty((self.n_tris, 9), np.float32)
+        n = np.empty((self.n_tris, 9), np.float32)
+        check(lib().grt_scene_download_geometry(self.h, _ptr(v), _ptr(n)))
+        return v, n
+
+    def device_geometry(self):
+        """(verts, normals) device pointers of the current pose."""
+        v, n = _P(), _P()
+        check(lib().grt_scene_device_pointers(self.h, ctypes.byref(v), ctypes.byref(n)))
+        return v.value, n.value
+
+    def build_grid(self, lam: float = 5.0):
+        check(lib().grt_grid_build(self.h, lam))
+
+    def rebuild_grid_async(self, lam: float = 5.0):
+        check(lib().grt_grid_rebuild_async(self.h, lam))
+
+    def reserve_grid(self, cells: int, refs: int):
+        check(lib().grt_grid_reserve(self.h, cells, refs))
+
+    def check_grid(self):
+        check(lib().grt_grid_check(self.h))
+
+    def upload_grid(self, frame: dict, offsets: np.ndarray, refs: np.ndarray):
+        gi = GridInfo()
+        for k in ("bmin", "bmax", "cell", "inv_cell"):
+            getattr(gi, k)[:] = [float(x) for x in frame[k]]
+        gi.n[:] = [int(x) for x in frame["n"]]
+        offsets = np.ascontiguousarray(offsets, np.uint32)
+        refs = np.ascontiguousarray(refs, np.uint32)
+        check(lib().grt_grid_upload(self.h, ctypes.byref(gi), _ptr(offsets), _ptr(refs)))
+
+    def grid_info(self) -> dict:
+        gi = GridInfo()
+        check(lib().grt_grid_info_get(self.h, ctypes.byref(gi)))
+        return {"bmin": np.array(gi.bmin[:], np.float32), "bmax": np.array(gi.bmax[:], np.float32),
+                "cell": np.array(gi.cell[:], np.float32),
+                "inv_cell": np.array(gi.inv_cell[:], np.float32),
+                "n": np.array(gi.n[:], np.int32), "m": int(gi.m), "r": int(gi.r)}
+
+    def download_grid(self):
+        info = self.grid_info()
+        offsets = np.empty(info["m"] + 1, np.uint32)
+        refs = np.empty(max(info["r"], 1), np.uint32)
+        check(lib().grt_grid_download(self.h, _ptr(offsets), _ptr(refs)))
+        return info, offsets, refs[:info["r"]]
+
+    def trace_closest(self, rays: RayBatch, hits: HitBatch, n: int | None = None,
+                      image_width: int = 0, counters=None, offset: int = 0):
+        n = rays.n if n is None else n
+        check(lib().grt_trace_closest(self.h, ctypes.byref(rays.c(offset)),
+                                      ctypes.byref(hits.c(offset)), n, image_width,
+                                      _ptr(counters)))
+
+    def trace_any(self, rays: RayBatch, occluded, n: int | None = None, image_width: int = 0,
+                  counters=None):
+        n = rays.n if n is None else n
+        check(lib().grt_trace_any(self.h, ctypes.byref(rays.c()), _ptr(occluded), n, image_width,
+                                  _ptr(counters)))
+
+    def gen_shadow(self, rays: RayBatch, hits: HitBatch, light, offset: float, out: RayBatch):
+        f3 = ctypes.c_float * 3
+        check(lib().grt_gen_shadow(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()),
+                                   rays.n, f3(*light), offset, ctypes.byref(out.c())))
+
+    def gen_reflection(self, rays: RayBatch, hits: HitBatch, offset: float, out: RayBatch):
+        check(lib().grt_gen_reflection(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()),
+                                       rays.n, offset, ctypes.byref(out.c())))
+
+    def gen_ao(self, rays: RayBatch, hits: HitBatch, spp: int, seed: int, offset: float,
+               ao_tmax: float, out: RayBatch):
+        check(lib().grt_gen_ao(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()), rays.n,
+                               spp, seed, offset, ao_tmax, ctypes.byref(out.c())))
+
+    def shade_headlight(self, rays: RayBatch, hits: HitBatch, rgba):
+        check(lib().grt_shade_headlight(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()),
+                                        rays.n, _ptr(rgba)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().grt_scene_destroy(self.h)
+            self.h = None

@@ -1,0 +1,178 @@
+/*
+ * grt.h -- C ABI of libgrt, the MI355X (gfx950) uniform-grid ray tracer.
+ *
+ * This is the drop-in boundary for the reference's CUDA tracer C ABI
+ * (src/rtcuda/RayTracing.h:6-44) and the GLSL renderer's pass/buffer contract
+ * (src/rtgl/GpuRenderer.cpp:74-319, 598-703).  Signatures use plain pointers
+ * and sizes only.  Unless stated otherwise:
+ *   - "_host" pointers are host memory and are copied before return;
+ *   - ray, hit and image buffers are DEVICE pointers owned by the caller
+ *     (structure of arrays, one float/int per ray);
+ *   - work is enqueued on the context's stream (grt_set_stream) and the call
+ *     returns without synchronising;
+ *   - every function returns GRT_OK (0) or an error code, never aborts, and
+ *     grt_last_error() describes the last failure of the calling thread.
+ * Error codes mirror the reference's RT_OK / RT_NO_MEMORY / RT_INVALID_PARAM
+ * (include/rt/common.h:19-21).
+ */
+#ifndef GRT_GRT_H
+#define GRT_GRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRT_OK             0x0000 /* RT_OK */
+#define GRT_NO_MEMORY      0x0001 /* RT_NO_MEMORY */
+#define GRT_INVALID_PARAM  0x0003 /* RT_INVALID_PARAM */
+#define GRT_DEVICE_ERROR   0x0004
+#define GRT_NOT_READY      0x0005 /* grid not built / scene empty */
+#define GRT_CAPACITY       0x0006 /* a device-side capacity was exceeded */
+
+typedef struct grt_context_t* grt_context;
+typedef struct grt_scene_t* grt_scene;
+
+/* Ray batch, structure of arrays of device pointers (n floats each).
+ * Replaces the reference's per-thread Ray (src/rtcuda/KernelDependencies.h:70-78;
+ * include/rt/Ray.h:9-24).  A ray is inactive when tmin > tmax. */
+typedef struct {
+    float* ox; float* oy; float* oz;
+    float* dx; float* dy; float* dz;
+    float* tmin; float* tmax;
+} grt_rays;
+
+/* Closest-hit records (include/rt/Hit.h:9-17; KernelDependencies.h:80-86):
+ * id = triangle index or -1 (miss), t = distance (+inf on miss), u,v = the
+ * barycentric weights of vertices 1 and 2 (Hit::v1Coord, v2Coord). */
+typedef struct {
+    int32_t* id; float* t; float* u; float* v;
+} grt_hits;
+
+/* Pinhole camera as passed by the reference C ABI cudaSetCameraParameters
+ * (RayTracing.h:20-23): position, base direction (ray through pixel corner
+ * (0,0)), near-plane U and V vectors; image size. */
+typedef struct {
+    float pos[3]; float base[3]; float nu[3]; float nv[3];
+    uint32_t width; uint32_t height;
+} grt_camera;
+
+/* Grid frame and size (UniformGridAccStruct.h:22-33; cudaSetBoxMin/Max and
+ * cudaSetGridParameters, RayTracing.h:33-36). */
+typedef struct {
+    float bmin[3]; float bmax[3]; float cell[3]; float inv_cell[3];
+    int32_t n[3];
+    uint32_t m;   /* cells */
+    uint32_t r;   /* references */
+} grt_grid_info;
+
+/* ---- context ---------------------------------------------------------- */
+/* rtInitContext (RayTracing.h:8; RayTracing.cu:89-99). */
+int grt_init(int device, grt_context* out);
+int grt_shutdown(grt_context ctx);
+const char* grt_last_error(void);
+/* Enqueue on this hipStream_t from now on (NULL = the HIP null stream).  A new
+ * context starts on a non-blocking stream of its own. */
+int grt_set_stream(grt_context ctx, void* hip_stream);
+int grt_synchronize(grt_context ctx);
+/* Bytes of device memory currently held by the context's scenes. */
+int grt_memory_used(grt_context ctx, uint64_t* bytes);
+/* Thread-block size of the traversal kernels (64..256, multiple of 64);
+ * cudaSetBlockSize (RayTracing.h:39) was a no-op in the reference. */
+int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
+
+/* ---- scene ------------------------------------------------------------ */
+int grt_scene_create(grt_context ctx, grt_scene* out);
+int grt_scene_destroy(grt_scene scene);
+/* cudaTransferTriangleVertices (RayTracing.h:27): 9 floats per triangle
+ * (v0,v1,v2), n_tris triangles.  Also packs the 48-byte traversal records. */
+int grt_scene_upload_triangles(grt_scene scene, const float* verts_host, uint32_t n_tris);
+/* cudaTransferTriangleNormals (RayTracing.h:28): 9 floats per triangle. */
+int grt_scene_upload_normals(grt_scene scene, const float* normals_host, uint32_t n_tris);
+/* Same from device memory (device-to-device copy). */
+int grt_scene_set_triangles_device(grt_scene scene, const float* verts_dev,
+                                   const float* normals_dev, uint32_t n_tris);
+int grt_scene_triangle_count(grt_scene scene, uint32_t* n_tris);
+/* Device pointers of the current vertices / normals (9 floats per triangle). */
+int grt_scene_device_pointers(grt_scene scene, float** verts_dev, float** normals_dev);
+/* Copy the current vertices / normals to the host (either pointer may be NULL). */
+int grt_scene_download_geometry(grt_scene scene, float* verts_host, float* normals_host);
+/* Keyframe update on the device (GpuRenderer.cpp:94-130,705-795 uploaded
+ * whole keyframes instead): current = M_c * rest for triangle t < n_cluster_tris
+ * with c = t / tris_per_cluster, M_c the row-major 3x4 matrix xf_host[12c..];
+ * normals get the 3x3 part; other triangles keep the rest pose.  The rest pose
+ * is the last uploaded geometry. */
+int grt_scene_transform_clusters(grt_scene scene, const float* xf_host, uint32_t n_clusters,
+                                 uint32_t tris_per_cluster, uint32_t n_cluster_tris);
+
+/* ---- grid ------------------------------------------------------------- */
+/* Device build of the uniform grid (UniformGridAccStructBuilder::buildGeometry,
+ * UniformGridAccStructBuilder.cpp:58-149; GpuGrid::rebuildGrid call site
+ * GpuRenderer.cpp:122,663-677): lambda replaces the reference's k=6.  Sizes
+ * device buffers as needed (synchronises the stream twice). */
+int grt_grid_build(grt_scene scene, float lambda);
+/* Rebuild without host synchronisation, reusing the capacities of the last
+ * grt_grid_build / grt_grid_reserve.  Overflow is flagged on the device and
+ * reported by grt_grid_check. */
+int grt_grid_rebuild_async(grt_scene scene, float lambda);
+int grt_grid_reserve(grt_scene scene, uint32_t cells, uint32_t refs);
+/* Synchronise and report the device status of the last (re)build. */
+int grt_grid_check(grt_scene scene);
+/* Upload a grid built elsewhere (cudaTransferCellPointers /
+ * cudaTransferCellTriangleIds / cudaSetBoxMin / cudaSetBoxMax /
+ * cudaSetGridParameters, RayTracing.h:31-36): frame from info (bmin, bmax,
+ * cell, inv_cell, n), offsets[m+1], refs[offsets[m]] plain triangle ids. */
+int grt_grid_upload(grt_scene scene, const grt_grid_info* info, const uint32_t* offsets_host,
+                    const uint32_t* refs_host);
+int grt_grid_info_get(grt_scene scene, grt_grid_info* out);
+int grt_grid_download(grt_scene scene, uint32_t* offsets_host, uint32_t* refs_host);
+
+/* ---- tracing ---------------------------------------------------------- */
+/* Closest hit for n rays (traverse3ddda + intersectTriangles,
+ * UniformGridAccStruct.cpp:30-42,113-196,362-378; kernel rayTrace,
+ * RayTracing_kernel.cu:140-275).  image_width != 0 maps each wave to an 8x8
+ * tile of a row-major image of that width.  counters_dev: NULL or 2 uint32 per
+ * ray {cells visited, triangle tests}. */
+int grt_trace_closest(grt_scene scene, const grt_rays* rays, const grt_hits* hits, uint32_t n,
+                      uint32_t image_width, uint32_t* counters_dev);
+/* Any hit in (tmin, tmax] (findAnyHit, shadow_FRAG.glsl:147-245; the
+ * reference's IAccStruct::traceAnyInstance stub, IAccStruct.cpp:108-114).
+ * occluded_dev: 1 byte per ray. */
+int grt_trace_any(grt_scene scene, const grt_rays* rays, uint8_t* occluded_dev, uint32_t n,
+                  uint32_t image_width, uint32_t* counters_dev);
+
+/* ---- ray generation --------------------------------------------------- */
+/* PinholeCamera::setLookAt/setPerspective/setViewport/newFrame
+ * (PinholeCamera.cpp:23-55,91-118), host only. */
+int grt_camera_lookat(const float eye[3], const float center[3], const float up[3],
+                      float fovy_deg, float znear, uint32_t width, uint32_t height,
+                      grt_camera* out);
+/* Primary rays, pixel i = y*width + x (PinholeCamera::getRayDirection,
+ * PinholeCamera.cpp:131-137): tmin = 2e-4 (Context rayEpsilon), tmax = +inf. */
+int grt_gen_primary(grt_context ctx, const grt_camera* cam, const grt_rays* out);
+/* Shadow rays toward a point light, one slot per input ray
+ * (shadow_FRAG.glsl:259-270; SimplePointLight::illuminate,
+ * SimplePointLight.cpp:18-37).  Inactive slots: tmin=1, tmax=-1. */
+int grt_gen_shadow(grt_scene scene, const grt_rays* in, const grt_hits* hits, uint32_t n,
+                   const float light[3], float offset, const grt_rays* out);
+/* Mirror reflection rays (Sample::initReflectionRay, Sample.cpp:44-52). */
+int grt_gen_reflection(grt_scene scene, const grt_rays* in, const grt_hits* hits, uint32_t n,
+                       float offset, const grt_rays* out);
+/* spp cosine-weighted ambient-occlusion rays per input ray, slot i*spp+k. */
+int grt_gen_ao(grt_scene scene, const grt_rays* in, const grt_hits* hits, uint32_t n,
+               uint32_t spp, uint64_t seed, float offset, float ao_tmax, const grt_rays* out);
+/* Headlight shading to RGBA8 (RayTracing_kernel.cu:104-120): the image the
+ * reference's cudaRayTrace wrote into its pixel buffer. */
+int grt_shade_headlight(grt_scene scene, const grt_rays* rays, const grt_hits* hits, uint32_t n,
+                        uint32_t* rgba_dev);
+
+/* ---- test hooks ------------------------------------------------------- */
+/* Exclusive scan of n uint32 (device pointers; out has n+1 entries). */
+int grt_debug_scan(grt_context ctx, const uint32_t* in_dev, uint32_t* out_dev, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

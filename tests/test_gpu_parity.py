@@ -1,0 +1,324 @@
+"""GPU parity: the HIP path against the CPU oracle on identical seeded inputs.
+
+Bar: bit-exact.  Grid frames, offsets and references are integers / exactly
+specified binary32 values; traversal results (id, t, u, v) and the per-ray
+counters (cells visited, triangle tests) are compared bit for bit because both
+sides evaluate the same operation sequence (explicit FMAs, IEEE division and
+sqrt, no contraction).  All calls go through the C ABI (libgrt.so).
+"""
+import numpy as np
+import pytest
+
+import grt
+import pyoracle as orc
+from conftest import SEED_C1, SEED_C2, c1_camera, rng
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    a = np.asarray(a)
+    return a.view(np.uint32) if a.dtype == np.float32 else a
+
+
+def assert_same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
+    diff = np.nonzero(bits(a) != bits(b))[0]
+    assert diff.size == 0, (f"{what}: {diff.size} of {a.size} differ, first at {diff[:5]}: "
+                            f"{a[diff[:5]]} vs {b[diff[:5]]}")
+
+
+def assert_frame(info, fr):
+    for k in ("bmin", "bmax", "cell", "inv_cell"):
+        assert_same(np.asarray(info[k], np.float32), np.asarray(fr[k], np.float32), k)
+    assert list(info["n"]) == list(fr["n"])
+    assert info["m"] == fr["m"]
+
+
+def build_both(ctx, v, n, lam):
+    sc = grt.Scene(ctx)
+    sc.upload(v, n)
+    sc.build_grid(lam)
+    info, offs, refs = sc.download_grid()
+    fr = orc.grid_frame(v, lam)
+    _, ooffs, orefs = orc.grid_build(v, fr)
+    return sc, fr, info, offs, refs, ooffs, orefs
+
+
+def check_grid(ctx, v, n, lam):
+    sc, fr, info, offs, refs, ooffs, orefs = build_both(ctx, v, n, lam)
+    assert_frame(info, fr)
+    assert_same(offs, ooffs, "offsets")
+    assert info["r"] == len(orefs)
+    assert_same(refs, orefs, "refs")
+    return sc, fr, ooffs, orefs
+
+
+def trace_both(sc, fr, offs, refs, v, rays, image_width=0):
+    n = rays["ox"].shape[0]
+    rb = grt.RayBatch.from_numpy(rays)
+    hb = grt.HitBatch.empty(n)
+    import torch
+    cnt = torch.empty(2 * n, dtype=torch.int32, device="cuda:0")
+    sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
+    g = hb.numpy()
+    gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+    o = orc.trace_closest(fr, offs, refs, v, rays, counters=True)
+    return g, gc, o
+
+
+def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
+    g, gc, o = trace_both(sc, fr, offs, refs, v, rays, image_width)
+    for k in ("id", "t", "u", "v"):
+        assert_same(g[k], o[k], f"hit.{k}")
+    assert_same(gc[:, 0], o["cells"], "cells visited")
+    assert_same(gc[:, 1], o["tests"], "triangle tests")
+    return g, o
+
+
+def check_any(sc, fr, offs, refs, v, rays):
+    import torch
+    n = rays["ox"].shape[0]
+    rb = grt.RayBatch.from_numpy(rays)
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    cnt = torch.empty(2 * n, dtype=torch.int32, device="cuda:0")
+    sc.trace_any(rb, occ, counters=cnt)
+    o = orc.trace_any(fr, offs, refs, v, rays, counters=True)
+    gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+    assert_same(occ.cpu().numpy(), o["occluded"], "occluded")
+    assert_same(gc[:, 0], o["cells"], "cells visited")
+    assert_same(gc[:, 1], o["tests"], "triangle tests")
+    return o
+
+
+# ---------------------------------------------------------------------------
+# Scan
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 4095, 4096, 4097, 100003, 4096 * 1024 + 7])
+def test_scan_exclusive(gpu_ctx, n):
+    import torch
+    x = rng(n).integers(0, 50, size=n, dtype=np.uint32)
+    xd = torch.from_numpy(x.view(np.int32)).cuda()
+    out = torch.empty(n + 1, dtype=torch.int32, device="cuda:0")
+    gpu_ctx.debug_scan(xd, out)
+    got = out.cpu().numpy().view(np.uint32)
+    ref = np.zeros(n + 1, np.uint64)
+    ref[1:] = np.cumsum(x, dtype=np.uint64)
+    assert np.array_equal(got, ref.astype(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# Grid build
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("lam", [2.0, 5.0, 8.0])
+def test_grid_build_c1(gpu_ctx, c1_scene, lam):
+    v, n = c1_scene
+    check_grid(gpu_ctx, v, n, lam)
+
+
+def test_grid_build_rebuild_twice_same(gpu_ctx, c1_scene):
+    v, n = c1_scene
+    sc = grt.Scene(gpu_ctx)
+    sc.upload(v, n)
+    sc.build_grid(5.0)
+    a = sc.download_grid()
+    sc.rebuild_grid_async(5.0)
+    sc.check_grid()
+    b = sc.download_grid()
+    assert_same(a[1], b[1], "offsets")
+    assert_same(a[2], b[2], "refs")
+
+
+def _edge_scenes():
+    r = rng(7)
+    out = {}
+    out["single"] = np.array([[0.1, 0.2, 0.3, 0.9, 0.25, 0.35, 0.4, 0.8, 0.6]], np.float32)
+    planar = r.random((200, 9), dtype=np.float32)
+    planar[:, 1::3] = 0.5
+    out["planar_y"] = planar
+    degen = r.random((300, 9), dtype=np.float32)
+    degen[::7, 3:6] = degen[::7, 0:3]
+    degen[::7, 6:9] = degen[::7, 0:3]
+    out["degenerate"] = degen
+    dup = np.repeat(r.random((50, 9), dtype=np.float32), 4, axis=0)
+    out["duplicates"] = dup
+    big = r.random((500, 9), dtype=np.float32) * 0.05 + 0.4
+    big = np.vstack([big, np.array([[-1, -1, -1, 2, -1, -1, -1, 2, 2]], np.float32)])
+    out["big_triangle"] = big
+    # One hot cell holding > 16384 references (exercises the tiled global sort).
+    hot = r.random((40000, 9), dtype=np.float32) * 1e-3 + 0.5
+    spread = r.random((64, 9), dtype=np.float32)
+    out["hot_cell"] = np.vstack([hot, spread]).astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("name", ["single", "planar_y", "degenerate", "duplicates",
+                                  "big_triangle", "hot_cell"])
+def test_grid_build_edge(gpu_ctx, name):
+    v = _edge_scenes()[name]
+    n = np.zeros_like(v)
+    check_grid(gpu_ctx, v, n, 5.0)
+
+
+def test_grid_build_stadium_small(gpu_ctx):
+    v, n, _ = grt.gen_stadium(SEED_C2, 16, 2048, 16 * 2048 - 2)
+    check_grid(gpu_ctx, v, n, 5.0)
+
+
+def test_grid_build_stadium_full(gpu_ctx):
+    """Config 2 size: 2^20 triangles, 256 clusters + ground plane."""
+    v, n, _ = grt.gen_stadium(SEED_C2, 256, 4096, (1 << 20) - 2)
+    check_grid(gpu_ctx, v, n, 5.0)
+
+
+# ---------------------------------------------------------------------------
+# Rays
+# ---------------------------------------------------------------------------
+def test_gen_primary_parity(gpu_ctx):
+    cam = c1_camera(640, 360)
+    rb = grt.RayBatch.empty(640 * 360)
+    gpu_ctx.gen_primary(cam, rb)
+    g = rb.numpy()
+    o = orc.primary_rays(grt.camera_dict(cam))
+    for k in g:
+        assert_same(g[k], o[k], f"primary.{k}")
+
+
+def test_camera_host_parity():
+    for eye, ctr, up, fov, w, h in [((0.5, 0.5, -1.5), (0.5, 0.5, 0.0), (0, 1, 0), 60.0, 512, 512),
+                                    ((0.3, 2.0, -3.0), (0.5, 0.2, 0.5), (0, 1, 0), 45.0, 1920, 1080),
+                                    ((5, 5, 5), (0, 0, 0), (0, 0, 1), 30.0, 64, 48)]:
+        a = grt.camera_dict(grt.camera_lookat(eye, ctr, up, fov, w, h))
+        b = orc.camera_lookat(eye, ctr, up, fov, w, h)
+        for k in ("pos", "base", "nu", "nv"):
+            assert_same(np.float32(a[k]), np.float32(b[k]), k)
+
+
+@pytest.mark.parametrize("image_width", [0, 512])
+def test_closest_c1_primary(gpu_ctx, c1_scene, image_width):
+    v, n = c1_scene
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, n, 5.0)
+    rays = orc.primary_rays(grt.camera_dict(c1_camera()))
+    g, o = check_closest(sc, fr, offs, refs, v, rays, image_width)
+    assert (o["id"] >= 0).sum() > 10000
+
+
+def _random_rays(n, seed, inside=True):
+    r = rng(seed)
+    o = r.random((n, 3), dtype=np.float32) if inside else \
+        (r.random((n, 3), dtype=np.float32) * 4 - 1.5).astype(np.float32)
+    d = r.normal(size=(n, 3)).astype(np.float32)
+    # axis-aligned and zero components, including negative zero
+    d[::11, 1:] = 0.0
+    d[1::13, 0] = -0.0
+    d[2::17, :2] = 0.0
+    return {"ox": o[:, 0], "oy": o[:, 1], "oz": o[:, 2], "dx": d[:, 0], "dy": d[:, 1],
+            "dz": d[:, 2], "tmin": np.full(n, 1e-4, np.float32),
+            "tmax": np.where(np.arange(n) % 5 == 0, np.float32(0.3), np.float32(np.inf))}
+
+
+@pytest.mark.parametrize("inside", [True, False])
+def test_closest_random_rays(gpu_ctx, c1_scene, inside):
+    v, n = c1_scene
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, n, 5.0)
+    check_closest(sc, fr, offs, refs, v, _random_rays(50000, 3 + inside, inside))
+
+
+def test_any_random_rays(gpu_ctx, c1_scene):
+    v, n = c1_scene
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, n, 5.0)
+    check_any(sc, fr, offs, refs, v, _random_rays(50000, 11))
+
+
+def _secondary_parity(gpu_ctx, sc, fr, offs, refs, v, nrm, rays, light):
+    """Primary hits -> shadow, reflection and AO rays generated on both sides,
+    then traced on both sides."""
+    g, o = check_closest(sc, fr, offs, refs, v, rays)
+    rb = grt.RayBatch.from_numpy(rays)
+    hb = grt.HitBatch.from_numpy(g)
+    n = rays["ox"].shape[0]
+    # shadow
+    sb = grt.RayBatch.empty(n)
+    sc.gen_shadow(rb, hb, light, 1e-3, sb)
+    so = orc.gen_shadow(rays, o, nrm, light, 1e-3)
+    sg = sb.numpy()
+    for k in sg:
+        assert_same(sg[k], so[k], f"shadow.{k}")
+    check_any(sc, fr, offs, refs, v, so)
+    # reflection
+    fb = grt.RayBatch.empty(n)
+    sc.gen_reflection(rb, hb, 1e-3, fb)
+    fo = orc.gen_reflection(rays, o, nrm, 1e-3)
+    fg = fb.numpy()
+    for k in fg:
+        assert_same(fg[k], fo[k], f"reflection.{k}")
+    check_closest(sc, fr, offs, refs, v, fo)
+    # ambient occlusion
+    spp = 4
+    ab = grt.RayBatch.empty(n * spp)
+    sc.gen_ao(rb, hb, spp, 1234, 1e-3, 0.05, ab)
+    ao = orc.gen_ao(rays, o, nrm, spp, 1234, 1e-3, 0.05)
+    ag = ab.numpy()
+    for k in ag:
+        assert_same(ag[k], ao[k], f"ao.{k}")
+    check_any(sc, fr, offs, refs, v, ao)
+    # headlight image
+    import torch
+    rgba = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    sc.shade_headlight(rb, hb, rgba)
+    img = rgba.cpu().numpy().view(np.uint32)
+    idx = np.nonzero(o["id"] >= 0)[0][:200]
+    for i in idx:
+        d = (rays["dx"][i], rays["dy"][i], rays["dz"][i])
+        assert img[i] == orc.headlight(d, nrm[o["id"][i]], o["id"][i], o["u"][i], o["v"][i])
+    assert np.all(img[o["id"] < 0] == 0xFFFFFFFF)
+
+
+def test_secondary_c1(gpu_ctx, c1_scene):
+    v, nrm = c1_scene
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, nrm, 5.0)
+    rays = orc.primary_rays(grt.camera_dict(c1_camera(256, 256)))
+    _secondary_parity(gpu_ctx, sc, fr, offs, refs, v, nrm, rays, (1000.0, 1000.0, 1000.0))
+
+
+def test_secondary_stadium(gpu_ctx):
+    v, nrm, _ = grt.gen_stadium(SEED_C2, 16, 2048, 16 * 2048 - 2)
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, nrm, 5.0)
+    cam = grt.camera_lookat((0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0, 1, 0), 60.0, 192, 108)
+    rays = orc.primary_rays(grt.camera_dict(cam))
+    _secondary_parity(gpu_ctx, sc, fr, offs, refs, v, nrm, rays, (1000.0, 1000.0, 1000.0))
+
+
+def test_uploaded_grid_matches_built(gpu_ctx, c1_scene):
+    v, n = c1_scene
+    fr = orc.grid_frame(v, 5.0)
+    _, offs, refs = orc.grid_build(v, fr)
+    sc = grt.Scene(gpu_ctx)
+    sc.upload(v, n)
+    sc.upload_grid(fr, offs, refs)
+    rays = orc.primary_rays(grt.camera_dict(c1_camera(128, 128)))
+    check_closest(sc, fr, offs, refs, v, rays)
+
+
+def test_transform_parity(gpu_ctx):
+    v, nrm, centers = grt.gen_stadium(SEED_C2, 16, 2048, 16 * 2048 - 2)
+    sc = grt.Scene(gpu_ctx)
+    sc.upload(v, nrm)
+    for frame in (1, 2):
+        xf = grt.gen_cluster_transforms(SEED_C2, frame, centers, 0.05)
+        sc.transform_clusters(xf, 2048, 16 * 2048 - 2)
+        ov, on = orc.transform(v, nrm, xf, 2048, 16 * 2048 - 2)
+        gv, gn = sc.download_geometry()
+        assert_same(gv.ravel(), ov.ravel(), "transformed vertices")
+        assert_same(gn.ravel(), on.ravel(), "transformed normals")
+        sc.build_grid(5.0)
+        info, offs, refs = sc.download_grid()
+        fr = orc.grid_frame(ov, 5.0)
+        _, ooffs, orefs = orc.grid_build(ov, fr)
+        assert_frame(info, fr)
+        assert_same(offs, ooffs, "offsets after transform")
+        assert_same(refs, orefs, "refs after transform")
+        cam = grt.camera_lookat((0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0, 1, 0), 60.0, 96, 54)
+        rays = orc.primary_rays(grt.camera_dict(cam))
+        check_closest(sc, fr, ooffs, orefs, ov, rays)

@@ -1,0 +1,428 @@
+"""Pins for the CPU oracle (oracle/grt_oracle.c), CPU only.
+
+The reference ships no tests or golden vectors for this path and cannot be
+compiled here (it needs the un-vendored vrbase library), so every oracle
+function is pinned by checks that do not restate its formula: closed forms,
+known answers, FP64 brute force and geometric invariants.  Each check is
+chosen so that a plausible slip (swapped u/v, dropped cell-exit bound, wrong
+truncation, wrong step order) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import grt
+import pyoracle as orc
+from conftest import SEED_C1, SEED_C2, rng
+
+
+# ---------------------------------------------------------------------------
+# a2: frame
+# ---------------------------------------------------------------------------
+def test_cbrtf_is_correctly_rounded_cube_root():
+    xs = np.concatenate([np.geomspace(1e-30, 1e30, 20000), rng(1).random(5000) * 1e4,
+                         [1.0, 8.0, 27.0, 64.0, 125.0, 1e6, 0.125]]).astype(np.float32)
+    got = np.array([orc.cbrtf(float(x)) for x in xs], np.float32)
+    exact = np.cbrt(xs.astype(np.float64)).astype(np.float32)
+    # exact cubes
+    for c in (1.0, 2.0, 3.0, 4.0, 5.0, 100.0, 0.5):
+        assert orc.cbrtf(float(np.float32(c) ** 3)) == np.float32(c)
+    ulp = np.abs(got.view(np.int32).astype(np.int64) - exact.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1
+    assert (ulp == 0).mean() > 0.999
+
+
+@pytest.mark.parametrize("lam", [2.0, 5.0, 8.0])
+def test_frame_closed_form(lam):
+    r = rng(int(lam))
+    for _ in range(200):
+        mn = (r.random(3) * 10 - 5).astype(np.float32)
+        ext = (r.random(3) * 10 + 0.01).astype(np.float32)
+        if r.random() < 0.2:
+            ext[r.integers(3)] = 0.0  # planar axis
+        mx = (mn + ext).astype(np.float32)
+        n_tris = int(r.integers(1, 2_000_000))
+        f = orc.grid_frame_from_box(mn, mx, n_tris, lam)
+        ext64 = mx.astype(np.float64) - mn.astype(np.float64)
+        pad = np.where(ext64 == 0, 0.01, 0.01 * ext64)
+        assert np.allclose(f["bmin"], mn - pad, rtol=1e-6, atol=1e-6)
+        assert np.allclose(f["bmax"], mx + pad, rtol=1e-6, atol=1e-6)
+        diag = f["bmax"].astype(np.float64) - f["bmin"].astype(np.float64)
+        fac = (lam * n_tris / np.prod(diag)) ** (1.0 / 3.0)
+        want = np.maximum(np.ceil(diag * fac), 1)
+        near_int = np.abs(diag * fac - np.round(diag * fac)) < 1e-4
+        assert np.all((f["n"] == want) | near_int), (f["n"], want, diag * fac)
+        assert np.allclose(f["cell"] * f["n"], diag, rtol=2e-6)
+        assert np.allclose(f["cell"] * f["inv_cell"], 1.0, rtol=2e-7)
+        assert f["m"] == int(np.prod(f["n"].astype(np.int64)))
+
+
+def test_frame_of_vertices_is_padded_bbox():
+    v, _ = grt.gen_spheres(SEED_C1, 64)
+    f = orc.grid_frame(v, 5.0)
+    pts = v.reshape(-1, 3)
+    lo, hi = pts.min(0), pts.max(0)
+    ext = hi.astype(np.float64) - lo
+    assert np.allclose(f["bmin"], lo - 0.01 * ext, rtol=1e-6)
+    assert np.allclose(f["bmax"], hi + 0.01 * ext, rtol=1e-6)
+    # M is close to lambda*N for a near-cubic box
+    assert 0.9 * 5 * len(v) < f["m"] < 1.4 * 5 * len(v)
+
+
+# ---------------------------------------------------------------------------
+# a3-a5: grid build
+# ---------------------------------------------------------------------------
+def _geo_cells(frame, tri, tol):
+    """Cells whose box intersects the triangle's AABB, in float64; with tol>0
+    the overlap must exceed tol*cell on every axis (strict), tol<0 allows a
+    gap of |tol|*cell (closed, with slack)."""
+    p = tri.reshape(3, 3).astype(np.float64)
+    lo, hi = p.min(0), p.max(0)
+    bmin = frame["bmin"].astype(np.float64)
+    cell = (frame["bmax"].astype(np.float64) - bmin) / frame["n"]
+    out = []
+    rngs = []
+    for d in range(3):
+        k = np.arange(frame["n"][d])
+        c0 = bmin[d] + k * cell[d]
+        c1 = c0 + cell[d]
+        ov = np.minimum(c1, hi[d]) - np.maximum(c0, lo[d])
+        rngs.append(k[ov > tol * cell[d]])
+    for z in rngs[2]:
+        for y in rngs[1]:
+            for x in rngs[0]:
+                out.append(int(x + frame["n"][0] * (y + frame["n"][1] * z)))
+    return set(out)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_grid_build_matches_geometry(seed):
+    r = rng(100 + seed)
+    n = int(r.integers(1, 64))
+    centers = r.random((n, 3))
+    v = (np.repeat(centers, 3, axis=0) + 0.3 * (r.random((3 * n, 3)) - 0.5)).reshape(n, 9)
+    v = v.astype(np.float32)
+    f = orc.grid_frame(v, 0.5 + seed)
+    assert f["m"] <= 8 ** 3 * 4
+    counts, offs, refs = orc.grid_build(v, f)
+    # offsets are the exclusive scan of counts, R is the total
+    assert offs[0] == 0 and np.array_equal(np.diff(offs.astype(np.int64)), counts)
+    assert offs[-1] == len(refs) == counts.sum()
+    cells_of = {t: set() for t in range(n)}
+    for c in range(f["m"]):
+        seg = refs[offs[c]:offs[c + 1]]
+        assert np.all(np.diff(seg.astype(np.int64)) > 0), "refs ascending, no duplicates"
+        for t in seg:
+            cells_of[int(t)].add(c)
+    for t in range(n):
+        strict = _geo_cells(f, v[t], 1e-4)
+        closed = _geo_cells(f, v[t], -1e-4)
+        assert strict <= cells_of[t] <= closed, t
+        lo, hi = orc.tri_cell_range(f, v[t])
+        assert len(cells_of[t]) == int(np.prod(hi - lo + 1))
+
+
+def test_grid_build_brute_force_tiny():
+    """Cell membership by brute force over all (cell, triangle) pairs."""
+    v = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0],
+                  [0.5, 0.5, 0.5, 0.9, 0.5, 0.5, 0.5, 0.9, 0.6],
+                  [0.1, 0.8, 0.2, 0.15, 0.85, 0.25, 0.12, 0.9, 0.3]], np.float32)
+    f = orc.grid_frame(v, 20.0)
+    counts, offs, refs = orc.grid_build(v, f)
+    for c in range(f["m"]):
+        seg = set(int(x) for x in refs[offs[c]:offs[c + 1]])
+        want = {t for t in range(len(v)) if c in _geo_cells(f, v[t], 1e-4)}
+        maybe = {t for t in range(len(v)) if c in _geo_cells(f, v[t], -1e-4)}
+        assert want <= seg <= maybe
+
+
+# ---------------------------------------------------------------------------
+# a6: camera
+# ---------------------------------------------------------------------------
+def test_camera_closed_form():
+    W, H = 640, 480
+    cam = orc.camera_lookat((1, 2, 3), (1, 2, 10), (0, 1, 0), 60.0, W, H)
+    base, nu, nv = (np.array(cam[k], np.float64) for k in ("base", "nu", "nv"))
+    sy = math.tan(math.radians(30.0))
+    sx = sy * W / H
+    # centre of the image looks straight ahead at distance zNear
+    assert np.allclose(base + 0.5 * nu + 0.5 * nv, [0, 0, 1], atol=1e-6)
+    assert np.isclose(np.linalg.norm(nu), 2 * sx, rtol=1e-6)
+    assert np.isclose(np.linalg.norm(nv), 2 * sy, rtol=1e-6)
+    assert abs(nu @ nv) < 1e-6 and abs(nu @ [0, 0, 1]) < 1e-6
+    assert nv[1] > 0          # image y grows upward
+    rays = orc.primary_rays(cam)
+    d = np.stack([rays["dx"], rays["dy"], rays["dz"]], 1).astype(np.float64)
+    assert np.allclose(d[0], base, atol=1e-7)       # pixel (0,0) = corner = baseDir
+    y, x = np.divmod(np.arange(W * H), W)
+    want = base + np.outer(x / W, nu) + np.outer(y / H, nv)
+    assert np.allclose(d, want, atol=1e-6)
+    assert np.all(rays["tmin"] == np.float32(2e-4)) and np.all(np.isinf(rays["tmax"]))
+
+
+# ---------------------------------------------------------------------------
+# a9: Moller-Trumbore
+# ---------------------------------------------------------------------------
+TRI = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32)
+
+
+def test_mt_known_answers():
+    hit, t, u, v = orc.mt(TRI, (1 / 3, 1 / 3, 1.0), (0, 0, -1))
+    assert hit and abs(t - 1) < 1e-6 and abs(u - 1 / 3) < 1e-6 and abs(v - 1 / 3) < 1e-6
+    assert orc.mt(TRI, (0, 0, 2), (0, 0, -1))[1:] == (2.0, 0.0, 0.0)      # vertex 0
+    assert orc.mt(TRI, (1, 0, 2), (0, 0, -1))[1:] == (2.0, 1.0, 0.0)      # vertex 1: u=1
+    assert orc.mt(TRI, (0, 1, 2), (0, 0, -1))[1:] == (2.0, 0.0, 1.0)      # vertex 2: v=1
+    assert not orc.mt(TRI, (0.2, 0.2, 0.0), (1, 0, 0))[0]                 # in-plane ray
+    assert not orc.mt(TRI, (0.2, 0.2, -1.0), (0, 0, -1), 0.0)[0]          # behind origin
+    assert orc.mt(TRI, (0.2, 0.2, -1.0), (0, 0, 1))[0]                    # back face hit
+    assert not orc.mt(TRI, (0.6, 0.6, 1.0), (0, 0, -1))[0]                # outside u+v>1
+    assert not orc.mt(TRI, (0.2, 0.2, 1.0), (0, 0, -1), 0.0, 0.5)[0]      # beyond tmax
+
+
+def test_mt_barycentrics_reconstruct_hit_point():
+    r = rng(5)
+    for _ in range(500):
+        tri = r.random(9).astype(np.float32)
+        w = r.dirichlet([1, 1, 1])
+        p = w[0] * tri[0:3] + w[1] * tri[3:6] + w[2] * tri[6:9]
+        o = (p + r.normal(size=3)).astype(np.float32)
+        d = (p - o).astype(np.float32)
+        hit, t, u, v = orc.mt(tri, o, d, -1.0, 10.0)
+        if min(w) < 1e-3:
+            continue
+        assert hit
+        q = (1 - u - v) * tri[0:3] + u * tri[3:6] + v * tri[6:9]
+        assert np.allclose(q, o + t * d, atol=2e-5)
+        # binary32 MT loses ~|o-v0|/edge ulps; a u/v swap would be off by ~0.1
+        assert abs(u - w[1]) < 2e-3 and abs(v - w[2]) < 2e-3
+
+
+# ---------------------------------------------------------------------------
+# a7/a8: clip + DDA
+# ---------------------------------------------------------------------------
+def _frame_unit(n):
+    """A grid over [0,1]^3 built from a box whose padding we undo by hand."""
+    f = orc.grid_frame_from_box([0, 0, 0], [1, 1, 1], n ** 3, 1.0)
+    return f
+
+
+def test_dda_axis_aligned_sequence():
+    f = _frame_unit(8)
+    nx, ny, nz = f["n"]
+    cy = cz = 3
+    y = f["bmin"][1] + (cy + 0.5) * f["cell"][1]
+    z = f["bmin"][2] + (cz + 0.5) * f["cell"][2]
+    cells = orc.dda_cells(f, (-1.0, y, z), (1.0, 0.0, 0.0))
+    want = [x + nx * (cy + ny * cz) for x in range(nx)]
+    assert list(cells) == want
+    cells = orc.dda_cells(f, (2.0, y, z), (-1.0, -0.0, 0.0))
+    assert list(cells) == want[::-1]
+
+
+def _segment_cells(f, o, d, t0, t1, tol):
+    """Cells whose box the FP64 segment crosses with overlap length > tol."""
+    bmin = f["bmin"].astype(np.float64)
+    cell = (f["bmax"].astype(np.float64) - bmin) / f["n"]
+    out = set()
+    for z in range(f["n"][2]):
+        for y in range(f["n"][1]):
+            for x in range(f["n"][0]):
+                lo = bmin + cell * (x, y, z)
+                hi = lo + cell
+                a, b = t0, t1
+                for k in range(3):
+                    if d[k] == 0:
+                        if not (lo[k] <= o[k] <= hi[k]):
+                            a, b = 1, 0
+                        continue
+                    ta, tb = (lo[k] - o[k]) / d[k], (hi[k] - o[k]) / d[k]
+                    a, b = max(a, min(ta, tb)), min(b, max(ta, tb))
+                if b - a > tol:
+                    out.add(int(x + f["n"][0] * (y + f["n"][1] * z)))
+    return out
+
+
+def test_dda_visits_cells_of_segment():
+    f = _frame_unit(6)
+    r = rng(9)
+    nx, ny = f["n"][0], f["n"][1]
+    for _ in range(300):
+        o = r.random(3).astype(np.float32)
+        d = r.normal(size=3).astype(np.float32)
+        cells = orc.dda_cells(f, o, d, 0.0, np.inf)
+        # consecutive cells are face neighbours
+        xyz = np.stack([cells % nx, (cells // nx) % ny, cells // (nx * ny)], 1)
+        assert np.all(np.abs(np.diff(xyz, axis=0)).sum(1) == 1)
+        assert len(set(cells.tolist())) == len(cells)
+        o64, d64 = o.astype(np.float64), d.astype(np.float64)
+        tb = [((f["bmin"][k] if d64[k] < 0 else f["bmax"][k]) - o64[k]) / d64[k] for k in range(3)]
+        t_exit = min(tb)
+        strict = _segment_cells(f, o64, d64, 0.0, t_exit, 1e-6)
+        closed = _segment_cells(f, o64, d64, 0.0, t_exit, -1e-6)
+        assert strict <= set(cells.tolist()) <= closed
+
+
+# ---------------------------------------------------------------------------
+# closest / any hit vs FP64 brute force
+# ---------------------------------------------------------------------------
+def _rays(n, seed, spread=3.0):
+    r = rng(seed)
+    o = ((r.random((n, 3)) - 0.5) * spread + 0.5).astype(np.float32)
+    target = r.random((n, 3)).astype(np.float32)
+    d = (target - o).astype(np.float32)
+    d[::7] = r.normal(size=(len(d[::7]), 3))
+    return {"ox": o[:, 0], "oy": o[:, 1], "oz": o[:, 2], "dx": d[:, 0], "dy": d[:, 1],
+            "dz": d[:, 2], "tmin": np.full(n, 2e-4, np.float32),
+            "tmax": np.full(n, np.inf, np.float32)}
+
+
+def _edge_ambiguous(b):
+    w = np.stack([b["u"], b["v"], 1 - b["u"] - b["v"]], 1)
+    return np.abs(w).min(1) < 1e-5
+
+
+@pytest.mark.parametrize("lam", [1.0, 5.0])
+def test_trace_closest_matches_brute_force(lam):
+    v, _ = grt.gen_spheres(SEED_C1, 16, 0.05, 0.15)
+    f = orc.grid_frame(v, lam)
+    _, offs, refs = orc.grid_build(v, f)
+    rays = _rays(6000, int(lam))
+    o = orc.trace_closest(f, offs, refs, v, rays)
+    b = orc.brute_closest_f64(v, rays)
+    assert (b["id"] >= 0).sum() > 1000
+    near_tie = np.abs(b["t2"] - b["t1"]) <= 1e-6 * np.abs(b["t1"])
+    excused = near_tie | _edge_ambiguous(b)
+    bad = (o["id"] != b["id"]) & ~excused
+    assert bad.sum() == 0, np.nonzero(bad)[0][:10]
+    ok = (o["id"] >= 0) & (o["id"] == b["id"])
+    assert np.max(np.abs(o["t"][ok] - b["t1"][ok]) / b["t1"][ok]) < 1e-5
+
+
+def test_trace_any_matches_brute_force():
+    v, _ = grt.gen_spheres(SEED_C1, 16, 0.05, 0.15)
+    f = orc.grid_frame(v, 5.0)
+    _, offs, refs = orc.grid_build(v, f)
+    rays = _rays(6000, 77)
+    rays["tmin"][:] = 0.0
+    rays["tmax"] = (rng(3).random(6000) * 2).astype(np.float32)
+    o = orc.trace_any(f, offs, refs, v, rays)
+    b = orc.brute_any_f64(v, rays)
+    tn = b["t_near"]
+    boundary = np.isfinite(tn) & (np.abs(tn - rays["tmax"]) <= 1e-5 * np.abs(tn))
+    bad = (o["occluded"] != b["occluded"]) & ~boundary
+    assert bad.sum() == 0
+    assert 100 < o["occluded"].sum() < 5900
+
+
+def test_closest_needs_cell_exit_bound():
+    """Two triangles: the far one is listed in the first cell the ray visits,
+    the near one only in a later cell.  The cell-exit bound makes the traversal
+    return the near one (reference intersectTriangles, ray.tfar = cell exit)."""
+    far = [0.05, 0.0, 0.9, 0.05, 1.0, 0.9, 0.05, 0.0, -0.9]   # spans x=0.05 plane, long in z
+    near = [0.6, 0.4, 0.45, 0.6, 0.6, 0.45, 0.6, 0.5, 0.55]
+    v = np.array([far, near, [0, 0, 0, 1, 1, 1, 1, 0, 1]], np.float32)
+    f = orc.grid_frame(v, 4.0)
+    _, offs, refs = orc.grid_build(v, f)
+    rays = {"ox": np.float32([0.9]), "oy": np.float32([0.5]), "oz": np.float32([0.5]),
+            "dx": np.float32([-1.0]), "dy": np.float32([0.0]), "dz": np.float32([0.0]),
+            "tmin": np.float32([0.0]), "tmax": np.float32([np.inf])}
+    o = orc.trace_closest(f, offs, refs, v, rays)
+    b = orc.brute_closest_f64(v, rays)
+    assert o["id"][0] == b["id"][0] == 1
+
+
+# ---------------------------------------------------------------------------
+# a10-a12: secondary rays
+# ---------------------------------------------------------------------------
+def _one_hit_scene():
+    v = np.array([[-1, 0, -1, 1, 0, -1, 0, 0, 1]], np.float32)      # y=0 plane
+    n = np.array([[0, 1, 0, 0, 1, 0, 0, 1, 0]], np.float32)
+    rays = {"ox": np.float32([0.1, 0.2]), "oy": np.float32([1.0, 1.0]),
+            "oz": np.float32([0.0, 0.1]), "dx": np.float32([0.3, 0.0]),
+            "dy": np.float32([-1.0, 1.0]), "dz": np.float32([0.2, 0.0]),
+            "tmin": np.float32([0, 0]), "tmax": np.float32([np.inf, np.inf])}
+    f = orc.grid_frame(v, 5.0)
+    _, offs, refs = orc.grid_build(v, f)
+    hits = orc.trace_closest(f, offs, refs, v, rays)
+    return v, n, rays, hits
+
+
+def test_reflection_closed_form():
+    v, n, rays, hits = _one_hit_scene()
+    assert hits["id"][0] == 0 and hits["id"][1] == -1
+    r = orc.gen_reflection(rays, hits, n, 1e-3)
+    d = np.array([0.3, -1.0, 0.2])
+    rd = np.array([r["dx"][0], r["dy"][0], r["dz"][0]])
+    assert np.allclose(rd, [0.3, 1.0, 0.2], atol=1e-6)        # d - 2(d.n)n
+    assert np.isclose(np.linalg.norm(rd), np.linalg.norm(d), rtol=1e-6)
+    h = np.array([0.1, 0.0, 0.0]) + d * hits["t"][0] + np.array([0, 0, 0])
+    assert np.allclose([r["ox"][0], r["oy"][0], r["oz"][0]], [0.4, 1e-3, 0.2], atol=1e-6)
+    assert r["tmin"][1] > r["tmax"][1]                          # miss -> inactive
+
+
+def test_shadow_closed_form():
+    v, n, rays, hits = _one_hit_scene()
+    light = (3.0, 5.0, -2.0)
+    s = orc.gen_shadow(rays, hits, n, light, 1e-3)
+    hp = np.array([0.4, 0.0, 0.2])
+    L = np.array(light) - hp
+    sd = np.array([s["dx"][0], s["dy"][0], s["dz"][0]])
+    assert np.allclose(sd, L / np.linalg.norm(L), atol=1e-6)
+    assert np.isclose(s["tmax"][0], np.linalg.norm(L), rtol=1e-6) and s["tmin"][0] == 0
+    # light below the surface: no shadow ray
+    s2 = orc.gen_shadow(rays, hits, n, (0.0, -5.0, 0.0), 1e-3)
+    assert s2["tmin"][0] > s2["tmax"][0]
+
+
+def test_shading_normal_interpolates():
+    n9 = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1], np.float32)
+    assert np.allclose(orc.shading_normal(n9, 0, 0), [1, 0, 0])
+    assert np.allclose(orc.shading_normal(n9, 1, 0), [0, 1, 0])
+    assert np.allclose(orc.shading_normal(n9, 0, 1), [0, 0, 1])
+    assert np.allclose(orc.shading_normal(n9, 1 / 3, 1 / 3), np.ones(3) / math.sqrt(3), atol=1e-6)
+
+
+def test_sincos_turns():
+    t = np.linspace(0, 1, 4001, endpoint=False).astype(np.float32)
+    sc = np.array([orc.sincos_turns(float(x)) for x in t])
+    assert np.max(np.abs(sc[:, 0] - np.sin(2 * np.pi * t.astype(np.float64)))) < 2e-6
+    assert np.max(np.abs(sc[:, 1] - np.cos(2 * np.pi * t.astype(np.float64)))) < 2e-6
+
+
+def test_ao_cosine_weighted():
+    v, n, rays, hits = _one_hit_scene()
+    spp = 20000
+    a = orc.gen_ao({k: x[:1] for k, x in rays.items()}, {k: x[:1] for k, x in hits.items()}, n,
+                   spp, 42, 1e-3, 0.05)
+    d = np.stack([a["dx"], a["dy"], a["dz"]], 1).astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-6)
+    cos = d[:, 1]
+    assert cos.min() >= 0.0
+    assert abs(cos.mean() - 2 / 3) < 0.01          # E[cos] of a cosine lobe
+    assert abs((cos ** 2).mean() - 0.5) < 0.01     # E[cos^2] = 1/2
+    assert abs(d[:, 0].mean()) < 0.02 and abs(d[:, 2].mean()) < 0.02
+    assert np.all(a["tmax"] == np.float32(0.05))
+
+
+def test_uniform24_is_uniform():
+    u = np.array([orc.uniform24(7, i, 0) for i in range(20000)])
+    assert 0.0 <= u.min() and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.01
+    assert np.all(u * 2 ** 24 == np.floor(u * 2 ** 24))
+
+
+def test_transform_is_rigid():
+    v, nrm, centers = grt.gen_stadium(SEED_C2, 4, 64, 4 * 64 - 2)
+    xf = grt.gen_cluster_transforms(SEED_C2, 3, centers, 0.05)
+    tv, tn = orc.transform(v, nrm, xf, 64, 4 * 64 - 2)
+    m = xf.reshape(-1, 3, 4).astype(np.float64)
+    for t in (0, 70, 200, 253):
+        c = t // 64
+        p = v[t].reshape(3, 3).astype(np.float64)
+        want = p @ m[c, :, :3].T + m[c, :, 3]
+        assert np.allclose(tv[t].reshape(3, 3), want, atol=1e-6)
+        assert np.allclose(np.linalg.norm(p[1] - p[0]),
+                           np.linalg.norm(tv[t].reshape(3, 3)[1] - tv[t].reshape(3, 3)[0]),
+                           rtol=1e-4)
+    assert np.array_equal(tv[-2:], v[-2:])          # ground plane untouched
