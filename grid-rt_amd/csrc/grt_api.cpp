@@ -116,6 +116,8 @@ int grid_reserve_refs(grt_scene_t* sc, uint32_t refs)
     const uint64_t list_cap = cap / (kSortSmallCells + 1) + 1;
     if ((rc = sc->sort_list.reserve(sc->ctx, list_cap * 4, false)) != GRT_OK)
         return rc;
+    if ((rc = sc->trir.reserve(sc->ctx, cap * 48, false)) != GRT_OK)
+        return rc;
     sc->refs_cap = (uint32_t)cap;
     sc->list_cap = (uint32_t)list_cap;
     return GRT_OK;
@@ -171,10 +173,14 @@ static int enqueue_scatter_sort(grt_scene_t* sc)
                                             sc->small_counters.as<uint32_t>(), s),
                         "grid scatter")) != GRT_OK)
         return rc;
-    return hip_check(launch_grid_sort(fr, sc->offsets.as<uint32_t>(), sc->refs.as<uint32_t>(),
-                                      nullptr, sc->sort_list.as<uint32_t>(), sc->list_cap,
-                                      sc->small_counters.as<uint32_t>() + 1, sc->cells_cap, s),
-                     "grid sort");
+    if ((rc = hip_check(launch_grid_sort(fr, sc->offsets.as<uint32_t>(), sc->refs.as<uint32_t>(),
+                                         nullptr, sc->sort_list.as<uint32_t>(), sc->list_cap,
+                                         sc->small_counters.as<uint32_t>() + 1, sc->cells_cap, s),
+                        "grid sort")) != GRT_OK)
+        return rc;
+    return hip_check(launch_gather_refs(fr, sc->refs.as<uint32_t>(), sc->tri.as<float4>(),
+                                        sc->trir.as<float4>(), sc->refs_cap, s),
+                     "grid records");
 }
 
 int scene_common_buffers(grt_scene_t* sc)
@@ -271,6 +277,19 @@ int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any)
         return set_error(GRT_INVALID_PARAM, "block sizes must be 64..256 in steps of 64");
     ctx->block_closest = closest;
     ctx->block_any = any;
+    return GRT_OK;
+}
+
+int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill)
+{
+    if (!ctx)
+        return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: NULL context");
+    if (mode != GRT_TRACE_SIMPLE && mode != GRT_TRACE_PERSISTENT)
+        return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: unknown mode %d", mode);
+    if (refill < 1 || refill > 64)
+        return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: refill must be 1..64");
+    ctx->trace_mode = mode;
+    ctx->refill = refill;
     return GRT_OK;
 }
 
@@ -643,6 +662,9 @@ int grt_grid_upload(grt_scene sc, const grt_grid_info* info, const uint32_t* off
     if (e == hipSuccess && r)
         e = hipMemcpyAsync(sc->refs.ptr, refs_host, (size_t)r * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
+        e = launch_gather_refs(sc->frame.as<DevFrame>(), sc->refs.as<uint32_t>(),
+                               sc->tri.as<float4>(), sc->trir.as<float4>(), sc->refs_cap, s);
+    if (e == hipSuccess)
         e = hipStreamSynchronize(s);
     if (e != hipSuccess)
         return hip_check(e, "grid upload");
@@ -720,6 +742,7 @@ static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint
     a.offsets = sc->offsets.as<uint32_t>();
     a.refs = sc->refs.as<uint32_t>();
     a.tri = sc->tri.as<float4>();
+    a.trir = sc->trir.as<float4>();
     a.n = n;
     a.image_width = image_width;
     a.counters = counters;
@@ -735,9 +758,14 @@ int grt_trace_closest(grt_scene sc, const grt_rays* rays, const grt_hits* hits, 
     if (!rays_ok(rays) || !hits_ok(hits))
         return set_error(GRT_INVALID_PARAM, "grt_trace_closest: NULL ray or hit buffer");
     const TraceArgs a = trace_args(sc, n, image_width, counters_dev);
-    return hip_check(launch_trace_closest(a, to_soa(rays),
-                                          HitSoA{hits->id, hits->t, hits->u, hits->v},
-                                          sc->ctx->block_closest, sc->ctx->stream),
+    const HitSoA h{hits->id, hits->t, hits->u, hits->v};
+    grt_context_t* ctx = sc->ctx;
+    if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
+        return hip_check(launch_trace_persistent(a, to_soa(rays), h, nullptr, false,
+                                                 sc->small_counters.as<uint32_t>() + 2,
+                                                 ctx->block_closest, ctx->refill, ctx->stream),
+                         "trace_closest");
+    return hip_check(launch_trace_closest(a, to_soa(rays), h, ctx->block_closest, ctx->stream),
                      "trace_closest");
 }
 
@@ -750,8 +778,13 @@ int grt_trace_any(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev, uin
     if (!rays_ok(rays) || !occluded_dev)
         return set_error(GRT_INVALID_PARAM, "grt_trace_any: NULL ray or output buffer");
     const TraceArgs a = trace_args(sc, n, image_width, counters_dev);
-    return hip_check(launch_trace_any(a, to_soa(rays), occluded_dev, sc->ctx->block_any,
-                                      sc->ctx->stream),
+    grt_context_t* ctx = sc->ctx;
+    if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
+        return hip_check(launch_trace_persistent(a, to_soa(rays), HitSoA{}, occluded_dev, true,
+                                                 sc->small_counters.as<uint32_t>() + 2,
+                                                 ctx->block_any, ctx->refill, ctx->stream),
+                         "trace_any");
+    return hip_check(launch_trace_any(a, to_soa(rays), occluded_dev, ctx->block_any, ctx->stream),
                      "trace_any");
 }
 

@@ -20,6 +20,7 @@
 namespace grt {
 
 constexpr uint32_t kSmallCells = 16;
+constexpr uint32_t kHugeCells = 65536;
 constexpr uint32_t kSortTile = 16384;   // LDS tile of the per-cell sort (64 KB)
 
 // ---------------------------------------------------------------------------
@@ -182,20 +183,27 @@ __global__ __launch_bounds__(64) void k_frame(const float* __restrict__ partial,
                                               uint32_t n_tris, float lambda, uint32_t cells_cap,
                                               DevFrame* __restrict__ fr)
 {
-    if (threadIdx.x != 0)
-        return;
-    float mn[3], mx[3];
-    for (int d = 0; d < 3; ++d) {
-        mn[d] = partial[d];
-        mx[d] = partial[3 + d];
-    }
-    for (uint32_t p = 1; p < nparts; ++p)
+    // One wave reduces the block partials (min/max are exact, so the order
+    // does not matter), then lane 0 computes the frame.
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t p = threadIdx.x; p < nparts; p += 64)
         for (int d = 0; d < 3; ++d) {
             const float a = partial[6 * p + d], b = partial[6 * p + 3 + d];
             mn[d] = a < mn[d] ? a : mn[d];
             mx[d] = b > mx[d] ? b : mx[d];
         }
-    frame_from_box(mn, mx, n_tris, lambda, cells_cap, fr);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float a = __shfl_xor(mn[d], off);
+            const float b = __shfl_xor(mx[d], off);
+            mn[d] = a < mn[d] ? a : mn[d];
+            mx[d] = b > mx[d] ? b : mx[d];
+        }
+    if (threadIdx.x == 0)
+        frame_from_box(mn, mx, n_tris, lambda, cells_cap, fr);
 }
 
 __global__ void k_frame_box(float x0, float y0, float z0, float x1, float y1, float z1,
@@ -265,8 +273,14 @@ __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ 
     const CellBox b = tri_cells(fr, verts + 9ull * t);
     const uint32_t k = (uint32_t)b.w[0] * (uint32_t)b.w[1] * (uint32_t)b.w[2];
     if (k > kSmallCells) {
-        if (!SCATTER)
-            big_list[atomicAdd(big_count, 1u)] = t;
+        // Medium triangles fill the list from the front, huge ones from the
+        // back (each triangle is on at most one list, so they never meet).
+        if (!SCATTER) {
+            if (k > kHugeCells)
+                big_list[n_tris - 1u - atomicAdd(big_count + 3, 1u)] = t;
+            else
+                big_list[atomicAdd(big_count, 1u)] = t;
+        }
         return;
     }
     const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
@@ -279,7 +293,51 @@ __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ 
         }
 }
 
-// Triangles on the big list: one block per triangle, threads stride its cells.
+// Huge triangles (more than kHugeCells cells, e.g. ground planes): the whole
+// grid strides over the cells of each one in turn.
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void k_grid_pass_huge(const DevFrame* __restrict__ fr,
+                                                        const float* __restrict__ verts,
+                                                        uint32_t n_tris,
+                                                        uint32_t* __restrict__ counts,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        uint32_t* __restrict__ refs,
+                                                        const uint32_t* __restrict__ big_list,
+                                                        const uint32_t* __restrict__ big_count)
+{
+    const uint32_t bad = SCATTER ? (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)
+                                 : (kStatusFrameInvalid | kStatusCellsOver);
+    if (fr->status & bad)
+        return;
+    const uint32_t nh = big_count[3];
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = 0; i < nh; ++i) {
+        const uint32_t t = big_list[n_tris - 1u - i];
+        const CellBox b = tri_cells(fr, verts + 9ull * t);
+        const uint32_t wx = (uint32_t)b.w[0], wy = (uint32_t)b.w[1];
+        const uint32_t k = wx * wy * (uint32_t)b.w[2];
+        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < k; j += stride) {
+            const uint32_t x = j % wx;
+            const uint32_t yz = j / wx;
+            const uint32_t y = yz % wy;
+            const uint32_t z = yz / wy;
+            const uint32_t c = ((uint32_t)b.lo[0] + x) +
+                               nx * (((uint32_t)b.lo[1] + y) + ny * ((uint32_t)b.lo[2] + z));
+            cell_op<SCATTER>(c, t, counts, offsets, refs);
+        }
+    }
+}
+
+__global__ void k_zero_build_counters(uint32_t* __restrict__ c)
+{
+    if (threadIdx.x == 0) {
+        c[0] = 0u;
+        c[3] = 0u;
+    }
+}
+
+// Triangles on the medium list: one block per triangle, threads stride its cells.
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_grid_pass_big(const DevFrame* __restrict__ fr,
                                                        const float* __restrict__ verts,
@@ -691,6 +749,26 @@ __global__ __launch_bounds__(256) void k_transform_pack(uint32_t n_tris, uint32_
     pack_one(t, v, tri);
 }
 
+// Per-reference triangle records: trir[k] = tri[refs[k]], so a cell's
+// triangles are contiguous and traversal needs no index indirection.
+__global__ __launch_bounds__(256) void k_gather_refs(const DevFrame* __restrict__ fr,
+                                                     const uint32_t* __restrict__ refs,
+                                                     const float4* __restrict__ tri,
+                                                     float4* __restrict__ trir)
+{
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    const uint64_t r_total = fr->r;
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < r_total;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = refs[k];
+        const float4 a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+        trir[3 * k] = a;
+        trir[3 * k + 1] = b;
+        trir[3 * k + 2] = c;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, uint64_t n)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -732,13 +810,16 @@ hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_
                              uint32_t* counts, uint32_t* big_list, uint32_t* big_count,
                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, big_count, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(k_zero_build_counters, dim3(1), dim3(64), 0, s, big_count);
     hipLaunchKernelGGL(k_grid_pass<false>, dim3(grid_for(n_tris, 256)), dim3(256), 0, s, fr, verts,
                        n_tris, counts, (const uint32_t*)nullptr, (uint32_t*)nullptr, big_list,
                        big_count);
     hipLaunchKernelGGL(k_grid_pass_big<false>, dim3(2048), dim3(256), 0, s, fr, verts, counts,
                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)big_list,
                        (const uint32_t*)big_count);
+    hipLaunchKernelGGL(k_grid_pass_huge<false>, dim3(2048), dim3(256), 0, s, fr, verts, n_tris,
+                       counts, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                       (const uint32_t*)big_list, (const uint32_t*)big_count);
     return hipGetLastError();
 }
 
@@ -779,6 +860,8 @@ hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t 
                        n_tris, counts, offsets, refs, (uint32_t*)nullptr, (uint32_t*)nullptr);
     hipLaunchKernelGGL(k_grid_pass_big<true>, dim3(2048), dim3(256), 0, s, fr, verts, counts, offsets,
                        refs, big_list, big_count);
+    hipLaunchKernelGGL(k_grid_pass_huge<true>, dim3(2048), dim3(256), 0, s, fr, verts, n_tris,
+                       counts, offsets, refs, big_list, big_count);
     return hipGetLastError();
 }
 
@@ -792,6 +875,14 @@ hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_
                        refs, list, list_cap, list_count);
     hipLaunchKernelGGL(k_sort_large, dim3(1024), dim3(256), 0, s, fr, offsets, refs,
                        (const uint32_t*)list, list_cap, (const uint32_t*)list_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_refs(const DevFrame* fr, const uint32_t* refs, const float4* tri,
+                              float4* trir, uint32_t refs_cap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather_refs, dim3(grid_for(refs_cap, 256, 16384)), dim3(256), 0, s, fr,
+                       refs, tri, trir);
     return hipGetLastError();
 }
 

@@ -43,6 +43,8 @@ struct grt_context_t {
     uint64_t bytes_used = 0;
     uint32_t block_closest = 128;
     uint32_t block_any = 128;
+    int trace_mode = GRT_TRACE_PERSISTENT;
+    uint32_t refill = 16;
     std::set<grt_scene_t*> scenes;
 };
 
@@ -57,7 +59,7 @@ struct grt_scene_t {
     size_t xf_host_bytes = 0;
     // Grid.
     grt::DevMem frame, partial, counts, offsets, refs, block_sums, big_list, sort_list,
-        small_counters;
+        small_counters, trir;
     uint32_t cells_cap = 0, refs_cap = 0, list_cap = 0;
     bool grid_valid = false;
     bool info_stale = false;
@@ -68,7 +70,7 @@ struct grt_scene_t {
     {
         return {&verts,  &normals,    &rest_verts, &rest_normals, &tri,       &xf,
                 &frame,  &partial,    &counts,     &offsets,      &refs,      &block_sums,
-                &big_list, &sort_list, &small_counters};
+                &big_list, &sort_list, &small_counters, &trir};
     }
     ~grt_scene_t()
     {

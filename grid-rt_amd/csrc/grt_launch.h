@@ -21,8 +21,10 @@ hipError_t launch_frame_from_verts(const float* verts, uint32_t n_tris, float la
 hipError_t launch_frame_from_box(const float bmin[3], const float bmax[3], uint32_t n_tris,
                                  float lambda, uint32_t cells_cap, DevFrame* fr, hipStream_t s);
 
-// Count pass: per-cell counts (atomics) and the list of triangles spanning
-// more than kSmallCells cells.  counts must be zero on entry.
+// Count pass: per-cell counts (atomics) and the lists of triangles spanning
+// more than 16 cells (medium: from the front of big_list, count in
+// big_count[0]; huge, over 65536 cells: from the back, count in big_count[3]).
+// big_list holds n_tris entries; counts must be zero on entry.
 hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_tris,
                              uint32_t* counts, uint32_t* big_list, uint32_t* big_count,
                              hipStream_t s);
@@ -42,6 +44,10 @@ hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t 
 hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_t* refs,
                             uint32_t* scratch, uint32_t* list, uint32_t list_cap,
                             uint32_t* list_count, uint32_t cells_cap, hipStream_t s);
+
+// trir[k] = tri[refs[k]] for k < fr->r (3 float4 per reference).
+hipError_t launch_gather_refs(const DevFrame* fr, const uint32_t* refs, const float4* tri,
+                              float4* trir, uint32_t refs_cap, hipStream_t s);
 
 // Pack 9-float triangles into the 48-byte traversal record
 // {v0.xyz, e1.x} {e1.yz, e2.xy} {e2.z, id, 0, 0}.
@@ -63,6 +69,7 @@ struct TraceArgs {
     const uint32_t* offsets;
     const uint32_t* refs;
     const float4* tri;
+    const float4* trir;     // per-reference records, 3 float4 per reference
     uint32_t n;
     uint32_t image_width;   // 0: lane i traces ray i; else 8x8 pixel tiles per wave
     uint32_t* counters;     // NULL or 2 per ray {cells, tests}
@@ -71,6 +78,12 @@ hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const Hi
                                 uint32_t block, hipStream_t s);
 hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occluded,
                             uint32_t block, hipStream_t s);
+// Persistent traversal (closest hit, or any hit when `any`): lanes refill
+// from the slot counter (one uint32, zeroed by the launcher) once `refill`
+// lanes of a wave are idle.
+hipError_t launch_trace_persistent(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                                   uint8_t* occluded, bool any, uint32_t* slot_counter,
+                                   uint32_t block, uint32_t refill, hipStream_t s);
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_gen_shadow(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
                              const float* normals, float lx, float ly, float lz, float offset,

@@ -1,272 +1,18 @@
-// grt_trace.hip -- one-ray-per-lane 3D-DDA traversal with Moller-Trumbore on
-// gfx950, plus ray generation from a pinhole camera and from hit records.
+// grt_raygen.hip -- ray generation on gfx950: primary rays from a pinhole
+// camera, and shadow, mirror-reflection and ambient-occlusion rays from
+// closest-hit records; plus the headlight shading of the reference CUDA tracer.
 //
 // Reference semantics:
-//   clip + DDA set-up   AabbIntersection.cpp:37-63, UniformGridAccStruct.cpp:118-149
-//   DDA loop            UniformGridAccStruct.cpp:160-195 (single-axis step,
-//                       ties to y then z)
-//   cell loop           UniformGridAccStruct.cpp:362-378 (hits bounded by the
-//                       current cell's exit)
-//   Moller-Trumbore     RayTracing_kernel.cu:56-89 (no determinant epsilon)
-//   any hit             shadow_FRAG.glsl:122-245
 //   primary rays        PinholeCamera.cpp:131-137, RayTracing_kernel.cu:145-161
-//   shadow/reflection   shadow_FRAG.glsl:259-270, Sample.cpp:7-52,95-114
-//
-// Data in HBM: rays and hits are structures of arrays (coalesced 4-byte lanes);
-// the grid is offsets[m+1] + refs[R] (uint32); triangles are 48-byte records
-// {v0.xyz, e1.x}{e1.yz, e2.xy}{e2.z, id, -, -} read with three 16-byte loads.
+//   shadow rays         shadow_FRAG.glsl:259-270, SimplePointLight.cpp:18-37
+//   reflection rays     Sample.cpp:7-11,44-52, traversalIntersection_FRAG.glsl:285-296
+//   shading normal      Sample.cpp:100-114, RayTracing_kernel.cu:92-102
+//   headlight shading   RayTracing_kernel.cu:104-120
+// Every kernel is one output ray (or pixel) per lane, structure-of-arrays in
+// and out, and follows the CPU oracle's operation order bit for bit.
 #include "grt_launch.h"
 
 namespace grt {
-
-// Lane -> ray index.  With image_width != 0 each wave64 covers an 8x8 pixel
-// tile of a row-major image so neighbouring lanes trace neighbouring pixels.
-__device__ __forceinline__ uint32_t ray_index(uint32_t img_w, uint32_t n, bool& valid)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (img_w == 0) {
-        valid = g < n;
-        return g;
-    }
-    const uint32_t wave = g >> 6, lane = g & 63u;
-    const uint32_t tiles_x = (img_w + 7u) >> 3;
-    const uint32_t tx = wave % tiles_x, ty = wave / tiles_x;
-    const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
-    const uint32_t i = py * img_w + px;
-    valid = (px < img_w) && (i < n);
-    return i;
-}
-
-static uint32_t ray_grid(uint32_t n, uint32_t img_w, uint32_t block)
-{
-    if (img_w == 0)
-        return (n + block - 1) / block;
-    const uint32_t h = (n + img_w - 1) / img_w;
-    const uint64_t threads = (uint64_t)((img_w + 7u) >> 3) * ((h + 7u) >> 3) * 64u;
-    return (uint32_t)((threads + block - 1) / block);
-}
-
-// DDA state in registers.
-struct Dda {
-    int32_t cx, cy, cz;
-    int32_t sx, sy, sz;
-    int32_t ox, oy, oz;   // out-of-grid limits
-    float tx, ty, tz;     // t at the next boundary per axis
-    float dtx, dty, dtz;
-};
-
-// Ray::update + clipRay + DDA set-up.  Returns false when the ray misses the
-// grid box (or its interval is empty).
-__device__ __forceinline__ bool dda_setup(const DevFrame* __restrict__ fr, float o0, float o1,
-                                          float o2, float d0, float d1, float d2, float tmin,
-                                          float tmax, Dda& s)
-{
-    const float i0 = 1.0f / d0, i1 = 1.0f / d1, i2 = 1.0f / d2;
-    float tn = tmin, tf = tmax;
-    {
-        const float a = (fr->bmin[0] - o0) * i0, b = (fr->bmax[0] - o0) * i0;
-        tn = tmax_f(tn, tmin_f(a, b));
-        tf = tmin_f(tf, tmax_f(a, b));
-    }
-    {
-        const float a = (fr->bmin[1] - o1) * i1, b = (fr->bmax[1] - o1) * i1;
-        tn = tmax_f(tn, tmin_f(a, b));
-        tf = tmin_f(tf, tmax_f(a, b));
-    }
-    {
-        const float a = (fr->bmin[2] - o2) * i2, b = (fr->bmax[2] - o2) * i2;
-        tn = tmax_f(tn, tmin_f(a, b));
-        tf = tmin_f(tf, tmax_f(a, b));
-    }
-    if (tn > tf)
-        return false;
-    const float p0 = o0 + d0 * tn, p1 = o1 + d1 * tn, p2 = o2 + d2 * tn;
-    s.cx = vox(p0, fr->bmin[0], fr->inv[0], fr->n[0]);
-    s.cy = vox(p1, fr->bmin[1], fr->inv[1], fr->n[1]);
-    s.cz = vox(p2, fr->bmin[2], fr->inv[2], fr->n[2]);
-    const bool n0 = __builtin_signbit(d0), n1 = __builtin_signbit(d1), n2 = __builtin_signbit(d2);
-    s.sx = n0 ? -1 : 1;
-    s.sy = n1 ? -1 : 1;
-    s.sz = n2 ? -1 : 1;
-    s.ox = n0 ? -1 : fr->n[0];
-    s.oy = n1 ? -1 : fr->n[1];
-    s.oz = n2 ? -1 : fr->n[2];
-    const float w0 = (float)(s.cx + (n0 ? 0 : 1)) * fr->cell[0] + fr->bmin[0];
-    const float w1 = (float)(s.cy + (n1 ? 0 : 1)) * fr->cell[1] + fr->bmin[1];
-    const float w2 = (float)(s.cz + (n2 ? 0 : 1)) * fr->cell[2] + fr->bmin[2];
-    s.tx = (w0 - o0) * i0;
-    s.ty = (w1 - o1) * i1;
-    s.tz = (w2 - o2) * i2;
-    s.dtx = fabsf(fr->cell[0] * i0);
-    s.dty = fabsf(fr->cell[1] * i1);
-    s.dtz = fabsf(fr->cell[2] * i2);
-    return true;
-}
-
-// Single-axis advance with the reference tie order.  False when leaving.
-__device__ __forceinline__ bool dda_step(Dda& s)
-{
-    if (s.tx < s.ty && s.tx < s.tz) {
-        s.cx += s.sx;
-        if (s.cx == s.ox)
-            return false;
-        s.tx += s.dtx;
-    } else if (s.ty < s.tz) {
-        s.cy += s.sy;
-        if (s.cy == s.oy)
-            return false;
-        s.ty += s.dty;
-    } else {
-        s.cz += s.sz;
-        if (s.cz == s.oz)
-            return false;
-        s.tz += s.dtz;
-    }
-    return true;
-}
-
-// Moller-Trumbore on a packed 48-byte record; returns the inside test and
-// writes t,u,v.
-__device__ __forceinline__ bool mt_test(const float4* __restrict__ tri, uint32_t r, float o0,
-                                        float o1, float o2, float d0, float d1, float d2,
-                                        float& t, float& u, float& v)
-{
-    const float4 A = tri[3ull * r];
-    const float4 B = tri[3ull * r + 1];
-    const float4 C = tri[3ull * r + 2];
-    const float v0x = A.x, v0y = A.y, v0z = A.z;
-    const float e1x = A.w, e1y = B.x, e1z = B.y;
-    const float e2x = B.z, e2y = B.w, e2z = C.x;
-    const float px = __builtin_fmaf(d1, e2z, -(d2 * e2y));
-    const float py = __builtin_fmaf(d2, e2x, -(d0 * e2z));
-    const float pz = __builtin_fmaf(d0, e2y, -(d1 * e2x));
-    const float det = dot3(px, py, pz, e1x, e1y, e1z);
-    const float inv = 1.0f / det;
-    const float tvx = o0 - v0x, tvy = o1 - v0y, tvz = o2 - v0z;
-    const float qx = __builtin_fmaf(tvy, e1z, -(tvz * e1y));
-    const float qy = __builtin_fmaf(tvz, e1x, -(tvx * e1z));
-    const float qz = __builtin_fmaf(tvx, e1y, -(tvy * e1x));
-    u = dot3(px, py, pz, tvx, tvy, tvz) * inv;
-    v = dot3(qx, qy, qz, d0, d1, d2) * inv;
-    t = dot3(qx, qy, qz, e2x, e2y, e2z) * inv;
-    return (u >= 0.0f) && (v >= 0.0f) && ((u + v) <= 1.0f);
-}
-
-__device__ __forceinline__ bool grid_ok(const DevFrame* __restrict__ fr)
-{
-    return (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)) == 0;
-}
-
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a, RaySoA rays, HitSoA hits)
-{
-    bool valid;
-    const uint32_t i = ray_index(a.image_width, a.n, valid);
-    if (!valid)
-        return;
-    const DevFrame* __restrict__ fr = a.fr;
-    const uint32_t* __restrict__ offsets = a.offsets;
-    const uint32_t* __restrict__ refs = a.refs;
-    const float4* __restrict__ tri = a.tri;
-    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
-    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
-    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
-    int32_t id = -1;
-    float best = INFINITY, bu = 0.0f, bv = 0.0f;
-    uint32_t cells = 0, tests = 0;
-    Dda s;
-    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
-        const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
-        for (;;) {
-            if (COUNT)
-                ++cells;
-            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
-            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            const uint32_t b = offsets[ci], e = offsets[ci + 1];
-            if (b < e) {
-                const float bound = exit_t * kExitScale;
-                for (uint32_t k = b; k < e; ++k) {
-                    const uint32_t r = refs[k];
-                    if (COUNT)
-                        ++tests;
-                    float t, u, v;
-                    if (mt_test(tri, r, o0, o1, o2, d0, d1, d2, t, u, v) && t > tmin &&
-                        t <= tmax && t <= bound && t < best) {
-                        best = t;
-                        id = (int32_t)r;
-                        bu = u;
-                        bv = v;
-                    }
-                }
-                if (id >= 0)
-                    break;
-            }
-            if (exit_t > tmax)
-                break;
-            if (!dda_step(s))
-                break;
-        }
-    }
-    hits.id[i] = id;
-    hits.t[i] = best;
-    hits.u[i] = bu;
-    hits.v[i] = bv;
-    if (COUNT) {
-        a.counters[2ull * i] = cells;
-        a.counters[2ull * i + 1] = tests;
-    }
-}
-
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_trace_any(TraceArgs a, RaySoA rays,
-                                                   uint8_t* __restrict__ occluded)
-{
-    bool valid;
-    const uint32_t i = ray_index(a.image_width, a.n, valid);
-    if (!valid)
-        return;
-    const DevFrame* __restrict__ fr = a.fr;
-    const uint32_t* __restrict__ offsets = a.offsets;
-    const uint32_t* __restrict__ refs = a.refs;
-    const float4* __restrict__ tri = a.tri;
-    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
-    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
-    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
-    bool occ = false;
-    uint32_t cells = 0, tests = 0;
-    Dda s;
-    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
-        const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
-        for (;;) {
-            if (COUNT)
-                ++cells;
-            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
-            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            const uint32_t b = offsets[ci], e = offsets[ci + 1];
-            for (uint32_t k = b; k < e; ++k) {
-                if (COUNT)
-                    ++tests;
-                float t, u, v;
-                if (mt_test(tri, refs[k], o0, o1, o2, d0, d1, d2, t, u, v) && t > tmin &&
-                    t <= tmax) {
-                    occ = true;
-                    break;
-                }
-            }
-            if (occ)
-                break;
-            if (exit_t > tmax)
-                break;
-            if (!dda_step(s))
-                break;
-        }
-    }
-    occluded[i] = occ ? 1 : 0;
-    if (COUNT) {
-        a.counters[2ull * i] = cells;
-        a.counters[2ull * i + 1] = tests;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Ray generation
@@ -518,32 +264,6 @@ __global__ __launch_bounds__(256) void k_shade_headlight(uint32_t n, RaySoA in, 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
-                                uint32_t block, hipStream_t s)
-{
-    if (a.n == 0)
-        return hipSuccess;
-    const uint32_t g = ray_grid(a.n, a.image_width, block);
-    if (a.counters)
-        hipLaunchKernelGGL(k_trace_closest<true>, dim3(g), dim3(block), 0, s, a, rays, hits);
-    else
-        hipLaunchKernelGGL(k_trace_closest<false>, dim3(g), dim3(block), 0, s, a, rays, hits);
-    return hipGetLastError();
-}
-
-hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occluded,
-                            uint32_t block, hipStream_t s)
-{
-    if (a.n == 0)
-        return hipSuccess;
-    const uint32_t g = ray_grid(a.n, a.image_width, block);
-    if (a.counters)
-        hipLaunchKernelGGL(k_trace_any<true>, dim3(g), dim3(block), 0, s, a, rays, occluded);
-    else
-        hipLaunchKernelGGL(k_trace_any<false>, dim3(g), dim3(block), 0, s, a, rays, occluded);
-    return hipGetLastError();
-}
-
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s)
 {
     const uint32_t n = cam.width * cam.height;
@@ -598,3 +318,4 @@ hipError_t launch_shade_headlight(uint32_t n, const RaySoA& in, const HitSoAIn& 
 }
 
 }  // namespace grt
+

@@ -1,0 +1,685 @@
+// grt_traverse.hip -- one-ray-per-lane 3D-DDA traversal of the uniform grid
+// with Moller-Trumbore intersection, on gfx950 (wave64).
+//
+// Reference semantics:
+//   clip + DDA set-up   AabbIntersection.cpp:37-63, UniformGridAccStruct.cpp:118-149
+//   DDA loop            UniformGridAccStruct.cpp:160-195 (single-axis step,
+//                       ties to y then z)
+//   cell loop           UniformGridAccStruct.cpp:362-378 (hits bounded by the
+//                       current cell's exit)
+//   Moller-Trumbore     RayTracing_kernel.cu:56-89 (no determinant epsilon)
+//   any hit             shadow_FRAG.glsl:122-245
+//
+// Data in HBM: rays and hits are structures of arrays (coalesced 4-byte
+// lanes); the grid is offsets[m+1] (uint32) plus one 48-byte record per
+// reference, trir[k] = {v0.xyz, e1.x}{e1.yz, e2.xy}{e2.z, id, -, -}, stored
+// contiguously per cell, so a cell's triangles stream without an index
+// indirection.  Each traversal loop loads the next record while the current
+// one is tested.
+//
+// Two schedules with identical per-ray results:
+//   k_trace_closest / k_trace_any   one ray per lane for the whole launch;
+//       when every active lane of a wave is in the same cell the records are
+//       read with wave-uniform (scalar) loads;
+//   k_trace_persistent              persistent waves that refill idle lanes
+//       from a shared slot counter and advance each lane by one work item
+//       (one triangle test or one cell) per iteration.
+#include "grt_launch.h"
+
+namespace grt {
+
+// ---------------------------------------------------------------------------
+// Ray indexing
+// ---------------------------------------------------------------------------
+
+// Slot -> ray index.  With img_w != 0 each group of 64 consecutive slots is an
+// 8x8 pixel tile of a row-major image of that width.
+__device__ __forceinline__ uint32_t slot_ray(uint32_t slot, uint32_t img_w, uint32_t n,
+                                             bool& valid)
+{
+    if (img_w == 0) {
+        valid = slot < n;
+        return slot;
+    }
+    const uint32_t wave = slot >> 6, lane = slot & 63u;
+    const uint32_t tiles_x = (img_w + 7u) >> 3;
+    const uint32_t tx = wave % tiles_x, ty = wave / tiles_x;
+    const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
+    const uint32_t i = py * img_w + px;
+    valid = (px < img_w) && (i < n);
+    return i;
+}
+
+static uint64_t slot_count(uint32_t n, uint32_t img_w)
+{
+    if (img_w == 0)
+        return n;
+    const uint64_t h = (n + (uint64_t)img_w - 1) / img_w;
+    return (uint64_t)((img_w + 7u) >> 3) * ((h + 7u) >> 3) * 64u;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// ---------------------------------------------------------------------------
+// DDA
+// ---------------------------------------------------------------------------
+struct Dda {
+    int32_t cx, cy, cz;
+    int32_t sx, sy, sz;
+    int32_t ox, oy, oz;   // out-of-grid limits
+    float tx, ty, tz;     // t at the next boundary per axis
+    float dtx, dty, dtz;
+};
+
+__device__ __forceinline__ bool grid_ok(const DevFrame* __restrict__ fr)
+{
+    return (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)) == 0;
+}
+
+// Ray::update + clipRay + DDA set-up.  False when the ray misses the grid box
+// (or its interval is empty).
+__device__ __forceinline__ bool dda_setup(const DevFrame* __restrict__ fr, float o0, float o1,
+                                          float o2, float d0, float d1, float d2, float tmin,
+                                          float tmax, Dda& s)
+{
+    const float i0 = 1.0f / d0, i1 = 1.0f / d1, i2 = 1.0f / d2;
+    float tn = tmin, tf = tmax;
+    {
+        const float a = (fr->bmin[0] - o0) * i0, b = (fr->bmax[0] - o0) * i0;
+        tn = tmax_f(tn, tmin_f(a, b));
+        tf = tmin_f(tf, tmax_f(a, b));
+    }
+    {
+        const float a = (fr->bmin[1] - o1) * i1, b = (fr->bmax[1] - o1) * i1;
+        tn = tmax_f(tn, tmin_f(a, b));
+        tf = tmin_f(tf, tmax_f(a, b));
+    }
+    {
+        const float a = (fr->bmin[2] - o2) * i2, b = (fr->bmax[2] - o2) * i2;
+        tn = tmax_f(tn, tmin_f(a, b));
+        tf = tmin_f(tf, tmax_f(a, b));
+    }
+    if (tn > tf)
+        return false;
+    const float p0 = o0 + d0 * tn, p1 = o1 + d1 * tn, p2 = o2 + d2 * tn;
+    s.cx = vox(p0, fr->bmin[0], fr->inv[0], fr->n[0]);
+    s.cy = vox(p1, fr->bmin[1], fr->inv[1], fr->n[1]);
+    s.cz = vox(p2, fr->bmin[2], fr->inv[2], fr->n[2]);
+    const bool n0 = __builtin_signbit(d0), n1 = __builtin_signbit(d1), n2 = __builtin_signbit(d2);
+    s.sx = n0 ? -1 : 1;
+    s.sy = n1 ? -1 : 1;
+    s.sz = n2 ? -1 : 1;
+    s.ox = n0 ? -1 : fr->n[0];
+    s.oy = n1 ? -1 : fr->n[1];
+    s.oz = n2 ? -1 : fr->n[2];
+    const float w0 = (float)(s.cx + (n0 ? 0 : 1)) * fr->cell[0] + fr->bmin[0];
+    const float w1 = (float)(s.cy + (n1 ? 0 : 1)) * fr->cell[1] + fr->bmin[1];
+    const float w2 = (float)(s.cz + (n2 ? 0 : 1)) * fr->cell[2] + fr->bmin[2];
+    s.tx = (w0 - o0) * i0;
+    s.ty = (w1 - o1) * i1;
+    s.tz = (w2 - o2) * i2;
+    s.dtx = fabsf(fr->cell[0] * i0);
+    s.dty = fabsf(fr->cell[1] * i1);
+    s.dtz = fabsf(fr->cell[2] * i2);
+    return true;
+}
+
+// Single-axis advance with the reference tie order.  False when leaving.
+__device__ __forceinline__ bool dda_step(Dda& s)
+{
+    if (s.tx < s.ty && s.tx < s.tz) {
+        s.cx += s.sx;
+        if (s.cx == s.ox)
+            return false;
+        s.tx += s.dtx;
+    } else if (s.ty < s.tz) {
+        s.cy += s.sy;
+        if (s.cy == s.oy)
+            return false;
+        s.ty += s.dty;
+    } else {
+        s.cz += s.sz;
+        if (s.cz == s.oz)
+            return false;
+        s.tz += s.dtz;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Moller-Trumbore
+// ---------------------------------------------------------------------------
+
+// Split in two so the IEEE division runs only when some lane of the wave may
+// hit.  mt_pre evaluates the numerators in the specification's order;
+// mt_maybe is a conservative filter that returns false only when mt_exact is
+// certainly a miss for the window (tmin, ub], ub an upper bound on accepted t
+// (tmax, cell exit bound, best so far).  With r = RN(1/det), each of
+// u = RN(un*r), v, t is within 2^-23 relative of un/det, vn/det, tn/det (no
+// underflow inside the guarded magnitude ranges), so the margins 2^-40, 2^-16
+// and 2^-20 keep every rejection on the miss side of the rounded comparison.
+struct MtPre {
+    float det, un, vn, tn;
+};
+
+__device__ __forceinline__ MtPre mt_pre(const float4 A, const float4 B, const float4 C, float o0,
+                                        float o1, float o2, float d0, float d1, float d2)
+{
+    const float v0x = A.x, v0y = A.y, v0z = A.z;
+    const float e1x = A.w, e1y = B.x, e1z = B.y;
+    const float e2x = B.z, e2y = B.w, e2z = C.x;
+    const float px = __builtin_fmaf(d1, e2z, -(d2 * e2y));
+    const float py = __builtin_fmaf(d2, e2x, -(d0 * e2z));
+    const float pz = __builtin_fmaf(d0, e2y, -(d1 * e2x));
+    MtPre m;
+    m.det = dot3(px, py, pz, e1x, e1y, e1z);
+    const float tvx = o0 - v0x, tvy = o1 - v0y, tvz = o2 - v0z;
+    const float qx = __builtin_fmaf(tvy, e1z, -(tvz * e1y));
+    const float qy = __builtin_fmaf(tvz, e1x, -(tvx * e1z));
+    const float qz = __builtin_fmaf(tvx, e1y, -(tvy * e1x));
+    m.un = dot3(px, py, pz, tvx, tvy, tvz);
+    m.vn = dot3(qx, qy, qz, d0, d1, d2);
+    m.tn = dot3(qx, qy, qz, e2x, e2y, e2z);
+    return m;
+}
+
+__device__ __forceinline__ bool mt_maybe(const MtPre& m, float tmin, float ub)
+{
+    const float ad = fabsf(m.det);
+    if (!(ad >= 0x1p-100f && ad <= 0x1p100f))
+        return true;
+    const bool neg = m.det < 0.0f;
+    const float us = neg ? -m.un : m.un;
+    const float vs = neg ? -m.vn : m.vn;
+    const float ts = neg ? -m.tn : m.tn;
+    const float thr = ad * 0x1p-40f;
+    if (us < -thr || vs < -thr)
+        return false;
+    if ((us + vs) > ad * (1.0f + 0x1p-16f))
+        return false;
+    if (ub >= 0x1p-20f && ub <= 0x1p40f && ts > (ub * ad) * (1.0f + 0x1p-20f))
+        return false;
+    if (tmin == 0.0f) {
+        if (ts <= 0.0f)
+            return false;
+    } else if (tmin >= 0x1p-20f && tmin <= 0x1p20f && ts <= (tmin * ad) * (1.0f - 0x1p-20f)) {
+        return false;
+    }
+    return true;
+}
+
+// The specification's division and inside test on the numerators.
+__device__ __forceinline__ bool mt_exact(const MtPre& m, float& t, float& u, float& v)
+{
+    const float inv = 1.0f / m.det;
+    u = m.un * inv;
+    v = m.vn * inv;
+    t = m.tn * inv;
+    return (u >= 0.0f) && (v >= 0.0f) && ((u + v) <= 1.0f);
+}
+
+// Closest-hit candidate update for one record: the hit window of
+// intersectTriangles -- (tmin, tmax], at most the cell exit bound, nearer than
+// the best so far.  The triangle id is stored in the record's C.y.
+__device__ __forceinline__ void closest_test(const float4 A, const float4 B, const float4 C,
+                                             float o0, float o1, float o2, float d0, float d1,
+                                             float d2, float tmin, float tmax, float bound,
+                                             int32_t& id, float& best, float& bu, float& bv)
+{
+    const MtPre m = mt_pre(A, B, C, o0, o1, o2, d0, d1, d2);
+    if (mt_maybe(m, tmin, tmin_f(tmin_f(tmax, bound), best))) {
+        float t, u, v;
+        if (mt_exact(m, t, u, v) && t > tmin && t <= tmax && t <= bound && t < best) {
+            best = t;
+            id = (int32_t)__float_as_uint(C.y);
+            bu = u;
+            bv = v;
+        }
+    }
+}
+
+__device__ __forceinline__ bool any_test(const float4 A, const float4 B, const float4 C, float o0,
+                                         float o1, float o2, float d0, float d1, float d2,
+                                         float tmin, float tmax)
+{
+    const MtPre m = mt_pre(A, B, C, o0, o1, o2, d0, d1, d2);
+    if (!mt_maybe(m, tmin, tmax))
+        return false;
+    float t, u, v;
+    return mt_exact(m, t, u, v) && t > tmin && t <= tmax;
+}
+
+// Read-only view in the constant address space: with a wave-uniform index the
+// compiler reads it through the scalar unit (loads only).
+typedef const __attribute__((address_space(4))) float* KF4;
+
+__device__ __forceinline__ float4 kload4(KF4 p, uint64_t i)
+{
+    return make_float4(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]);
+}
+
+// ---------------------------------------------------------------------------
+// One ray per lane
+// ---------------------------------------------------------------------------
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a, RaySoA rays, HitSoA hits)
+{
+    bool valid;
+    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
+    if (!valid)
+        return;
+    const DevFrame* __restrict__ fr = a.fr;
+    const uint32_t* __restrict__ offsets = a.offsets;
+    const float4* __restrict__ trir = a.trir;
+    const KF4 ktr = (KF4)a.trir;
+    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
+    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
+    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
+    int32_t id = -1;
+    float best = INFINITY, bu = 0.0f, bv = 0.0f;
+    uint32_t cells = 0, tests = 0;
+    Dda s;
+    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
+        const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+        for (;;) {
+            if (COUNT)
+                ++cells;
+            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+            const uint32_t b = offsets[ci], e = offsets[ci + 1];
+            if (b < e) {
+                const float bound = exit_t * kExitScale;
+                const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+                const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
+                if (__ballot(b != b0 || e != e0) == 0) {
+                    // Every active lane is in this cell: wave-uniform records.
+                    float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
+                           C = kload4(ktr, 3ull * b0 + 2);
+                    for (uint32_t k = b0; k < e0; ++k) {
+                        const float4 cA = A, cB = B, cC = C;
+                        if (k + 1 < e0) {
+                            const uint64_t q = 3ull * (k + 1);
+                            A = kload4(ktr, q);
+                            B = kload4(ktr, q + 1);
+                            C = kload4(ktr, q + 2);
+                        }
+                        closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id,
+                                     best, bu, bv);
+                    }
+                } else {
+                    float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
+                    for (uint32_t k = b; k < e; ++k) {
+                        const float4 cA = A, cB = B, cC = C;
+                        if (k + 1 < e) {
+                            const uint64_t q = 3ull * (k + 1);
+                            A = trir[q];
+                            B = trir[q + 1];
+                            C = trir[q + 2];
+                        }
+                        closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id,
+                                     best, bu, bv);
+                    }
+                }
+                if (COUNT)
+                    tests += e - b;
+                if (id >= 0)
+                    break;
+            }
+            if (exit_t > tmax)
+                break;
+            if (!dda_step(s))
+                break;
+        }
+    }
+    hits.id[i] = id;
+    hits.t[i] = best;
+    hits.u[i] = bu;
+    hits.v[i] = bv;
+    if (COUNT) {
+        a.counters[2ull * i] = cells;
+        a.counters[2ull * i + 1] = tests;
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_any(TraceArgs a, RaySoA rays,
+                                                   uint8_t* __restrict__ occluded)
+{
+    bool valid;
+    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
+    if (!valid)
+        return;
+    const DevFrame* __restrict__ fr = a.fr;
+    const uint32_t* __restrict__ offsets = a.offsets;
+    const float4* __restrict__ trir = a.trir;
+    const KF4 ktr = (KF4)a.trir;
+    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
+    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
+    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
+    bool occ = false;
+    uint32_t cells = 0, tests = 0;
+    Dda s;
+    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
+        const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+        for (;;) {
+            if (COUNT)
+                ++cells;
+            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+            const uint32_t b = offsets[ci], e = offsets[ci + 1];
+            if (b < e) {
+                const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+                const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
+                if (__ballot(b != b0 || e != e0) == 0) {
+                    float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
+                           C = kload4(ktr, 3ull * b0 + 2);
+                    for (uint32_t k = b0; k < e0; ++k) {
+                        const float4 cA = A, cB = B, cC = C;
+                        if (k + 1 < e0) {
+                            const uint64_t q = 3ull * (k + 1);
+                            A = kload4(ktr, q);
+                            B = kload4(ktr, q + 1);
+                            C = kload4(ktr, q + 2);
+                        }
+                        if (COUNT)
+                            ++tests;
+                        if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax)) {
+                            occ = true;
+                            break;
+                        }
+                    }
+                } else {
+                    float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
+                    for (uint32_t k = b; k < e; ++k) {
+                        const float4 cA = A, cB = B, cC = C;
+                        if (k + 1 < e) {
+                            const uint64_t q = 3ull * (k + 1);
+                            A = trir[q];
+                            B = trir[q + 1];
+                            C = trir[q + 2];
+                        }
+                        if (COUNT)
+                            ++tests;
+                        if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax)) {
+                            occ = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (occ)
+                break;
+            if (exit_t > tmax)
+                break;
+            if (!dda_step(s))
+                break;
+        }
+    }
+    occluded[i] = occ ? 1 : 0;
+    if (COUNT) {
+        a.counters[2ull * i] = cells;
+        a.counters[2ull * i + 1] = tests;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent waves with lane refill
+// ---------------------------------------------------------------------------
+
+// Each wave keeps its lanes busy by fetching new rays (one atomic per wave
+// for all idle lanes, once at least `refill` lanes are idle or the wave has
+// drained) and advances every lane by one work item per iteration -- one
+// triangle test, or one cell (DDA step + cell range).  Lanes never wait for the
+// longest ray of their wave and inactive ray slots cost one iteration.  Every
+// wave exits once the slot counter passes n_slots and its lanes are done.
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a, RaySoA rays, HitSoA hits,
+                                                          uint8_t* __restrict__ occluded,
+                                                          uint32_t* __restrict__ slot_counter,
+                                                          uint32_t n_slots, uint32_t refill)
+{
+    const DevFrame* __restrict__ fr = a.fr;
+    const uint32_t* __restrict__ offsets = a.offsets;
+    const float4* __restrict__ trir = a.trir;
+    const bool ok = grid_ok(fr);
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    const uint32_t lane = threadIdx.x & 63u;
+
+    int32_t ray = -1;         // ray index traced by this lane, -1 = idle
+    bool exhausted = false;   // no slots left for this lane
+    float o0 = 0, o1 = 0, o2 = 0, d0 = 0, d1 = 0, d2 = 0, tmin = 0, tmax = 0;
+    Dda s = {};
+    uint32_t k = 0, e = 0, cells = 0, tests = 0;
+    bool loaded = false;
+    float exit_t = 0.0f, bound = 0.0f;
+    int32_t id = -1;
+    float best = INFINITY, bu = 0.0f, bv = 0.0f;
+    float4 A = {}, B = {}, C = {};   // record k of the current cell, loaded ahead
+
+    for (;;) {
+        // ---- refill idle lanes -------------------------------------------
+        const bool need = (ray < 0) && !exhausted;
+        const uint64_t need_mask = __ballot(need);
+        const uint64_t busy_mask = __ballot(ray >= 0);
+        const uint32_t n_need = (uint32_t)__popcll(need_mask);
+        if (need_mask != 0 && (n_need >= refill || busy_mask == 0)) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(need_mask);
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(slot_counter, n_need);
+            base = __builtin_amdgcn_readlane(base, leader);
+            if (need) {
+                const uint32_t slot = base + lanes_below(need_mask);
+                if (slot >= n_slots) {
+                    exhausted = true;
+                } else {
+                    bool valid;
+                    const uint32_t ri = slot_ray(slot, a.image_width, a.n, valid);
+                    if (valid) {
+                        o0 = rays.ox[ri];
+                        o1 = rays.oy[ri];
+                        o2 = rays.oz[ri];
+                        d0 = rays.dx[ri];
+                        d1 = rays.dy[ri];
+                        d2 = rays.dz[ri];
+                        tmin = rays.tmin[ri];
+                        tmax = rays.tmax[ri];
+                        id = -1;
+                        best = INFINITY;
+                        bu = 0.0f;
+                        bv = 0.0f;
+                        cells = 0;
+                        tests = 0;
+                        loaded = false;
+                        k = e = 0;
+                        ray = (int32_t)ri;
+                        if (!(ok && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s))) {
+                            if (ANY) {
+                                occluded[ri] = 0;
+                            } else {
+                                hits.id[ri] = -1;
+                                hits.t[ri] = INFINITY;
+                                hits.u[ri] = 0.0f;
+                                hits.v[ri] = 0.0f;
+                            }
+                            if (COUNT) {
+                                a.counters[2ull * ri] = 0;
+                                a.counters[2ull * ri + 1] = 0;
+                            }
+                            ray = -1;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (__ballot(!exhausted) == 0)
+                break;
+            continue;
+        }
+        if (ray < 0)
+            continue;
+
+        // ---- one work item ---------------------------------------------------
+        bool finish = false;
+        if (k < e) {
+            const float4 cA = A, cB = B, cC = C;
+            ++k;
+            if (k < e) {
+                A = trir[3ull * k];
+                B = trir[3ull * k + 1];
+                C = trir[3ull * k + 2];
+            }
+            if (COUNT)
+                ++tests;
+            if (ANY) {
+                if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax)) {
+                    id = 1;
+                    finish = true;
+                }
+            } else {
+                closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu,
+                             bv);
+            }
+        } else {
+            if (loaded) {
+                if (!ANY && id >= 0)
+                    finish = true;
+                else if (exit_t > tmax || !dda_step(s))
+                    finish = true;
+            }
+            if (!finish) {
+                if (COUNT)
+                    ++cells;
+                exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+                const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+                k = offsets[ci];
+                e = offsets[ci + 1];
+                bound = exit_t * kExitScale;
+                loaded = true;
+                if (k < e) {
+                    A = trir[3ull * k];
+                    B = trir[3ull * k + 1];
+                    C = trir[3ull * k + 2];
+                }
+            }
+        }
+        if (finish) {
+            const uint32_t ri = (uint32_t)ray;
+            if (ANY) {
+                occluded[ri] = id >= 0 ? 1 : 0;
+            } else {
+                hits.id[ri] = id;
+                hits.t[ri] = best;
+                hits.u[ri] = bu;
+                hits.v[ri] = bv;
+            }
+            if (COUNT) {
+                a.counters[2ull * ri] = cells;
+                a.counters[2ull * ri + 1] = tests;
+            }
+            ray = -1;
+        }
+    }
+}
+
+__global__ void k_fill_u32_one(uint32_t* __restrict__ p)
+{
+    if (threadIdx.x == 0)
+        p[0] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static uint32_t ray_grid(uint32_t n, uint32_t img_w, uint32_t block)
+{
+    return (uint32_t)((slot_count(n, img_w) + block - 1) / block);
+}
+
+hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                                uint32_t block, hipStream_t s)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    const uint32_t g = ray_grid(a.n, a.image_width, block);
+    if (a.counters)
+        hipLaunchKernelGGL(k_trace_closest<true>, dim3(g), dim3(block), 0, s, a, rays, hits);
+    else
+        hipLaunchKernelGGL(k_trace_closest<false>, dim3(g), dim3(block), 0, s, a, rays, hits);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occluded,
+                            uint32_t block, hipStream_t s)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    const uint32_t g = ray_grid(a.n, a.image_width, block);
+    if (a.counters)
+        hipLaunchKernelGGL(k_trace_any<true>, dim3(g), dim3(block), 0, s, a, rays, occluded);
+    else
+        hipLaunchKernelGGL(k_trace_any<false>, dim3(g), dim3(block), 0, s, a, rays, occluded);
+    return hipGetLastError();
+}
+
+// Resident blocks on the device for a persistent kernel at a block size.
+template <class K>
+static uint32_t resident_blocks(K kernel, uint32_t block)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return (uint32_t)cus * (uint32_t)per_cu;
+}
+
+hipError_t launch_trace_persistent(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                                   uint8_t* occluded, bool any, uint32_t* slot_counter,
+                                   uint32_t block, uint32_t refill, hipStream_t s)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    const uint64_t n_slots = slot_count(a.n, a.image_width);
+    if (n_slots > 0xffffffffull)
+        return hipErrorInvalidValue;
+    refill = refill < 1 ? 1 : (refill > 64 ? 64 : refill);
+    hipLaunchKernelGGL(k_fill_u32_one, dim3(1), dim3(64), 0, s, slot_counter);
+    const uint32_t need = (uint32_t)((n_slots + block - 1) / block);
+#define GRT_LAUNCH_P(ANYV, CNTV)                                                                \
+    do {                                                                                        \
+        auto kern = k_trace_persistent<ANYV, CNTV>;                                             \
+        uint32_t g = resident_blocks(kern, block);                                              \
+        g = g < need ? g : need;                                                                \
+        hipLaunchKernelGGL(kern, dim3(g), dim3(block), 0, s, a, rays, hits, occluded,           \
+                           slot_counter, (uint32_t)n_slots, refill);                            \
+    } while (0)
+    if (any) {
+        if (a.counters)
+            GRT_LAUNCH_P(true, true);
+        else
+            GRT_LAUNCH_P(true, false);
+    } else {
+        if (a.counters)
+            GRT_LAUNCH_P(false, true);
+        else
+            GRT_LAUNCH_P(false, false);
+    }
+#undef GRT_LAUNCH_P
+    return hipGetLastError();
+}
+
+}  // namespace grt

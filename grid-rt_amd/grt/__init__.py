@@ -69,6 +69,7 @@ _SIGS = {
     "grt_synchronize": (ctypes.c_int, [_P]),
     "grt_memory_used": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "grt_set_block_size": (ctypes.c_int, [_P, _U32, _U32]),
+    "grt_set_trace_mode": (ctypes.c_int, [_P, ctypes.c_int, _U32]),
     "grt_scene_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "grt_scene_destroy": (ctypes.c_int, [_P]),
     "grt_scene_upload_triangles": (ctypes.c_int, [_P, _P, _U32]),
@@ -291,6 +292,9 @@ class Context:
 
     def set_block_size(self, closest: int, any_hit: int):
         check(lib().grt_set_block_size(self.h, closest, any_hit))
+
+    def set_trace_mode(self, persistent: bool = True, refill: int = 16):
+        check(lib().grt_set_trace_mode(self.h, 1 if persistent else 0, refill))
 
     def gen_primary(self, cam: Camera, out: RayBatch):
         check(lib().grt_gen_primary(self.h, ctypes.byref(cam), ctypes.byref(out.c())))

@@ -83,6 +83,14 @@ int grt_memory_used(grt_context ctx, uint64_t* bytes);
  * cudaSetBlockSize (RayTracing.h:39) was a no-op in the reference. */
 int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
 
+/* Traversal scheduling.  GRT_TRACE_SIMPLE: one ray per lane for the whole
+ * launch.  GRT_TRACE_PERSISTENT (default): persistent waves whose idle lanes
+ * fetch new rays once `refill` (1..64) lanes of a wave are idle.  Results are
+ * identical; only the schedule differs. */
+#define GRT_TRACE_SIMPLE     0
+#define GRT_TRACE_PERSISTENT 1
+int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill);
+
 /* ---- scene ------------------------------------------------------------ */
 int grt_scene_create(grt_context ctx, grt_scene* out);
 int grt_scene_destroy(grt_scene scene);

@@ -55,40 +55,47 @@ def check_grid(ctx, v, n, lam):
     return sc, fr, ooffs, orefs
 
 
-def trace_both(sc, fr, offs, refs, v, rays, image_width=0):
-    n = rays["ox"].shape[0]
-    rb = grt.RayBatch.from_numpy(rays)
-    hb = grt.HitBatch.empty(n)
-    import torch
-    cnt = torch.empty(2 * n, dtype=torch.int32, device="cuda:0")
-    sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
-    g = hb.numpy()
-    gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-    o = orc.trace_closest(fr, offs, refs, v, rays, counters=True)
-    return g, gc, o
+# Both traversal schedules must give identical per-ray results and counters.
+MODES = [(False, 16), (True, 16), (True, 1)]
 
 
 def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
-    g, gc, o = trace_both(sc, fr, offs, refs, v, rays, image_width)
-    for k in ("id", "t", "u", "v"):
-        assert_same(g[k], o[k], f"hit.{k}")
-    assert_same(gc[:, 0], o["cells"], "cells visited")
-    assert_same(gc[:, 1], o["tests"], "triangle tests")
+    import torch
+    n = rays["ox"].shape[0]
+    o = orc.trace_closest(fr, offs, refs, v, rays, counters=True)
+    rb = grt.RayBatch.from_numpy(rays)
+    for persistent, refill in MODES:
+        sc.ctx.set_trace_mode(persistent, refill)
+        hb = grt.HitBatch.empty(n)
+        cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
+        sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
+        g = hb.numpy()
+        gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+        tag = f"[persistent={persistent} refill={refill}]"
+        for k in ("id", "t", "u", "v"):
+            assert_same(g[k], o[k], f"hit.{k} {tag}")
+        assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
+        assert_same(gc[:, 1], o["tests"], f"triangle tests {tag}")
+    sc.ctx.set_trace_mode(True, 16)
     return g, o
 
 
 def check_any(sc, fr, offs, refs, v, rays):
     import torch
     n = rays["ox"].shape[0]
-    rb = grt.RayBatch.from_numpy(rays)
-    occ = torch.empty(n, dtype=torch.uint8, device="cuda:0")
-    cnt = torch.empty(2 * n, dtype=torch.int32, device="cuda:0")
-    sc.trace_any(rb, occ, counters=cnt)
     o = orc.trace_any(fr, offs, refs, v, rays, counters=True)
-    gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-    assert_same(occ.cpu().numpy(), o["occluded"], "occluded")
-    assert_same(gc[:, 0], o["cells"], "cells visited")
-    assert_same(gc[:, 1], o["tests"], "triangle tests")
+    rb = grt.RayBatch.from_numpy(rays)
+    for persistent, refill in MODES:
+        sc.ctx.set_trace_mode(persistent, refill)
+        occ = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
+        cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
+        sc.trace_any(rb, occ, counters=cnt)
+        gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+        tag = f"[persistent={persistent} refill={refill}]"
+        assert_same(occ.cpu().numpy(), o["occluded"], f"occluded {tag}")
+        assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
+        assert_same(gc[:, 1], o["tests"], f"triangle tests {tag}")
+    sc.ctx.set_trace_mode(True, 16)
     return o
 
 
