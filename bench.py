@@ -60,6 +60,12 @@ def parse():
     p.add_argument("--refill", type=int, default=16)
     p.add_argument("--lam", type=float, default=None,
                    help="grid resolution factor; default: tuned over %s" % (LAMBDAS,))
+    p.add_argument("--mode", default="frame", choices=["frame", "serve", "batch", "ao"],
+                   help="frame: the stadium frame (default, headline line, with a short "
+                        "serve-vs-plain latency section); serve / batch / ao: the serving, "
+                        "batching and ambient-occlusion workloads on their own")
+    p.add_argument("--no-serve", action="store_true", help="frame mode without the latency run")
+    p.add_argument("--serve-requests", type=int, default=120)
     return p.parse_args()
 
 
@@ -271,6 +277,19 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     torch.cuda.set_device(0)
+    if args.mode != "frame":
+        from grt import perf
+        if args.mode == "serve":
+            ctx = grt.Context(0)
+            scenes, tris = perf.build_serve_scenes(ctx)
+            res = perf.serve_vs_plain(ctx, scenes, n_requests=args.serve_requests)
+            res["triangles"] = tris
+        elif args.mode == "batch":
+            res = perf.batch_config()
+        else:
+            res = perf.ao_config()
+        print(json.dumps({"mode": args.mode, **res}))
+        return
     frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill)
     if args.block:
         frame.ctx.set_block_size(args.block, args.block)
@@ -354,6 +373,14 @@ def main():
                      "kernel": "k_trace_closest (primary)",
                      "bytes_per_launch": bytes_launch},
     }
+    if not args.no_serve:
+        # p50/p99 request latency, continuous batching vs the plain path, on
+        # the serving workload (16 resident sphere scenes, mixed requests).
+        from grt import perf
+        scenes, tris = perf.build_serve_scenes(frame.ctx)
+        lat = perf.serve_vs_plain(frame.ctx, scenes, n_requests=args.serve_requests)
+        lat["triangles"] = tris
+        out["latency"] = lat
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(frame, torch, args.cpu_seconds)
     print(json.dumps(out))

@@ -84,6 +84,30 @@ hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occ
 hipError_t launch_trace_persistent(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
                                    uint8_t* occluded, bool any, uint32_t* slot_counter,
                                    uint32_t block, uint32_t refill, hipStream_t s);
+// One item of a batched launch: a ray batch against one scene's grid.  The
+// slot range [slot_begin, slot_begin + slots) is padded to whole waves (64).
+struct alignas(16) BatchTask {
+    const DevFrame* fr;
+    const uint32_t* offsets;
+    const float4* trir;
+    RaySoA rays;
+    HitSoA hits;            // closest-hit outputs (any == 0)
+    uint8_t* occluded;      // any-hit output (any == 1)
+    uint32_t n;
+    uint32_t image_width;
+    uint32_t slot_begin;
+    uint32_t slots;
+    uint32_t any;
+    uint32_t pad[3];
+};
+
+// Map each 64-slot block to its task (n_blocks entries).
+hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* map,
+                           hipStream_t s);
+// Persistent multi-scene traversal over all tasks (mixed closest / any hit).
+hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
+                              uint32_t* slot_counter, uint32_t block, uint32_t refill,
+                              hipStream_t s);
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_gen_shadow(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
                              const float* normals, float lx, float ly, float lz, float offset,
@@ -94,6 +118,8 @@ hipError_t launch_gen_reflection(uint32_t n, const RaySoA& in, const HitSoAIn& h
 hipError_t launch_gen_ao(uint32_t n, uint32_t spp, uint64_t seed, const RaySoA& in,
                          const HitSoAIn& hits, const float* normals, float offset,
                          float ao_tmax, const RaySoAOut& out, hipStream_t s);
+hipError_t launch_count_active(const float* tmin, const float* tmax, uint32_t n,
+                               unsigned long long* count, hipStream_t s);
 hipError_t launch_shade_headlight(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
                                   const float* normals, uint32_t* rgba, hipStream_t s);
 

@@ -307,6 +307,28 @@ hipError_t launch_gen_ao(uint32_t n, uint32_t spp, uint64_t seed, const RaySoA& 
     return hipGetLastError();
 }
 
+// Number of active rays (tmin <= tmax) added to *count: one atomic per wave.
+__global__ __launch_bounds__(256) void k_count_active(const float* __restrict__ tmin,
+                                                      const float* __restrict__ tmax, uint32_t n,
+                                                      unsigned long long* __restrict__ count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n && tmin[i] <= tmax[i];
+    const uint64_t m = __ballot(act);
+    if ((threadIdx.x & 63u) == 0 && m != 0)
+        atomicAdd(count, (unsigned long long)__popcll(m));
+}
+
+hipError_t launch_count_active(const float* tmin, const float* tmax, uint32_t n,
+                               unsigned long long* count, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_count_active, dim3((n + 255) / 256), dim3(256), 0, s, tmin, tmax, n,
+                       count);
+    return hipGetLastError();
+}
+
 hipError_t launch_shade_headlight(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
                                   const float* normals, uint32_t* rgba, hipStream_t s)
 {

@@ -594,6 +594,166 @@ __global__ void k_fill_u32_one(uint32_t* __restrict__ p)
 }
 
 // ---------------------------------------------------------------------------
+// Batched multi-scene traversal (device work queue)
+// ---------------------------------------------------------------------------
+
+// map[b] = index of the task owning slots [64b, 64b+64).
+__global__ __launch_bounds__(256) void k_task_map(const BatchTask* __restrict__ tasks,
+                                                  uint32_t* __restrict__ map)
+{
+    const BatchTask& t = tasks[blockIdx.x];
+    const uint32_t b0 = t.slot_begin >> 6, nb = t.slots >> 6;
+    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x)
+        map[b0 + j] = blockIdx.x;
+}
+
+// Same per-lane algorithm as k_trace_persistent, but every lane carries its
+// own task (scene grid, ray and output buffers, closest or any hit): a refill
+// hands out global slots; slot s belongs to task map[s / 64].
+__global__ __launch_bounds__(256) void k_trace_batch(const BatchTask* __restrict__ tasks,
+                                                     const uint32_t* __restrict__ map,
+                                                     uint32_t total_slots,
+                                                     uint32_t* __restrict__ slot_counter,
+                                                     uint32_t refill)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    int32_t ray = -1;
+    bool exhausted = false;
+    uint32_t task = 0;
+    bool any = false;
+    const uint32_t* offsets = nullptr;
+    const float4* trir = nullptr;
+    uint32_t nx = 0, ny = 0;
+    float o0 = 0, o1 = 0, o2 = 0, d0 = 0, d1 = 0, d2 = 0, tmin = 0, tmax = 0;
+    Dda s = {};
+    uint32_t k = 0, e = 0;
+    bool loaded = false;
+    float exit_t = 0.0f, bound = 0.0f;
+    int32_t id = -1;
+    float best = INFINITY, bu = 0.0f, bv = 0.0f;
+    float4 A = {}, B = {}, C = {};
+
+    for (;;) {
+        const bool need = (ray < 0) && !exhausted;
+        const uint64_t need_mask = __ballot(need);
+        const uint64_t busy_mask = __ballot(ray >= 0);
+        const uint32_t n_need = (uint32_t)__popcll(need_mask);
+        if (need_mask != 0 && (n_need >= refill || busy_mask == 0)) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(need_mask);
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(slot_counter, n_need);
+            base = __builtin_amdgcn_readlane(base, leader);
+            if (need) {
+                const uint32_t slot = base + lanes_below(need_mask);
+                if (slot >= total_slots) {
+                    exhausted = true;
+                } else {
+                    task = map[slot >> 6];
+                    const BatchTask& T = tasks[task];
+                    bool valid;
+                    const uint32_t ri = slot_ray(slot - T.slot_begin, T.image_width, T.n, valid);
+                    if (valid) {
+                        const DevFrame* __restrict__ fr = T.fr;
+                        any = T.any != 0;
+                        offsets = T.offsets;
+                        trir = T.trir;
+                        nx = (uint32_t)fr->n[0];
+                        ny = (uint32_t)fr->n[1];
+                        o0 = T.rays.ox[ri];
+                        o1 = T.rays.oy[ri];
+                        o2 = T.rays.oz[ri];
+                        d0 = T.rays.dx[ri];
+                        d1 = T.rays.dy[ri];
+                        d2 = T.rays.dz[ri];
+                        tmin = T.rays.tmin[ri];
+                        tmax = T.rays.tmax[ri];
+                        id = -1;
+                        best = INFINITY;
+                        bu = 0.0f;
+                        bv = 0.0f;
+                        loaded = false;
+                        k = e = 0;
+                        ray = (int32_t)ri;
+                        if (!(grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s))) {
+                            if (any) {
+                                T.occluded[ri] = 0;
+                            } else {
+                                T.hits.id[ri] = -1;
+                                T.hits.t[ri] = INFINITY;
+                                T.hits.u[ri] = 0.0f;
+                                T.hits.v[ri] = 0.0f;
+                            }
+                            ray = -1;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (__ballot(!exhausted) == 0)
+                break;
+            continue;
+        }
+        if (ray < 0)
+            continue;
+
+        bool finish = false;
+        if (k < e) {
+            const float4 cA = A, cB = B, cC = C;
+            ++k;
+            if (k < e) {
+                A = trir[3ull * k];
+                B = trir[3ull * k + 1];
+                C = trir[3ull * k + 2];
+            }
+            if (any) {
+                if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax)) {
+                    id = 1;
+                    finish = true;
+                }
+            } else {
+                closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu,
+                             bv);
+            }
+        } else {
+            if (loaded) {
+                if (!any && id >= 0)
+                    finish = true;
+                else if (exit_t > tmax || !dda_step(s))
+                    finish = true;
+            }
+            if (!finish) {
+                exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+                const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+                k = offsets[ci];
+                e = offsets[ci + 1];
+                bound = exit_t * kExitScale;
+                loaded = true;
+                if (k < e) {
+                    A = trir[3ull * k];
+                    B = trir[3ull * k + 1];
+                    C = trir[3ull * k + 2];
+                }
+            }
+        }
+        if (finish) {
+            const BatchTask& T = tasks[task];
+            const uint32_t ri = (uint32_t)ray;
+            if (any) {
+                T.occluded[ri] = id >= 0 ? 1 : 0;
+            } else {
+                T.hits.id[ri] = id;
+                T.hits.t[ri] = best;
+                T.hits.u[ri] = bu;
+                T.hits.v[ri] = bv;
+            }
+            ray = -1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 static uint32_t ray_grid(uint32_t n, uint32_t img_w, uint32_t block)
@@ -679,6 +839,30 @@ hipError_t launch_trace_persistent(const TraceArgs& a, const RaySoA& rays, const
             GRT_LAUNCH_P(false, false);
     }
 #undef GRT_LAUNCH_P
+    return hipGetLastError();
+}
+
+hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* map, hipStream_t s)
+{
+    if (n_tasks == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_task_map, dim3(n_tasks), dim3(256), 0, s, tasks, map);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
+                              uint32_t* slot_counter, uint32_t block, uint32_t refill,
+                              hipStream_t s)
+{
+    if (total_slots == 0)
+        return hipSuccess;
+    refill = refill < 1 ? 1 : (refill > 64 ? 64 : refill);
+    hipLaunchKernelGGL(k_fill_u32_one, dim3(1), dim3(64), 0, s, slot_counter);
+    uint32_t g = resident_blocks(k_trace_batch, block);
+    const uint32_t need = (total_slots + block - 1) / block;
+    g = g < need ? g : need;
+    hipLaunchKernelGGL(k_trace_batch, dim3(g), dim3(block), 0, s, tasks, map, total_slots,
+                       slot_counter, refill);
     return hipGetLastError();
 }
 

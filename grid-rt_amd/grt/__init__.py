@@ -101,6 +101,17 @@ _SIGS = {
     "grt_shade_headlight": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32,
                                            _P]),
     "grt_debug_scan": (ctypes.c_int, [_P, _P, _P, _U32]),
+    "grt_batch_create": (ctypes.c_int, [_P, _U32, ctypes.POINTER(_P)]),
+    "grt_batch_set": (ctypes.c_int, [_P, _P, _U32]),
+    "grt_batch_trace": (ctypes.c_int, [_P]),
+    "grt_batch_slots": (ctypes.c_int, [_P, ctypes.POINTER(_U32)]),
+    "grt_batch_destroy": (ctypes.c_int, [_P]),
+    "grt_graph_begin": (ctypes.c_int, [_P]),
+    "grt_graph_end": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "grt_graph_launch": (ctypes.c_int, [_P]),
+    "grt_graph_destroy": (ctypes.c_int, [_P]),
+    "grt_stream_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "grt_stream_destroy": (ctypes.c_int, [_P]),
     # scene generators (host only)
     "grt_gen_spheres": (_U32, [ctypes.c_uint64, _U32, _F, _F, _P, _P]),
     "grt_gen_sphere_clusters": (_U32, [ctypes.c_uint64, _U32, _U32, _F, _F, _F, _P, _P]),
@@ -412,3 +423,147 @@ class Scene:
         if getattr(self, "h", None):
             lib().grt_scene_destroy(self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------------------
+# Batched multi-scene tracing and launch graphs
+# ---------------------------------------------------------------------------
+class BatchItem(ctypes.Structure):
+    _fields_ = [("scene", _P), ("rays", Rays), ("hits", Hits), ("occluded", _P),
+                ("n", _U32), ("image_width", _U32), ("any", _U32)]
+
+
+class Batch:
+    """Many (scene, ray batch) items traced in one persistent launch."""
+
+    def __init__(self, ctx: Context, max_items: int):
+        self.ctx = ctx
+        h = _P()
+        check(lib().grt_batch_create(ctx.h, max_items, ctypes.byref(h)))
+        self.h = h
+        self._keep = None
+
+    def set(self, items):
+        """items: iterable of (scene, rays, hits_or_None, occluded_or_None, image_width)."""
+        arr = (BatchItem * max(1, len(items)))()
+        for i, (scene, rays, hits, occ, width) in enumerate(items):
+            it = arr[i]
+            it.scene = scene.h
+            it.rays = rays.c()
+            if hits is not None:
+                it.hits = hits.c()
+                it.any = 0
+            else:
+                it.occluded = _ptr(occ)
+                it.any = 1
+            it.n = rays.n
+            it.image_width = width
+        self._keep = (arr, items)
+        check(lib().grt_batch_set(self.h, ctypes.cast(arr, _P), len(items)))
+
+    def slots(self) -> int:
+        v = _U32()
+        check(lib().grt_batch_slots(self.h, ctypes.byref(v)))
+        return v.value
+
+    def trace(self):
+        check(lib().grt_batch_trace(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().grt_batch_destroy(self.h)
+            self.h = None
+
+
+class Graph:
+    """hipGraph of the launches enqueued on a context's stream while capturing."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = None
+
+    def begin(self):
+        check(lib().grt_graph_begin(self.ctx.h))
+
+    def end(self):
+        h = _P()
+        check(lib().grt_graph_end(self.ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def launch(self):
+        check(lib().grt_graph_launch(self.h))
+
+    def close(self):
+        if self.h:
+            lib().grt_graph_destroy(self.h)
+            self.h = None
+
+
+def stream_create() -> int:
+    h = _P()
+    check(lib().grt_stream_create(ctypes.byref(h)))
+    return h.value
+
+
+def stream_destroy(handle: int):
+    check(lib().grt_stream_destroy(handle))
+
+
+# ---------------------------------------------------------------------------
+# Serving
+# ---------------------------------------------------------------------------
+class Request(ctypes.Structure):
+    _fields_ = [("scene", _P), ("camera", Camera), ("depth", _U32), ("arrival_s", ctypes.c_double),
+                ("out_hits", Hits), ("out_occluded", _P)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("start_s", ctypes.c_double), ("finish_s", ctypes.c_double),
+                ("rays", ctypes.c_uint64)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("max_active", _U32), ("batched", _U32),
+                ("light", _F * 3), ("offset", _F)]
+
+
+_SIGS.update({
+    "grt_serve_run": (ctypes.c_int, [_P, _P, _U32, ctypes.POINTER(ServeConfig), _P,
+                                     ctypes.POINTER(ctypes.c_double)]),
+})
+
+
+def serve_run(ctx: Context, requests, batched: bool, memory_budget: int, max_active: int = 64,
+              light=(1000.0, 1000.0, 1000.0), offset: float = 1e-4):
+    """requests: list of (scene, camera, depth, arrival_s), sorted by arrival.
+    Returns (arrival, start, finish, rays) numpy arrays and the wall time."""
+    L = lib()
+    fn = L.grt_serve_run
+    fn.restype, fn.argtypes = _SIGS["grt_serve_run"]
+    n = len(requests)
+    arr = (Request * max(1, n))()
+    for i, req in enumerate(requests):
+        scene, cam, depth, t = req[:4]
+        arr[i].scene = scene.h
+        arr[i].camera = cam
+        arr[i].depth = depth
+        arr[i].arrival_s = t
+        if len(req) > 4 and req[4] is not None:   # (HitBatch, occluded tensor) outputs
+            hb, occ = req[4]
+            arr[i].out_hits = hb.c()
+            arr[i].out_occluded = _ptr(occ)
+    res = (RequestResult * max(1, n))()
+    cfg = ServeConfig()
+    cfg.memory_budget = memory_budget
+    cfg.max_active = max_active
+    cfg.batched = 1 if batched else 0
+    cfg.light[:] = list(light)
+    cfg.offset = offset
+    wall = ctypes.c_double()
+    check(fn(ctx.h, ctypes.cast(arr, _P), n, ctypes.byref(cfg), ctypes.cast(res, _P),
+             ctypes.byref(wall)))
+    arrival = np.array([r[3] for r in requests], np.float64)
+    start = np.array([res[i].start_s for i in range(n)], np.float64)
+    finish = np.array([res[i].finish_s for i in range(n)], np.float64)
+    rays = np.array([res[i].rays for i in range(n)], np.uint64)
+    return arrival, start, finish, rays, wall.value

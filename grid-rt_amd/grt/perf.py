@@ -1,0 +1,322 @@
+"""Measurements of the batching, serving and ambient-occlusion workloads
+(see grt.workloads for their definitions).  Every timed region starts with
+scenes, grids and buffers resident in device memory.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import Batch, Context, Graph, HitBatch, RayBatch, Scene, serve_run, stream_create
+from . import workloads as W
+from .autotune import Autotuner
+
+
+def _torch():
+    import torch
+    return torch
+
+
+# ---------------------------------------------------------------------------
+# Serving: continuous batching vs the plain path
+# ---------------------------------------------------------------------------
+def build_serve_scenes(ctx: Context, lam: float = 8.0, n_scenes: int = 16, max_tris=None):
+    scenes, tris = [], 0
+    for i, nt in enumerate(W.serve_scene_sizes(n=n_scenes)):
+        if max_tris:
+            nt = min(nt, max_tris)
+        v, nrm = W.sphere_scene(W.SEED_SERVE + 100 + i, nt)
+        sc = Scene(ctx)
+        sc.upload(v, nrm)
+        sc.build_grid(lam)
+        scenes.append(sc)
+        tris += v.shape[0]
+    return scenes, tris
+
+
+def _requests(scenes, spec):
+    return [(scenes[s], W.camera(w, h), d, t) for (s, w, h, d, t) in spec]
+
+
+def _summary(arrival, start, finish, rays, wall):
+    lat = finish - arrival
+    return {"p50_ms": round(W.percentile_ms(lat, 50), 3),
+            "p99_ms": round(W.percentile_ms(lat, 99), 3),
+            "mean_ms": round(float(lat.mean() * 1e3), 3),
+            "mrays_s": round(float(rays.sum()) / wall / 1e6, 2),
+            "wall_s": round(wall, 3), "requests": int(len(lat))}
+
+
+def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120, load: float = 0.8,
+                   budget: int = 8 << 30, max_active: int = 64):
+    """Saturation throughput (all requests at t=0) and open-loop latency at
+    `load` x the plain path's capacity, for both paths."""
+    torch = _torch()
+    warm = _requests(scenes, W.serve_requests(6, 0.0, len(scenes), seed=W.SEED_SERVE + 7))
+    serve_run(ctx, warm, True, budget, max_active)
+    serve_run(ctx, warm, False, budget, max_active)
+    torch.cuda.synchronize()
+    closed = _requests(scenes, W.serve_requests(n_requests, 0.0, len(scenes)))
+    plain_sat = _summary(*serve_run(ctx, closed, False, budget, max_active))
+    serve_sat = _summary(*serve_run(ctx, closed, True, budget, max_active))
+    rate = load * n_requests / plain_sat["wall_s"]
+    spec = W.serve_requests(n_requests, rate, len(scenes))
+    opened = _requests(scenes, spec)
+    plain = _summary(*serve_run(ctx, opened, False, budget, max_active))
+    served = _summary(*serve_run(ctx, opened, True, budget, max_active))
+    return {
+        "arrival_rate_hz": round(rate, 2),
+        "load_vs_plain_capacity": load,
+        "serve": served, "plain": plain,
+        "saturation": {"serve": serve_sat, "plain": plain_sat},
+        "p99_headroom_x": round(plain["p99_ms"] / max(served["p99_ms"], 1e-9), 2),
+        "saturation_mrays_x": round(serve_sat["mrays_s"] / max(plain_sat["mrays_s"], 1e-9), 2),
+    }
+
+
+# ---------------------------------------------------------------------------
+# Batching: 64 scenes in one persistent launch per depth level, hipGraph
+# ---------------------------------------------------------------------------
+class BatchedJobs:
+    """Buffers and per-level batches of the batched configuration."""
+
+    def __init__(self, ctx: Context, jobs, lam: float):
+        torch = _torch()
+        self.ctx = ctx
+        self.jobs = []
+        for i, (tris, side, depth) in enumerate(jobs):
+            v, nrm = W.sphere_scene(W.SEED_BATCH + 1000 + i, tris)
+            sc = Scene(ctx)
+            sc.upload(v, nrm)
+            n = side * side
+            self.jobs.append({"scene": sc, "side": side, "depth": depth, "n": n,
+                              "cam": W.camera(side, side), "tris": v.shape[0],
+                              "rays": [RayBatch.empty(n), RayBatch.empty(n)],
+                              "hits": [HitBatch.empty(n), HitBatch.empty(n)],
+                              "shadow": RayBatch.empty(n),
+                              "occ": torch.empty(n, dtype=torch.uint8, device="cuda:0")})
+        self.levels = 1 + max(j["depth"] for j in self.jobs)
+        self.set_lambda(lam)
+
+    def set_lambda(self, lam):
+        for j in self.jobs:
+            j["scene"].build_grid(lam)
+        self.batches = []
+        for lvl in range(self.levels):
+            items = []
+            for j in self.jobs:
+                items += self._items(j, lvl)
+            b = Batch(self.ctx, max(1, len(items)))
+            b.set(items)
+            self.batches.append(b)
+        self.ctx.synchronize()
+
+    @staticmethod
+    def _items(j, lvl):
+        if lvl > j["depth"]:
+            return []
+        if lvl == 0:
+            return [(j["scene"], j["rays"][0], j["hits"][0], None, j["side"])]
+        c = (lvl - 1) & 1
+        items = [(j["scene"], j["shadow"], None, j["occ"], j["side"])]
+        if lvl < j["depth"]:
+            items.append((j["scene"], j["rays"][1 - c], j["hits"][1 - c], None, j["side"]))
+        return items
+
+    def _gen(self, j, lvl):
+        sc = j["scene"]
+        if lvl == 0:
+            self.ctx.gen_primary(j["cam"], j["rays"][0])
+            return
+        c = (lvl - 1) & 1
+        sc.gen_shadow(j["rays"][c], j["hits"][c], W.LIGHT, 1e-4, j["shadow"])
+        if lvl < j["depth"]:
+            sc.gen_reflection(j["rays"][c], j["hits"][c], 1e-4, j["rays"][1 - c])
+
+    def run_batched(self):
+        for lvl in range(self.levels):
+            for j in self.jobs:
+                if lvl <= j["depth"]:
+                    self._gen(j, lvl)
+            self.batches[lvl].trace()
+
+    def run_plain(self):
+        """One job after another, one launch per ray batch (the stand-alone
+        schedule: primary rays one per lane, the rest persistent)."""
+        for j in self.jobs:
+            sc = j["scene"]
+            for lvl in range(j["depth"] + 1):
+                self._gen(j, lvl)
+                for (s, rays, hits, occ, width) in self._items(j, lvl):
+                    self.ctx.set_trace_mode(lvl != 0, self.ctx_refill)
+                    if hits is not None:
+                        s.trace_closest(rays, hits, image_width=width)
+                    else:
+                        s.trace_any(rays, occ, image_width=width)
+            self.ctx.set_trace_mode(True, self.ctx_refill)
+
+    ctx_refill = 16
+
+    def active_rays(self):
+        torch = _torch()
+        total = 0
+        for j in self.jobs:
+            total += j["n"]
+        # secondary rays: count once by running the levels eagerly
+        for lvl in range(1, self.levels):
+            for j in self.jobs:
+                if lvl <= j["depth"]:
+                    self._gen(j, lvl)
+            self.batches[lvl].trace()
+            for j in self.jobs:
+                if lvl <= j["depth"]:
+                    f = j["shadow"].f
+                    total += int((f["tmin"] <= f["tmax"]).sum().item())
+                    if lvl < j["depth"]:
+                        f = j["rays"][1 - ((lvl - 1) & 1)].f
+                        total += int((f["tmin"] <= f["tmax"]).sum().item())
+        torch.cuda.synchronize()
+        return total
+
+
+def _time(fn, reps, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def batch_config(reps: int = 5):
+    """Batched (one persistent launch per level, hipGraph) vs plain; the
+    autotuner picks lambda in {2,5,8}, block size and refill threshold."""
+    torch = _torch()
+    ctx = Context(0)
+    stream = stream_create()
+    torch.cuda.synchronize()
+    ctx.set_stream(stream)
+    jobs = W.batch_jobs()
+    bj = BatchedJobs(ctx, jobs, 5.0)
+    tuner = Autotuner()
+    cands = [{"lam": lam, "block": blk, "refill": rf}
+             for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256) for rf in (8, 16, 32)]
+    cur = {"lam": 5.0}
+
+    def measure(c):
+        if c["lam"] != cur["lam"]:
+            bj.set_lambda(c["lam"])
+            cur["lam"] = c["lam"]
+        ctx.set_block_size(c["block"], c["block"])
+        ctx.set_trace_mode(True, c["refill"])
+        bj.run_batched()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        bj.run_batched()
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    best = tuner.tune("batch-64", cands, measure)
+    if best["lam"] != cur["lam"]:
+        bj.set_lambda(best["lam"])
+    ctx.set_block_size(best["block"], best["block"])
+    ctx.set_trace_mode(True, best["refill"])
+    BatchedJobs.ctx_refill = best["refill"]
+    rays = bj.active_rays()
+    g = Graph(ctx)
+    g.begin()
+    bj.run_batched()
+    g.end()
+    g.launch()
+    ctx.synchronize()
+
+    def replay():
+        g.launch()
+        ctx.synchronize()
+
+    t_graph = _time(replay, reps, torch)
+    t_eager = _time(lambda: (bj.run_batched(), ctx.synchronize()), reps, torch)
+    t_plain = _time(lambda: (bj.run_plain(), ctx.synchronize()), reps, torch)
+    g.close()
+    out = {"jobs": len(jobs), "triangles": int(sum(j["tris"] for j in bj.jobs)),
+           "primary_rays": int(sum(j["n"] for j in bj.jobs)), "rays_per_run": rays,
+           "tuned": best,
+           "batched_graph_ms": round(t_graph * 1e3, 3), "batched_eager_ms": round(t_eager * 1e3, 3),
+           "plain_ms": round(t_plain * 1e3, 3),
+           "mrays_s": {"batched_graph": round(rays / t_graph / 1e6, 2),
+                       "batched_eager": round(rays / t_eager / 1e6, 2),
+                       "plain": round(rays / t_plain / 1e6, 2)},
+           "autotune_top5": sorted(tuner.cache["batch-64"]["tried"],
+                                   key=lambda r: r["cost"])[:5]}
+    ctx.synchronize()
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Ambient occlusion on 2^24 triangles
+# ---------------------------------------------------------------------------
+def ao_config(lam: float = 8.0, reps: int = 3, width: int = 1024, height: int = 512,
+              spp: int = 8):
+    """Grid rebuild of 2^24 triangles and 8 AO rays per pixel (any hit,
+    tmax = 0.05 * scene extent) from primary hits."""
+    torch = _torch()
+    ctx = Context(0)
+    v, nrm = W.ao_scene()
+    sc = Scene(ctx)
+    sc.upload(v, nrm)
+    sc.build_grid(lam)
+    info = sc.grid_info()
+    sc.reserve_grid(int(info["m"] * 1.1) + 4096, int(info["r"] * 1.1) + 4096)
+    extent = float(np.max(info["bmax"] - info["bmin"]))
+    cam = W.camera(width, height)
+    n = width * height
+    prim, hits = RayBatch.empty(n), HitBatch.empty(n)
+    ao = RayBatch.empty(n * spp)
+    occ = torch.empty(n * spp, dtype=torch.uint8, device="cuda:0")
+    ctx.gen_primary(cam, prim)
+    ctx.set_trace_mode(False, 16)
+    sc.trace_closest(prim, hits, image_width=width)
+    sc.gen_ao(prim, hits, spp, W.SEED_AO, 1e-4, 0.05 * extent, ao)
+    ctx.synchronize()
+    active = int((ao.f["tmin"] <= ao.f["tmax"]).sum().item())
+    cnt = torch.empty(2 * n * spp, dtype=torch.int32, device="cuda:0")
+    res = {}
+    for mode in ("simple", "persistent"):
+        ctx.set_trace_mode(mode == "persistent", 16)
+        sc.trace_any(ao, occ, counters=cnt)
+        ctx.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            sc.trace_any(ao, occ)
+        ev[1].record()
+        torch.cuda.synchronize()
+        res[mode] = ev[0].elapsed_time(ev[1]) / reps
+    c = cnt.view(-1, 2).to(torch.int64)
+    act = (ao.f["tmin"] <= ao.f["tmax"])
+    cells = float(c[act, 0].double().mean().item())
+    tests = float(c[act, 1].double().mean().item())
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        sc.rebuild_grid_async(lam)
+    ev[1].record()
+    torch.cuda.synchronize()
+    sc.check_grid()
+    build_ms = ev[0].elapsed_time(ev[1]) / reps
+    best = min(res, key=res.get)
+    bytes_ray = 36 + 8 * cells + 48 * tests
+    n_tris = v.shape[0]
+    info = sc.grid_info()
+    build_bytes = 36 * n_tris + 4 * info["r"] + 4 * (info["m"] + 1)
+    return {"triangles": n_tris, "ao_rays": active, "spp": spp, "lambda": lam,
+            "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"]},
+            "trace_ms": {k: round(v_, 3) for k, v_ in res.items()}, "schedule": best,
+            "mrays_s": round(active / (res[best] * 1e-3) / 1e6, 2),
+            "mean_cells": round(cells, 2), "mean_tests": round(tests, 2),
+            "algorithmic_bytes_per_ray": round(bytes_ray, 1),
+            "trace_gbs": round(active * bytes_ray / (res[best] * 1e-3) / 1e9, 1),
+            "build_ms": round(build_ms, 3),
+            "build_algorithmic_gbs": round(build_bytes / (build_ms * 1e-3) / 1e9, 1)}

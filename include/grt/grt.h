@@ -176,6 +176,79 @@ int grt_gen_ao(grt_scene scene, const grt_rays* in, const grt_hits* hits, uint32
 int grt_shade_headlight(grt_scene scene, const grt_rays* rays, const grt_hits* hits, uint32_t n,
                         uint32_t* rgba_dev);
 
+/* ---- batched multi-scene tracing ------------------------------------- */
+/* The reference traced one scene per launch (cudaRayTrace,
+ * RayTracing.cu:227-253).  A batch holds items -- each a ray batch of one
+ * scene, closest hit (any = 0, outputs in hits) or any hit (any = 1, output in
+ * occluded) -- and traces all of them in one persistent launch whose waves
+ * fetch ray slots from a device work queue.  Item pointers are captured at
+ * grt_batch_set: call it again after a grt_grid_build or grt_grid_upload of any
+ * item's scene (grt_grid_rebuild_async keeps the pointers valid). */
+typedef struct grt_batch_t* grt_batch;
+typedef struct {
+    grt_scene scene;
+    grt_rays rays;
+    grt_hits hits;
+    uint8_t* occluded;
+    uint32_t n;
+    uint32_t image_width;
+    uint32_t any;
+} grt_batch_item;
+int grt_batch_create(grt_context ctx, uint32_t max_items, grt_batch* out);
+int grt_batch_set(grt_batch batch, const grt_batch_item* items, uint32_t n_items);
+int grt_batch_trace(grt_batch batch);
+int grt_batch_slots(grt_batch batch, uint32_t* slots);
+int grt_batch_destroy(grt_batch batch);
+
+/* ---- serving ------------------------------------------------------------ */
+/* A render request: a resident scene (with a built grid), a pinhole camera
+ * and a depth >= 1 of closest-hit generations (primary + depth-1 mirror
+ * bounces), each generation followed by shadow rays toward cfg->light.  The
+ * reference rendered one frame at a time with one launch per pass
+ * (GpuRenderer.cpp:74-319). */
+typedef struct {
+    grt_scene scene;
+    grt_camera camera;
+    uint32_t depth;
+    double arrival_s;        /* seconds after the start of the run (sorted) */
+    grt_hits out_hits;       /* optional (device, width*height): primary hits */
+    uint8_t* out_occluded;   /* optional (device): occlusion of the primary shadow rays */
+} grt_request;
+typedef struct {
+    double start_s;          /* admitted (seconds after the start) */
+    double finish_s;         /* results complete on the device */
+    uint64_t rays;           /* active rays traced */
+} grt_request_result;
+typedef struct {
+    uint64_t memory_budget;  /* bytes of device memory for request buffers */
+    uint32_t max_active;     /* requests in flight (batched mode) */
+    uint32_t batched;        /* 1: continuous batching; 0: plain path, one request
+                                at a time, one launch per ray batch */
+    float light[3];
+    float offset;            /* secondary-ray origin offset along the normal */
+} grt_serve_config;
+/* Runs the requests in real time (arrivals at their arrival_s), blocking until
+ * all complete.  Batched: each round traces the current step of every admitted
+ * request in one batched launch; arrivals are admitted at round boundaries,
+ * FIFO, while a slot and budget memory are free.  Buffers come from one slab of
+ * memory_budget bytes, so nothing is allocated inside the loop. */
+int grt_serve_run(grt_context ctx, const grt_request* requests, uint32_t n,
+                  const grt_serve_config* cfg, grt_request_result* out, double* wall_s);
+
+/* ---- launch graphs ---------------------------------------------------- */
+/* Capture every launch enqueued on the context's stream between begin and end
+ * into a hipGraph (the stream must not be the null stream; entry points that
+ * synchronise or copy from pageable host memory must not be called while
+ * capturing: trace, gen, batch_trace and grid_rebuild_async are safe). */
+typedef struct grt_graph_t* grt_graph;
+int grt_graph_begin(grt_context ctx);
+int grt_graph_end(grt_context ctx, grt_graph* out);
+int grt_graph_launch(grt_graph graph);
+int grt_graph_destroy(grt_graph graph);
+/* A non-blocking HIP stream at the default priority (opaque hipStream_t). */
+int grt_stream_create(void** out);
+int grt_stream_destroy(void* stream);
+
 /* ---- test hooks ------------------------------------------------------- */
 /* Exclusive scan of n uint32 (device pointers; out has n+1 entries). */
 int grt_debug_scan(grt_context ctx, const uint32_t* in_dev, uint32_t* out_dev, uint32_t n);

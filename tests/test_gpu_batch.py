@@ -1,0 +1,197 @@
+"""GPU tests of the batched launch (device work queue), hipGraph replay and the
+continuous-batching serving loop.  Bar: bit-exact -- an item traced inside a
+mixed batch, a replayed graph and a request served in a mix give exactly the
+results of the oracle / of the same work alone."""
+import numpy as np
+import pytest
+
+import grt
+import pyoracle as orc
+from grt import workloads as W
+from test_gpu_parity import _random_rays, assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(ctx, v, n, lam=8.0):
+    sc = grt.Scene(ctx)
+    sc.upload(v, n)
+    sc.build_grid(lam)
+    return sc
+
+
+def _oracle_grid(v, lam=8.0):
+    fr = orc.grid_frame(v, lam)
+    _, offs, refs = orc.grid_build(v, fr)
+    return fr, offs, refs
+
+
+@pytest.fixture(scope="module")
+def three_scenes(gpu_ctx):
+    import torch  # noqa: F401
+    a = grt.gen_spheres(11, 16, 0.05, 0.15)
+    b = grt.gen_spheres(W.SEED_SPHERES, 64, 0.02, 0.08)
+    c = grt.gen_stadium(W.SEED_STADIUM, 8, 1024, 8 * 1024 - 2)[:2]
+    out = []
+    for v, n in (a, b, c):
+        out.append((_scene(gpu_ctx, v, n), v, n, _oracle_grid(v)))
+    return out
+
+
+def _primary(width, height, eye=(0.5, 0.5, -1.5), center=(0.5, 0.5, 0.0)):
+    cam = grt.camera_lookat(eye, center, (0, 1, 0), 60.0, width, height)
+    return orc.primary_rays(grt.camera_dict(cam))
+
+
+def test_batch_items_match_oracle(gpu_ctx, three_scenes):
+    import torch
+    specs = [  # (scene, rays, any, image_width)
+        (0, _primary(96, 64), False, 96),
+        (1, _random_rays(1, 5), False, 0),
+        (1, _random_rays(65, 6), True, 0),
+        (2, _primary(100, 37, (0.5, 0.6, -1.5), (0.5, 0.3, 0.5)), False, 100),
+        (2, _random_rays(5000, 7), True, 0),
+        (0, _random_rays(777, 8), False, 0),
+        (1, _primary(64, 64), True, 64),
+    ]
+    want, items, outs = [], [], []
+    for si, rays, anyhit, width in specs:
+        sc, v, _, (fr, offs, refs) = three_scenes[si]
+        rb = grt.RayBatch.from_numpy(rays)
+        n = rb.n
+        if anyhit:
+            occ = torch.full((n,), 7, dtype=torch.uint8, device="cuda:0")
+            items.append((sc, rb, None, occ, width))
+            outs.append(occ)
+            want.append(orc.trace_any(fr, offs, refs, v, rays)["occluded"])
+        else:
+            hb = grt.HitBatch.empty(n)
+            items.append((sc, rb, hb, None, width))
+            outs.append(hb)
+            want.append(orc.trace_closest(fr, offs, refs, v, rays))
+    for order in (list(range(len(items))), list(reversed(range(len(items))))):
+        batch = grt.Batch(gpu_ctx, 16)
+        batch.set([items[i] for i in order])
+        assert batch.slots() % 64 == 0
+        batch.trace()
+        torch.cuda.synchronize()
+        for k, (spec, out, w) in enumerate(zip(specs, outs, want)):
+            if spec[2]:
+                assert_same(out.cpu().numpy(), w, f"item {k} occluded")
+            else:
+                g = out.numpy()
+                for f in ("id", "t", "u", "v"):
+                    assert_same(g[f], w[f], f"item {k} hit.{f}")
+        batch.close()
+
+
+def test_batch_rejects_bad_items(gpu_ctx, three_scenes):
+    batch = grt.Batch(gpu_ctx, 2)
+    sc = three_scenes[0][0]
+    rb = grt.RayBatch.empty(10)
+    with pytest.raises(grt.GrtError):
+        batch.set([(sc, rb, grt.HitBatch.empty(10), None, 0)] * 3)   # over capacity
+    empty = grt.Scene(gpu_ctx)
+    with pytest.raises(grt.GrtError):
+        batch.set([(empty, rb, grt.HitBatch.empty(10), None, 0)])   # no grid
+    batch.close()
+
+
+def test_graph_replay_matches_eager(gpu_ctx, three_scenes):
+    import torch
+    sc, v, nrm, _ = three_scenes[1]
+    cam = grt.camera_lookat((0.5, 0.5, -1.5), (0.5, 0.5, 0.0), (0, 1, 0), 60.0, 160, 120)
+    n = cam.width * cam.height
+    stream = grt.stream_create()
+    torch.cuda.synchronize()
+    gpu_ctx.set_stream(stream)
+    try:
+        rb, hb, sb = grt.RayBatch.empty(n), grt.HitBatch.empty(n), grt.RayBatch.empty(n)
+        occ = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+        torch.cuda.synchronize()
+
+        def work():
+            gpu_ctx.gen_primary(cam, rb)
+            sc.trace_closest(rb, hb, image_width=cam.width)
+            sc.gen_shadow(rb, hb, W.LIGHT, 1e-3, sb)
+            sc.trace_any(sb, occ, image_width=cam.width)
+
+        work()
+        gpu_ctx.synchronize()
+        eager_h, eager_o = hb.numpy(), occ.cpu().numpy()
+        g = grt.Graph(gpu_ctx)
+        g.begin()
+        work()
+        g.end()
+        for rep in range(3):
+            for t in hb.f.values():
+                t.fill_(-3)
+            occ.fill_(9)
+            torch.cuda.synchronize()
+            g.launch()
+            gpu_ctx.synchronize()
+            got = hb.numpy()
+            for f in ("id", "t", "u", "v"):
+                assert_same(got[f], eager_h[f], f"replay {rep} hit.{f}")
+            assert_same(occ.cpu().numpy(), eager_o, f"replay {rep} occluded")
+            sc.trace_closest(rb, grt.HitBatch.empty(n))   # eager work between replays
+            gpu_ctx.synchronize()
+        g.close()
+    finally:
+        gpu_ctx.synchronize()
+        gpu_ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        grt.stream_destroy(stream)
+
+
+def test_serve_result_same_alone_and_in_mix(gpu_ctx, three_scenes):
+    import torch
+    reqs = [(0, 64, 64, 1), (1, 128, 96, 3), (2, 200, 100, 2), (1, 96, 96, 2), (0, 80, 48, 3),
+            (2, 64, 64, 1)]
+
+    def make(batched_list, batched, arrivals):
+        outs, rq = [], []
+        for k, (si, w, h, d) in enumerate(batched_list):
+            cam = W.camera(w, h) if si != 2 else grt.camera_lookat(
+                (0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0, 1, 0), 60.0, w, h)
+            hb = grt.HitBatch.empty(w * h)
+            occ = torch.full((w * h,), 5, dtype=torch.uint8, device="cuda:0")
+            outs.append((hb, occ, cam))
+            rq.append((three_scenes[si][0], cam, d, arrivals[k], (hb, occ)))
+        torch.cuda.synchronize()
+        res = grt.serve_run(gpu_ctx, rq, batched=batched, memory_budget=256 << 20, max_active=8)
+        torch.cuda.synchronize()
+        return outs, res
+
+    mix, res_mix = make(reqs, True, [0.0] * len(reqs))
+    assert np.all(res_mix[2] >= res_mix[1]) and np.all(res_mix[3] > 0)
+    for k, r in enumerate(reqs):
+        alone, res_alone = make([r], False, [0.0])
+        assert res_alone[3][0] == res_mix[3][k], "same active-ray count alone and in the mix"
+        a, b = mix[k][0].numpy(), alone[0][0].numpy()
+        for f in ("id", "t", "u", "v"):
+            assert_same(a[f], b[f], f"request {k} primary hit.{f}")
+        assert_same(mix[k][1].cpu().numpy(), alone[0][1].cpu().numpy(), f"request {k} shadow")
+    # and request 0 against the oracle
+    sc, v, nrm, (fr, offs, refs) = three_scenes[0]
+    rays = orc.primary_rays(grt.camera_dict(mix[0][2]))
+    o = orc.trace_closest(fr, offs, refs, v, rays)
+    g = mix[0][0].numpy()
+    for f in ("id", "t", "u", "v"):
+        assert_same(g[f], o[f], f"oracle hit.{f}")
+    sh = orc.gen_shadow(rays, o, nrm, W.LIGHT, 1e-4)
+    assert_same(mix[0][1].cpu().numpy(), orc.trace_any(fr, offs, refs, v, sh)["occluded"],
+                "oracle shadow")
+
+
+def test_serve_budget_admission(gpu_ctx, three_scenes):
+    """A budget that fits one 256x256 request at a time still completes every
+    request (admission waits for memory), FIFO."""
+    reqs = [(three_scenes[0][0], W.camera(256, 256), 2, 0.0) for _ in range(4)]
+    arrival, start, finish, rays, wall = grt.serve_run(gpu_ctx, reqs, batched=True,
+                                                       memory_budget=12 << 20, max_active=8)
+    assert np.all(finish > 0) and np.all(np.diff(start) >= 0)
+    # each admission waits for the previous request to leave the budget
+    assert np.all(start[1:] >= finish[:-1] - 1e-9)
+    with pytest.raises(grt.GrtError):   # a request larger than the whole budget
+        grt.serve_run(gpu_ctx, [(three_scenes[0][0], W.camera(1024, 1024), 1, 0.0)],
+                      batched=True, memory_budget=1 << 20)
