@@ -5,7 +5,8 @@ Workload ("teapot in a stadium", 2^20 binary32 triangles): 256 Gaussian
 clusters of small triangles + a 2-triangle ground plane 100x the cluster
 extent.  One step = one frame:
   1. per-cluster rigid transform of the rest pose (device),
-  2. uniform-grid rebuild on the device (lambda = 5),
+  2. uniform-grid rebuild on the device (lambda chosen by the autotuner among
+     5..128 before the timed region),
   3. 1920x1080 primary rays, closest hit,
   4. shadow rays toward a point light, any hit,
   5. one mirror bounce, closest hit,
@@ -14,11 +15,12 @@ extent.  One step = one frame:
 second of frame time, with scene, grid buffers and ray buffers resident in
 HBM before the timed region.
 
-Also reported: per-stage milliseconds (HIP events on the tracing stream), a
-roofline for the dominant kernel (primary closest-hit traversal) from the
-algorithmic bytes per ray 32 + 16 + 8*C + 52*T (C cells, T triangle tests,
-counted on the device outside the timed region), and the CPU oracle timed on
-the host cores on a bounded sample of the same frame.
+Also reported: per-launch-group milliseconds (HIP events on the tracing
+stream), a roofline for the dominant traversal kernel from the algorithmic
+bytes per active ray -- 32 in + 16 out (closest) or 4 out (any) + 8*C + 52*T,
+C cells visited and T triangle tests counted on the device outside the timed
+region -- and the CPU oracle timed on the host cores on a sample of the same
+frame.
 """
 import argparse
 import json
@@ -66,7 +68,27 @@ def parse():
                         "batching and ambient-occlusion workloads on their own")
     p.add_argument("--no-serve", action="store_true", help="frame mode without the latency run")
     p.add_argument("--serve-requests", type=int, default=120)
+    p.add_argument("--profile", action="store_true",
+                   help="only the warmup and timed frames (with --lam and --sched: no "
+                        "tuning), for a profiler")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first copy 1 GiB on the device three times (counter calibration)")
     return p.parse_args()
+
+
+def intervals(ev):
+    return [ev[i].elapsed_time(ev[i + 1]) for i in range(len(ev) - 1)]
+
+
+# Traversal stages of a frame: (name, closest hit?).
+STAGES = (("primary", True), ("shadow", False), ("bounce", True), ("bounce_shadow", False))
+
+
+def kernel_name(closest, sched_char):
+    """The traversal kernel a stage launches under a schedule character."""
+    if sched_char == "p":
+        return "k_trace_persistent<%s, false>" % ("false" if closest else "true")
+    return "k_trace_closest<false>" if closest else "k_trace_any<false>"
 
 
 class Frame:
@@ -107,13 +129,17 @@ class Frame:
         info = self.scene.grid_info()
         self.scene.reserve_grid(int(info["m"] * 1.25) + 4096, int(info["r"] * 1.5) + 4096)
 
+    N_EVENTS = 11
+
     def time_stages(self, frames=2):
-        """Stage milliseconds of the last of `frames` frames."""
-        ev = [self.torch.cuda.Event(enable_timing=True) for _ in range(7)]
+        """Milliseconds of transform, build and the four ray stages (ray
+        generation + traversal each) of the last of `frames` frames."""
+        ev = [self.torch.cuda.Event(enable_timing=True) for _ in range(self.N_EVENTS)]
         for f in range(frames):
             self.run(2000 + f, ev if f == frames - 1 else None)
         self.torch.cuda.synchronize()
-        return [ev[i].elapsed_time(ev[i + 1]) for i in range(6)]
+        d = intervals(ev)
+        return [d[0], d[1], d[2] + d[3], d[4] + d[5], d[6] + d[7], d[8] + d[9]]
 
     def transforms(self, frame):
         if frame not in self.xf:
@@ -121,8 +147,11 @@ class Frame:
         return self.xf[frame]
 
     def run(self, frame, events=None):
+        """One frame.  Events (if given, N_EVENTS of them on the tracing
+        stream) bracket every launch group: transform, build, and for each of
+        the four ray stages its generation kernel and its traversal kernel."""
         sc, W = self.scene, WIDTH
-        ev = events if events is not None else [None] * 7
+        ev = events if events is not None else [None] * self.N_EVENTS
 
         def mark(i):
             if ev[i] is not None:
@@ -135,20 +164,24 @@ class Frame:
         mark(2)
         self.ctx.gen_primary(self.cam, self.prim)
         self.mode(0)
-        sc.trace_closest(self.prim, self.hits, image_width=W)
         mark(3)
+        sc.trace_closest(self.prim, self.hits, image_width=W)
+        mark(4)
         sc.gen_shadow(self.prim, self.hits, LIGHT, OFFSET, self.shadow)
         self.mode(1)
+        mark(5)
         sc.trace_any(self.shadow, self.occ, image_width=W)
-        mark(4)
+        mark(6)
         sc.gen_reflection(self.prim, self.hits, OFFSET, self.refl)
         self.mode(2)
+        mark(7)
         sc.trace_closest(self.refl, self.rhits, image_width=W)
-        mark(5)
+        mark(8)
         sc.gen_shadow(self.refl, self.rhits, LIGHT, OFFSET, self.shadow2)
         self.mode(3)
+        mark(9)
         sc.trace_any(self.shadow2, self.occ2, image_width=W)
-        mark(6)
+        mark(10)
 
     def mode(self, stage):
         """Traversal schedule of a stage: 's' one ray per lane, 'p' persistent."""
@@ -157,19 +190,6 @@ class Frame:
     def active(self, rb):
         f = rb.f
         return int((f["tmin"] <= f["tmax"]).sum().item())
-
-
-def algorithmic_bytes(frame, torch):
-    """Bytes per launch of the primary closest-hit kernel from device counters."""
-    cnt = torch.empty(2 * frame.n, dtype=torch.int32, device="cuda:0")
-    frame.mode(0)
-    frame.scene.trace_closest(frame.prim, frame.hits, image_width=WIDTH, counters=cnt)
-    torch.cuda.synchronize()
-    c = cnt.view(-1, 2).to(torch.int64)
-    cells = int(c[:, 0].sum().item())
-    tests = int(c[:, 1].sum().item())
-    n = frame.n
-    return 48 * n + 8 * cells + 52 * tests, cells / n, tests / n
 
 
 def autotune(frame, lams, sched):
@@ -202,15 +222,19 @@ def autotune(frame, lams, sched):
 
 
 def stage_counters(frame, torch):
-    """Mean/max cells and triangle tests over the active rays of each stage
-    (device counters, outside the timed region)."""
+    """Per traversal stage, from device counters (outside the timed region):
+    active rays, mean/max cells visited and triangle tests, and the stage's
+    algorithmic bytes -- per active ray 32 B ray in + 16 B hit out (closest) or
+    4 B flag out (any) + 8 B per cell visited (offsets[c], offsets[c+1]) +
+    52 B per triangle test (4 B reference + 48 B triangle record)."""
     out = {}
     cnt = torch.empty(2 * frame.n, dtype=torch.int32, device="cuda:0")
-    stages = [("primary", frame.prim, True), ("shadow", frame.shadow, False),
-              ("bounce", frame.refl, True), ("bounce_shadow", frame.shadow2, False)]
+    rays_of = {"primary": frame.prim, "shadow": frame.shadow, "bounce": frame.refl,
+               "bounce_shadow": frame.shadow2}
     scratch_h = frame.grt.HitBatch.empty(frame.n)
     scratch_o = torch.empty(frame.n, dtype=torch.uint8, device="cuda:0")
-    for i, (name, rays, closest) in enumerate(stages):
+    for i, (name, closest) in enumerate(STAGES):
+        rays = rays_of[name]
         frame.mode(i)
         if closest:
             frame.scene.trace_closest(rays, scratch_h, image_width=WIDTH, counters=cnt)
@@ -218,18 +242,23 @@ def stage_counters(frame, torch):
             frame.scene.trace_any(rays, scratch_o, image_width=WIDTH, counters=cnt)
         active = rays.f["tmin"] <= rays.f["tmax"]
         c = cnt.view(-1, 2)[active].to(torch.int64)
-        if c.shape[0] == 0:
-            continue
-        out[name] = {"rays": int(c.shape[0]),
-                     "mean_cells": round(float(c[:, 0].double().mean().item()), 1),
-                     "mean_tests": round(float(c[:, 1].double().mean().item()), 1),
-                     "max_tests": int(c[:, 1].max().item())}
+        n_act = int(c.shape[0])
+        cells = int(c[:, 0].sum().item()) if n_act else 0
+        tests = int(c[:, 1].sum().item()) if n_act else 0
+        per_ray = 48 if closest else 36
+        out[name] = {"rays": n_act,
+                     "mean_cells": round(cells / max(n_act, 1), 2),
+                     "mean_tests": round(tests / max(n_act, 1), 2),
+                     "max_tests": int(c[:, 1].max().item()) if n_act else 0,
+                     "algorithmic_bytes": per_ray * n_act + 8 * cells + 52 * tests}
     return out
 
 
 def cpu_baseline(frame, torch, budget_s):
-    """Oracle (C, OpenMP over rows, schedule(dynamic)) on a row sample of the
-    frame's primary + shadow rays, and its single-thread grid build."""
+    """The oracle (C, OpenMP over rows, schedule(dynamic)) on the frame's
+    primary + shadow rays: a row sample when the whole frame would take longer
+    than budget_s, else the whole frame repeated until about budget_s has been
+    spent; also its single-thread grid build of the frame."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or orc.max_threads()
@@ -241,36 +270,49 @@ def cpu_baseline(frame, torch, budget_s):
     build_s = time.perf_counter() - t0
     rays = orc.primary_rays(frame.grt.camera_dict(frame.cam))
 
-    def run_rows(stride):
+    def run_rows(stride, chunk_rows=16):
         rows = np.arange(0, HEIGHT, stride)
         idx = (rows[:, None] * WIDTH + np.arange(WIDTH)[None, :]).ravel()
-        sub = {k: a[idx] for k, a in rays.items()}
+        sub = {k: a[idx] for k, a in rays.items()} if stride > 1 else rays
         t = time.perf_counter()
         # MultiThreadRenderer.cpp:23-27: OpenMP schedule(dynamic, 16) over rows.
-        h = orc.trace_closest(fr, offs, refs, v, sub, threads=threads, chunk=16 * WIDTH)
+        h = orc.trace_closest(fr, offs, refs, v, sub, threads=threads, chunk=chunk_rows * WIDTH)
         sh = orc.gen_shadow(sub, h, nrm, LIGHT, OFFSET)
-        orc.trace_any(fr, offs, refs, v, sh, threads=threads, chunk=16 * WIDTH)
+        orc.trace_any(fr, offs, refs, v, sh, threads=threads, chunk=chunk_rows * WIDTH)
         dt = time.perf_counter() - t
         return len(idx) + int((sh["tmin"] <= sh["tmax"]).sum()), dt, len(rows)
 
-    n_probe, dt_probe, _ = run_rows(120)
-    rate = n_probe / max(dt_probe, 1e-6)
-    stride = int(np.clip(np.ceil(HEIGHT * WIDTH * 2 / max(rate * budget_s, 1.0)), 1, 120))
-    n_rays, spent, n_rows = run_rows(stride)
+    # Probe on every 16th row in one-row chunks (enough chunks for every
+    # thread) to size the sample.
+    run_rows(64, 1)
+    n_probe, dt_probe, _ = run_rows(16, 1)
+    full_s = dt_probe * 16
+    if full_s > budget_s:
+        stride = int(np.ceil(full_s / budget_s))
+        n_rays, spent, n_rows = run_rows(stride)
+        what = f"{n_rows} image rows (every {stride}th row)"
+    else:
+        reps = max(1, int(round(budget_s / max(full_s, 1e-3))))
+        n_rays, spent = 0, 0.0
+        for _ in range(reps):
+            n, dt, n_rows = run_rows(1)
+            n_rays += n
+            spent += dt
+        what = f"the whole frame ({HEIGHT} rows) {reps} times"
     return {"value": round(n_rays / spent / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
-            "sample": (f"CPU oracle on {n_rows} image rows (every {stride}th row) of the same "
-                       f"frame: primary closest-hit + shadow any-hit, {n_rays} rays in "
-                       f"{spent:.1f} s, OpenMP schedule(dynamic) over 16-row chunks; "
-                       f"single-thread grid build of the frame {build_s * 1e3:.0f} ms")}
+            "sample": (f"CPU oracle on {what}: primary closest-hit + shadow any-hit, "
+                       f"{n_rays} rays in {spent:.1f} s, OpenMP schedule(dynamic) over 16-row "
+                       f"chunks, same grid (lambda {frame.lam:g}); single-thread grid build of "
+                       f"the frame {build_s * 1e3:.0f} ms")}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of "
-                          "it (rays shard by image tile with the scene replicated, see DESIGN.md)",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark measures one GPU only; with more GPUs the "
+                          "rays would shard by image tile or request over replicated scenes, "
+                          "which is not implemented here", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     import grt
@@ -293,6 +335,8 @@ def main():
     frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill)
     if args.block:
         frame.ctx.set_block_size(args.block, args.block)
+    if args.copy_calibration:
+        copy_calibration(torch)
     lams = [args.lam] if args.lam else list(LAMBDAS)
     lam, sched, tune_table = autotune(frame, lams, args.sched)
     frame.sched = sched
@@ -302,8 +346,7 @@ def main():
     frame.scene.check_grid()
     torch.cuda.synchronize()
 
-    n_ev = 7
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)]
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(Frame.N_EVENTS)]
               for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -312,29 +355,52 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     frame.scene.check_grid()
-
-    stages = {name: 0.0 for name in frame.stage_names}
-    for s in range(args.steps):
-        for i, name in enumerate(frame.stage_names):
-            stages[name] += events[s][i].elapsed_time(events[s][i + 1])
-    stages = {k: v / args.steps for k, v in stages.items()}
     ms_per_step = wall * 1e3 / args.steps
+    # Mean over the timed frames of each launch group (HIP events recorded on
+    # the stream the kernels run on: the context enqueues on torch's stream).
+    d = np.mean([intervals(ev) for ev in events], axis=0)
+    trace_ms = {"primary": d[3], "shadow": d[5], "bounce": d[7], "bounce_shadow": d[9]}
+    stage_ms = {"transform": d[0], "build": d[1], "raygen": d[2] + d[4] + d[6] + d[8],
+                **trace_ms}
+    if args.profile:
+        print(json.dumps({"mode": "profile", "lambda": lam, "schedule": sched,
+                          "ms_per_step": round(ms_per_step, 3),
+                          "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}}))
+        return
 
     rays = {"primary": frame.n, "shadow": frame.active(frame.shadow),
             "bounce": frame.active(frame.refl), "bounce_shadow": frame.active(frame.shadow2)}
-    total_rays = sum(rays.values())
-    value = total_rays / (ms_per_step * 1e-3) / 1e6
+    value = sum(rays.values()) / (ms_per_step * 1e-3) / 1e6
     info = frame.scene.grid_info()
+    counters = stage_counters(frame, torch)
 
-    bytes_launch, mean_c, mean_t = algorithmic_bytes(frame, torch)
-    prim_ms = stages["primary"]
-    achieved = bytes_launch / (prim_ms * 1e-3) / 1e9
+    # Roofline of the dominant traversal kernel: the kernel (by name) with the
+    # most time per frame; achieved = its algorithmic bytes per launch over its
+    # average launch duration, both averaged over its launches in a frame.
+    by_kernel = {}
+    for i, (name, closest) in enumerate(STAGES):
+        k = kernel_name(closest, sched[i])
+        e = by_kernel.setdefault(k, {"stages": [], "ms": 0.0, "bytes": 0})
+        e["stages"].append(name)
+        e["ms"] += trace_ms[name]
+        e["bytes"] += counters[name]["algorithmic_bytes"]
+    dom = max(by_kernel, key=lambda k: by_kernel[k]["ms"])
+    dk = by_kernel[dom]
+    launches = len(dk["stages"])
+    achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "trace_closest_stadium_traffic.md")
+    tpath = os.path.join(ROOT, "profiles", "trace_stadium_traffic.md")
     if os.path.exists(tpath):
         for line in open(tpath):
-            if line.startswith("hbm_bytes_per_launch:"):
+            if line.startswith("hbm_bytes_per_launch[%s]:" % dom):
                 traffic = float(line.split(":", 1)[1])
+    per_stage_roof = {}
+    for name, _ in STAGES:
+        gbs = counters[name]["algorithmic_bytes"] / (trace_ms[name] * 1e-3) / 1e9
+        per_stage_roof[name] = {"gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    ps_bytes = counters["primary"]["algorithmic_bytes"] + counters["shadow"]["algorithmic_bytes"]
+    ps_ms = trace_ms["primary"] + trace_ms["shadow"]
+    ps_gbs = ps_bytes / (ps_ms * 1e-3) / 1e9
 
     out = {
         "metric": "Mrays/s (primary+shadow) at 1 GPU; p50/p99 request latency, serve vs plain",
@@ -355,13 +421,13 @@ def main():
                         f"per-frame transform + device grid rebuild (lambda={lam:g}, autotuned)",
             "rays_per_frame": rays,
             "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"]},
-            "stage_ms": {k: round(v, 3) for k, v in stages.items()},
+            "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
             "mrays_primary_shadow_only": round((rays["primary"] + rays["shadow"]) /
-                                               ((stages["primary"] + stages["shadow"]) * 1e-3) /
-                                               1e6, 2),
-            "mean_cells_per_primary_ray": round(mean_c, 2),
-            "mean_tests_per_primary_ray": round(mean_t, 2),
-            "stage_counters": stage_counters(frame, torch),
+                                               (ps_ms * 1e-3) / 1e6, 2),
+            "roofline_primary_shadow": {"gbs": round(ps_gbs, 1),
+                                        "frac": round(ps_gbs / HBM_PEAK_GBS, 4)},
+            "roofline_by_stage": per_stage_roof,
+            "stage_counters": counters,
             "traversal": {"schedule_per_stage": sched, "refill": args.refill,
                           "block": args.block or 128, "lambda": lam},
             "autotune_frame_ms_by_lambda": tune_table,
@@ -370,8 +436,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_trace_closest (primary)",
-                     "bytes_per_launch": bytes_launch},
+                     "kernel": "grt::" + dom, "stages": dk["stages"],
+                     "avg_launch_ms": round(dk["ms"] / launches, 3),
+                     "algorithmic_bytes_per_launch": dk["bytes"] // launches},
     }
     if not args.no_serve:
         # p50/p99 request latency, continuous batching vs the plain path, on
@@ -384,6 +451,17 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(frame, torch, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def copy_calibration(torch, n=1 << 28, reps=3):
+    """A plain device copy of n float32 (1 GiB read + 1 GiB write), so a
+    counter run can scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
 
 
 if __name__ == "__main__":

@@ -28,7 +28,14 @@ struct DevMem {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
-// PinholeCamera::setLookAt + newFrame on the host (camera.cpp).
+// PinholeCamera::setLookAt + newFrame on the host (camera.cpp).  lookat_basis
+// gives the camera axes (X right, Y up, Z backwards); camera_from_basis is
+// newFrame for a position and axes.
+void lookat_basis(const float eye[3], const float center[3], const float up[3], float ax[3],
+                  float ay[3], float az[3]);
+void camera_from_basis(const float pos[3], const float ax[3], const float ay[3], const float az[3],
+                       float fovy_deg, float znear, uint32_t width, uint32_t height,
+                       grt_camera* out);
 void camera_lookat(const float eye[3], const float center[3], const float up[3], float fovy_deg,
                    float znear, uint32_t width, uint32_t height, grt_camera* out);
 

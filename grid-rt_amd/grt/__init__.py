@@ -118,6 +118,25 @@ _SIGS = {
     "grt_gen_stadium": (_U32, [ctypes.c_uint64, _U32, _U32, _U32, _P, _P, _P]),
     "grt_gen_cluster_transforms": (None, [ctypes.c_uint64, _U32, _U32, _P, _F, _P]),
     "grt_gen_uniform": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64]),
+    # the reference tracer's flat C ABI (include/grt/rt_cabi.h)
+    "rtInitContext": (ctypes.c_bool, []),
+    "rtShutdownContext": (ctypes.c_int, []),
+    "grtLastError": (ctypes.c_char_p, []),
+    "grtRegisterBuffer": (ctypes.c_int, [_P]),
+    "grtUnregisterBuffer": (ctypes.c_int, [_P]),
+    "grtSetScreenSize": (ctypes.c_int, [_U32, _U32]),
+    "grtSetCameraParameters": (ctypes.c_int, [_F] * 12),
+    "grtTransferTriangleVertices": (ctypes.c_int, [_P, _U32]),
+    "grtTransferTriangleNormals": (ctypes.c_int, [_P, _U32]),
+    "grtTransferCellPointers": (ctypes.c_int, [_P, _U32]),
+    "grtTransferCellTriangleIds": (ctypes.c_int, [_P, _U32]),
+    "grtSetBoxMin": (ctypes.c_int, [_F, _F, _F]),
+    "grtSetBoxMax": (ctypes.c_int, [_F, _F, _F]),
+    "grtSetGridParameters": (ctypes.c_int, [_F, _F, _F, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32]),
+    "grtSetBlockSize": (ctypes.c_int, [_U32, _U32]),
+    "grtRayTrace": (ctypes.c_int, []),
+    "grtGetHitRecords": (ctypes.c_int, [_P, _P, _P, _P]),
 }
 
 

@@ -882,3 +882,14 @@ uint32_t orc_headlight_rgba8(const float d[3], const float* n9, int32_t id, floa
     const float b = (0.1f + (1.0f - 0.1f) * ndd) * 1.0f;
     return to_u8(r) | (to_u8(g) << 8) | (to_u8(b) << 16) | (255u << 24);
 }
+
+void orc_headlight_image(uint32_t n, const float* dx, const float* dy, const float* dz,
+                         const float* normals9, const int32_t* id, const float* u,
+                         const float* v, uint32_t* rgba)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        const float d[3] = {dx[i], dy[i], dz[i]};
+        const float* n9 = id[i] >= 0 ? normals9 + (size_t)id[i] * 9 : normals9;
+        rgba[i] = orc_headlight_rgba8(d, n9, id[i], u[i], v[i]);
+    }
+}

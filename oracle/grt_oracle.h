@@ -180,6 +180,11 @@ void orc_brute_any_f64(const float* verts, uint32_t n_tris, uint32_t n_rays,
 
 /* Headlight shading of a hit (RayTracing_kernel.cu:92-120) to RGBA8. */
 uint32_t orc_headlight_rgba8(const float d[3], const float* n9, int32_t id, float u, float v);
+/* The same for n rays: pixel i from ray direction (dx,dy,dz)[i] and hit
+ * (id,u,v)[i]; normals9 holds 9 floats per triangle. */
+void orc_headlight_image(uint32_t n, const float* dx, const float* dy, const float* dz,
+                         const float* normals9, const int32_t* id, const float* u,
+                         const float* v, uint32_t* rgba);
 
 /* Wall-clock seconds (for the CPU baseline). */
 double orc_now(void);

@@ -73,6 +73,7 @@ def lib():
                                       + [ctypes.c_int]),
             "orc_brute_any_f64": (None, [_P, _U32, _U32] + [_P] * 8 + [_P, _P, ctypes.c_int]),
             "orc_headlight_rgba8": (_U32, [_F * 3, _P, ctypes.c_int32, _F, _F]),
+            "orc_headlight_image": (None, [_U32, _P, _P, _P, _P, _P, _P, _P, _P]),
             "orc_now": (ctypes.c_double, []),
             "orc_max_threads": (ctypes.c_int, []),
         }
@@ -335,6 +336,19 @@ def brute_any_f64(verts, rays: dict, threads: int = 0) -> dict:
 
 def headlight(d, n9, hid: int, u: float, v: float) -> int:
     return lib().orc_headlight_rgba8((_F * 3)(*d), _p(_f32(n9).reshape(9)), hid, u, v)
+
+
+def headlight_image(rays: dict, hits: dict, normals) -> np.ndarray:
+    """RGBA8 headlight image (uint32 per ray) of closest-hit records."""
+    n = rays["dx"].shape[0]
+    out = np.empty(n, np.uint32)
+    nrm = _f32(normals).reshape(-1)
+    ids = np.ascontiguousarray(hits["id"], np.int32)
+    d = [_f32(rays[k]) for k in ("dx", "dy", "dz")]
+    uv = [_f32(hits[k]) for k in ("u", "v")]
+    lib().orc_headlight_image(n, _p(d[0]), _p(d[1]), _p(d[2]), _p(nrm), _p(ids), _p(uv[0]),
+                              _p(uv[1]), _p(out))
+    return out
 
 
 def now() -> float:
