@@ -60,6 +60,9 @@ def parse():
                    help="traversal schedule per stage (primary, shadow, bounce, bounce "
                         "shadow): s = one ray per lane, p = persistent refill; default: tuned")
     p.add_argument("--refill", type=int, default=16)
+    p.add_argument("--coop", type=int, default=16,
+                   help="cooperative cell tests in drained persistent waves: minimum "
+                        "references left (0 = off)")
     p.add_argument("--lam", type=float, default=None,
                    help="grid resolution factor; default: tuned over %s" % (LAMBDAS,))
     p.add_argument("--mode", default="frame", choices=["frame", "serve", "batch", "ao"],
@@ -335,6 +338,7 @@ def main():
     frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill)
     if args.block:
         frame.ctx.set_block_size(args.block, args.block)
+    frame.ctx.set_coop(args.coop)
     if args.copy_calibration:
         copy_calibration(torch)
     lams = [args.lam] if args.lam else list(LAMBDAS)
@@ -363,7 +367,7 @@ def main():
     stage_ms = {"transform": d[0], "build": d[1], "raygen": d[2] + d[4] + d[6] + d[8],
                 **trace_ms}
     if args.profile:
-        print(json.dumps({"mode": "profile", "lambda": lam, "schedule": sched,
+        print(json.dumps({"mode": "profile", "lambda": lam, "schedule": sched, "coop": args.coop,
                           "ms_per_step": round(ms_per_step, 3),
                           "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}}))
         return
@@ -429,7 +433,7 @@ def main():
             "roofline_by_stage": per_stage_roof,
             "stage_counters": counters,
             "traversal": {"schedule_per_stage": sched, "refill": args.refill,
-                          "block": args.block or 128, "lambda": lam},
+                          "block": args.block or 128, "lambda": lam, "coop": args.coop},
             "autotune_frame_ms_by_lambda": tune_table,
             "parallelism": "single GPU",
         },

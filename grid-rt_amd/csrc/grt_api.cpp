@@ -293,6 +293,14 @@ int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill)
     return GRT_OK;
 }
 
+int grt_set_coop(grt_context ctx, uint32_t min_refs)
+{
+    if (!ctx)
+        return set_error(GRT_INVALID_PARAM, "grt_set_coop: NULL context");
+    ctx->coop = min_refs;
+    return GRT_OK;
+}
+
 int grt_scene_create(grt_context ctx, grt_scene* out)
 {
     if (!ctx || !out)
@@ -746,6 +754,7 @@ static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint
     a.n = n;
     a.image_width = image_width;
     a.counters = counters;
+    a.coop = sc->ctx->coop;
     return a;
 }
 

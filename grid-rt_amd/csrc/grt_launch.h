@@ -73,6 +73,9 @@ struct TraceArgs {
     uint32_t n;
     uint32_t image_width;   // 0: lane i traces ray i; else 8x8 pixel tiles per wave
     uint32_t* counters;     // NULL or 2 per ray {cells, tests}
+    uint32_t coop;          // persistent kernels: wave-cooperative cell tests once the
+                            // ray queue is drained, for cells with at least this many
+                            // references left (0: off)
 };
 hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
                                 uint32_t block, hipStream_t s);
@@ -106,7 +109,7 @@ hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* m
                            hipStream_t s);
 // Persistent multi-scene traversal over all tasks (mixed closest / any hit).
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
-                              uint32_t* slot_counter, uint32_t block, uint32_t refill,
+                              uint32_t* slot_counter, uint32_t block, uint32_t refill, uint32_t coop,
                               hipStream_t s);
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_gen_shadow(uint32_t n, const RaySoA& in, const HitSoAIn& hits,

@@ -315,6 +315,7 @@ int grt_serve_run(grt_context ctx, const grt_request* reqs, uint32_t n_reqs,
                 a.n = it.n;
                 a.image_width = it.width;
                 a.counters = nullptr;
+                a.coop = ctx->coop;
                 hipError_t e;
                 if (it.primary)
                     e = launch_trace_closest(a, it.rays.in(), it.hits.out(), ctx->block_closest,

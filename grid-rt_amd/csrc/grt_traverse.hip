@@ -427,6 +427,154 @@ __global__ __launch_bounds__(256) void k_trace_any(TraceArgs a, RaySoA rays,
 }
 
 // ---------------------------------------------------------------------------
+// Wave-cooperative cell tests
+// ---------------------------------------------------------------------------
+//
+// Once a persistent wave finds the ray queue drained, its last rays would run
+// their remaining triangle tests one per iteration on single lanes while the
+// other lanes idle; a ray through a dense cell (thousands of references) then
+// sets the length of the whole launch.  In that phase the wave instead tests
+// the rest of such a cell 64 references at a time, one per lane.  The result
+// is the serial one: closest hit keeps the smallest t and, among equal t, the
+// earliest reference (a lexicographic (t, reference) minimum, since the serial
+// loop only replaces on a strictly smaller t); any hit stops at the first
+// occluding reference in order, and the test counter counts the references the
+// serial loop would have tested.
+
+__device__ __forceinline__ float bcast_f(float v, uint32_t l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)l));
+}
+
+__device__ __forceinline__ uint32_t bcast_u(uint32_t v, uint32_t l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
+template <class T>
+__device__ __forceinline__ T* bcast_p(T* p, uint32_t l)
+{
+    const uint64_t x = (uint64_t)p;
+    const uint64_t lo = bcast_u((uint32_t)x, l), hi = bcast_u((uint32_t)(x >> 32), l);
+    return (T*)((hi << 32) | lo);
+}
+
+__device__ __forceinline__ float wave_min_f(float v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = tmin_f(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t w = (uint32_t)__shfl_xor((int)v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// State of one lane's ray that the cooperative pass reads and updates.
+struct LaneRay {
+    float o0, o1, o2, d0, d1, d2, tmin, tmax, bound;
+    uint32_t k, e;        // references [k, e) of the current cell are left
+    int32_t id;
+    float best, bu, bv;
+    uint32_t tests;
+};
+
+// The whole wave (every lane, uniform control flow) finishes the current cell
+// of lane l: records [k, e) of `trir` (lane l's grid).  Updates lane l's k,
+// hit and test count.
+__device__ __forceinline__ void coop_cell(uint32_t l, bool any, const float4* trir, LaneRay& r)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const float o0 = bcast_f(r.o0, l), o1 = bcast_f(r.o1, l), o2 = bcast_f(r.o2, l);
+    const float d0 = bcast_f(r.d0, l), d1 = bcast_f(r.d1, l), d2 = bcast_f(r.d2, l);
+    const float tmin = bcast_f(r.tmin, l), tmax = bcast_f(r.tmax, l);
+    const uint32_t kb = bcast_u(r.k, l), ke = bcast_u(r.e, l);
+    const float4* __restrict__ tr = bcast_p(trir, l);
+    if (any) {
+        uint32_t first = 0xffffffffu;
+        for (uint32_t base = kb; base < ke; base += 64u) {
+            const uint32_t kk = base + lane;
+            bool h = false;
+            if (kk < ke) {
+                const float4 A = tr[3ull * kk], B = tr[3ull * kk + 1], C = tr[3ull * kk + 2];
+                h = any_test(A, B, C, o0, o1, o2, d0, d1, d2, tmin, tmax);
+            }
+            const uint64_t hm = __ballot(h);
+            if (hm != 0) {
+                first = base + (uint32_t)__builtin_ctzll(hm);
+                break;
+            }
+        }
+        if (lane == l) {
+            if (first != 0xffffffffu) {
+                r.tests += first - kb + 1u;
+                r.id = 1;
+            } else {
+                r.tests += ke - kb;
+            }
+            r.k = ke;
+        }
+        return;
+    }
+    const float bound = bcast_f(r.bound, l), best0 = bcast_f(r.best, l);
+    float lb = best0, lu = 0.0f, lv = 0.0f;
+    int32_t lid = -1;
+    uint32_t lk = 0xffffffffu;
+    for (uint32_t base = kb; base < ke; base += 64u) {
+        const uint32_t kk = base + lane;
+        if (kk < ke) {
+            const float4 A = tr[3ull * kk], B = tr[3ull * kk + 1], C = tr[3ull * kk + 2];
+            const MtPre m = mt_pre(A, B, C, o0, o1, o2, d0, d1, d2);
+            if (mt_maybe(m, tmin, tmin_f(tmin_f(tmax, bound), lb))) {
+                float t, u, v;
+                if (mt_exact(m, t, u, v) && t > tmin && t <= tmax && t <= bound && t < lb) {
+                    lb = t;
+                    lu = u;
+                    lv = v;
+                    lid = (int32_t)__float_as_uint(C.y);
+                    lk = kk;
+                }
+            }
+        }
+    }
+    const float mt = wave_min_f(lb);
+    const uint32_t mk = wave_min_u((lk != 0xffffffffu && lb == mt) ? lk : 0xffffffffu);
+    if (mk != 0xffffffffu) {
+        const uint32_t w = (uint32_t)__builtin_ctzll(__ballot(lk == mk));
+        const int32_t wid = (int32_t)bcast_u((uint32_t)lid, w);
+        const float wu = bcast_f(lu, w), wv = bcast_f(lv, w);
+        if (lane == l) {
+            r.best = mt;
+            r.id = wid;
+            r.bu = wu;
+            r.bv = wv;
+        }
+    }
+    if (lane == l) {
+        r.tests += ke - kb;
+        r.k = ke;
+    }
+}
+
+// Lanes whose current cell is worth a cooperative pass now: the queue is
+// drained for this wave and at least max(coop, 2 x busy lanes) references
+// are left.
+__device__ __forceinline__ uint64_t coop_lanes(bool busy, bool drained_lane, uint32_t k,
+                                               uint32_t e, uint32_t coop)
+{
+    if (coop == 0 || __ballot(drained_lane) == 0)
+        return 0;
+    const uint32_t nb = (uint32_t)__popcll(__ballot(busy));
+    const uint32_t thr = coop > 2u * nb ? coop : 2u * nb;
+    return __ballot(busy && k < e && e - k >= thr);
+}
+
+// ---------------------------------------------------------------------------
 // Persistent waves with lane refill
 // ---------------------------------------------------------------------------
 
@@ -521,6 +669,18 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a, RaySoA ra
                 break;
             continue;
         }
+        // ---- drained queue: finish dense cells with the whole wave -----------
+        if (const uint64_t hv0 = coop_lanes(ray >= 0, exhausted, k, e, a.coop)) {
+            LaneRay r = {o0, o1, o2, d0, d1, d2, tmin, tmax, bound, k, e, id, best, bu, bv, tests};
+            for (uint64_t hv = hv0; hv != 0; hv &= hv - 1)
+                coop_cell((uint32_t)__builtin_ctzll(hv), ANY, trir, r);
+            k = r.k;
+            id = r.id;
+            best = r.best;
+            bu = r.bu;
+            bv = r.bv;
+            tests = r.tests;
+        }
         if (ray < 0)
             continue;
 
@@ -547,7 +707,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a, RaySoA ra
             }
         } else {
             if (loaded) {
-                if (!ANY && id >= 0)
+                if (id >= 0)   // closest: hit in this cell; any: occluded
                     finish = true;
                 else if (exit_t > tmax || !dda_step(s))
                     finish = true;
@@ -614,7 +774,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(const BatchTask* __restrict
                                                      const uint32_t* __restrict__ map,
                                                      uint32_t total_slots,
                                                      uint32_t* __restrict__ slot_counter,
-                                                     uint32_t refill)
+                                                     uint32_t refill, uint32_t coop)
 {
     const uint32_t lane = threadIdx.x & 63u;
     int32_t ray = -1;
@@ -695,6 +855,19 @@ __global__ __launch_bounds__(256) void k_trace_batch(const BatchTask* __restrict
                 break;
             continue;
         }
+        if (const uint64_t hv0 = coop_lanes(ray >= 0, exhausted, k, e, coop)) {
+            uint32_t tests = 0;
+            LaneRay r = {o0, o1, o2, d0, d1, d2, tmin, tmax, bound, k, e, id, best, bu, bv, tests};
+            for (uint64_t hv = hv0; hv != 0; hv &= hv - 1) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(hv);
+                coop_cell(l, bcast_u(any ? 1u : 0u, l) != 0, trir, r);
+            }
+            k = r.k;
+            id = r.id;
+            best = r.best;
+            bu = r.bu;
+            bv = r.bv;
+        }
         if (ray < 0)
             continue;
 
@@ -718,7 +891,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(const BatchTask* __restrict
             }
         } else {
             if (loaded) {
-                if (!any && id >= 0)
+                if (id >= 0)   // closest: hit in this cell; any: occluded
                     finish = true;
                 else if (exit_t > tmax || !dda_step(s))
                     finish = true;
@@ -851,7 +1024,7 @@ hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* m
 }
 
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
-                              uint32_t* slot_counter, uint32_t block, uint32_t refill,
+                              uint32_t* slot_counter, uint32_t block, uint32_t refill, uint32_t coop,
                               hipStream_t s)
 {
     if (total_slots == 0)
@@ -862,7 +1035,7 @@ hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint3
     const uint32_t need = (total_slots + block - 1) / block;
     g = g < need ? g : need;
     hipLaunchKernelGGL(k_trace_batch, dim3(g), dim3(block), 0, s, tasks, map, total_slots,
-                       slot_counter, refill);
+                       slot_counter, refill, coop);
     return hipGetLastError();
 }
 

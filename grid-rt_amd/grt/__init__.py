@@ -70,6 +70,7 @@ _SIGS = {
     "grt_memory_used": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "grt_set_block_size": (ctypes.c_int, [_P, _U32, _U32]),
     "grt_set_trace_mode": (ctypes.c_int, [_P, ctypes.c_int, _U32]),
+    "grt_set_coop": (ctypes.c_int, [_P, _U32]),
     "grt_scene_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "grt_scene_destroy": (ctypes.c_int, [_P]),
     "grt_scene_upload_triangles": (ctypes.c_int, [_P, _P, _U32]),
@@ -325,6 +326,10 @@ class Context:
 
     def set_trace_mode(self, persistent: bool = True, refill: int = 16):
         check(lib().grt_set_trace_mode(self.h, 1 if persistent else 0, refill))
+
+    def set_coop(self, min_refs: int = 16):
+        """Cooperative cell tests in drained persistent waves (0: off)."""
+        check(lib().grt_set_coop(self.h, min_refs))
 
     def gen_primary(self, cam: Camera, out: RayBatch):
         check(lib().grt_gen_primary(self.h, ctypes.byref(cam), ctypes.byref(out.c())))

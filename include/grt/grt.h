@@ -90,6 +90,11 @@ int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
 #define GRT_TRACE_SIMPLE     0
 #define GRT_TRACE_PERSISTENT 1
 int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill);
+/* Persistent and batched traversal: once a wave finds the ray queue drained,
+ * the wave tests the rest of a lane's cell 64 references at a time when at
+ * least max(min_refs, 2 x its busy lanes) are left (0 = never; default 16).
+ * Results and counters are those of the one-lane loop. */
+int grt_set_coop(grt_context ctx, uint32_t min_refs);
 
 /* ---- scene ------------------------------------------------------------ */
 int grt_scene_create(grt_context ctx, grt_scene* out);

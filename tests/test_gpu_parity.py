@@ -55,8 +55,11 @@ def check_grid(ctx, v, n, lam):
     return sc, fr, ooffs, orefs
 
 
-# Both traversal schedules must give identical per-ray results and counters.
-MODES = [(False, 16), (True, 16), (True, 1)]
+# Every traversal schedule must give identical per-ray results and counters:
+# (persistent, refill, cooperative threshold).  The persistent waves finish
+# dense cells cooperatively once the queue is drained; thresholds 1 and 2 make
+# nearly every tail cell cooperative, 0 turns the pass off.
+MODES = [(False, 16, 16), (True, 16, 0), (True, 16, 16), (True, 1, 1), (True, 64, 2)]
 
 
 def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
@@ -64,19 +67,21 @@ def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
     n = rays["ox"].shape[0]
     o = orc.trace_closest(fr, offs, refs, v, rays, counters=True)
     rb = grt.RayBatch.from_numpy(rays)
-    for persistent, refill in MODES:
+    for persistent, refill, coop in MODES:
         sc.ctx.set_trace_mode(persistent, refill)
+        sc.ctx.set_coop(coop)
         hb = grt.HitBatch.empty(n)
         cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
         sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
         g = hb.numpy()
         gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-        tag = f"[persistent={persistent} refill={refill}]"
+        tag = f"[persistent={persistent} refill={refill} coop={coop}]"
         for k in ("id", "t", "u", "v"):
             assert_same(g[k], o[k], f"hit.{k} {tag}")
         assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
         assert_same(gc[:, 1], o["tests"], f"triangle tests {tag}")
     sc.ctx.set_trace_mode(True, 16)
+    sc.ctx.set_coop(16)
     return g, o
 
 
@@ -85,17 +90,19 @@ def check_any(sc, fr, offs, refs, v, rays):
     n = rays["ox"].shape[0]
     o = orc.trace_any(fr, offs, refs, v, rays, counters=True)
     rb = grt.RayBatch.from_numpy(rays)
-    for persistent, refill in MODES:
+    for persistent, refill, coop in MODES:
         sc.ctx.set_trace_mode(persistent, refill)
+        sc.ctx.set_coop(coop)
         occ = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
         cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
         sc.trace_any(rb, occ, counters=cnt)
         gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-        tag = f"[persistent={persistent} refill={refill}]"
+        tag = f"[persistent={persistent} refill={refill} coop={coop}]"
         assert_same(occ.cpu().numpy(), o["occluded"], f"occluded {tag}")
         assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
         assert_same(gc[:, 1], o["tests"], f"triangle tests {tag}")
     sc.ctx.set_trace_mode(True, 16)
+    sc.ctx.set_coop(16)
     return o
 
 
@@ -166,6 +173,25 @@ def test_grid_build_edge(gpu_ctx, name):
     v = _edge_scenes()[name]
     n = np.zeros_like(v)
     check_grid(gpu_ctx, v, n, 5.0)
+
+
+@pytest.mark.parametrize("name", ["degenerate", "duplicates", "big_triangle", "hot_cell"])
+def test_trace_edge(gpu_ctx, name):
+    """Traversal on the edge scenes.  Duplicates put exact t ties in one cell
+    (the earliest reference must win, also in cooperative passes); the hot
+    cell holds 40,000 references, so drained waves test it cooperatively."""
+    v = _edge_scenes()[name]
+    n = np.zeros_like(v)
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, n, 5.0)
+    rays = _random_rays(20000, 21, inside=False)
+    # aim half of the rays at the scene's centre so they cross its dense cells
+    c = v.reshape(-1, 3).mean(axis=0)
+    for k, ax in enumerate(("dx", "dy", "dz")):
+        o = rays["o" + ax[1]]
+        rays[ax][::2] = (c[k] - o[::2]).astype(np.float32)
+    g, o = check_closest(sc, fr, offs, refs, v, rays)
+    assert (o["id"] >= 0).sum() > 100
+    check_any(sc, fr, offs, refs, v, rays)
 
 
 def test_grid_build_stadium_small(gpu_ctx):
