@@ -6,7 +6,7 @@ clusters of small triangles + a 2-triangle ground plane 100x the cluster
 extent.  One step = one frame:
   1. per-cluster rigid transform of the rest pose (device),
   2. uniform-grid rebuild on the device (lambda chosen by the autotuner among
-     5..128 before the timed region),
+     8..512 before the timed region),
   3. 1920x1080 primary rays, closest hit,
   4. shadow rays toward a point light, any hit,
   5. one mirror bounce, closest hit,
@@ -39,7 +39,7 @@ TRIS_PER_CLUSTER = 4096
 N_CLUSTER_TRIS = (1 << 20) - 2
 WIDTH, HEIGHT = 1920, 1080
 LAMBDA = 5.0
-LAMBDAS = (5.0, 8.0, 16.0, 32.0, 64.0, 128.0)
+LAMBDAS = (8.0, 32.0, 64.0, 128.0, 256.0, 512.0)
 LIGHT = (1000.0, 1000.0, 1000.0)
 OFFSET = 1e-4
 EYE, CENTER, UP = (0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0.0, 1.0, 0.0)

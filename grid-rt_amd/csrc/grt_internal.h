@@ -50,7 +50,7 @@ struct grt_context_t {
     uint64_t bytes_used = 0;
     uint32_t block_closest = 128;
     uint32_t block_any = 128;
-    int trace_mode = GRT_TRACE_PERSISTENT;
+    int trace_mode = GRT_TRACE_SIMPLE;
     uint32_t refill = 16;
     uint32_t coop = 16;   // cooperative cell tests in drained persistent waves (0: off)
     std::set<grt_scene_t*> scenes;

@@ -107,9 +107,12 @@ struct alignas(16) BatchTask {
 // Map each 64-slot block to its task (n_blocks entries).
 hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* map,
                            hipStream_t s);
-// Persistent multi-scene traversal over all tasks (mixed closest / any hit).
+// Multi-scene traversal over all tasks (mixed closest / any hit): one ray per
+// lane (each wave in one task), or persistent waves refilled from the slot
+// counter.
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
-                              uint32_t* slot_counter, uint32_t block, uint32_t refill, uint32_t coop,
+                              uint32_t* slot_counter, bool persistent, uint32_t block,
+                              uint32_t refill, uint32_t coop,
                               hipStream_t s);
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_gen_shadow(uint32_t n, const RaySoA& in, const HitSoAIn& hits,

@@ -156,7 +156,6 @@ struct Item {
     uint8_t* occ;
     uint32_t n, width;
     bool any;
-    bool primary;
 };
 
 // Enqueue the ray generation of a job's current step and append its traces.
@@ -166,7 +165,7 @@ int issue_step(Job& j, const grt_serve_config& cfg, hipStream_t s, unsigned long
     hipError_t e = hipSuccess;
     if (j.step == 0) {
         e = launch_gen_primary(j.cam, j.ray[0].out(), s);
-        items.push_back(Item{j.sc, j.ray[0], j.hit[0], nullptr, j.n, j.width, false, true});
+        items.push_back(Item{j.sc, j.ray[0], j.hit[0], nullptr, j.n, j.width, false});
     } else {
         const int c = (j.step - 1) & 1;
         const HitSoAIn hin = j.hit[c].in(j.sc->n_tris);
@@ -175,14 +174,13 @@ int issue_step(Job& j, const grt_serve_config& cfg, hipStream_t s, unsigned long
                               cfg.light[2], cfg.offset, j.sh.out(), s);
         if (e == hipSuccess && count)
             e = launch_count_active(j.sh.f[6], j.sh.f[7], j.n, count, s);
-        items.push_back(Item{j.sc, j.sh, HitSet{}, j.occ, j.n, j.width, true, false});
+        items.push_back(Item{j.sc, j.sh, HitSet{}, j.occ, j.n, j.width, true});
         if (e == hipSuccess && j.step < j.depth) {
             e = launch_gen_reflection(j.n, j.ray[c].in(), hin, nrm, cfg.offset,
                                       j.ray[1 - c].out(), s);
             if (e == hipSuccess && count)
                 e = launch_count_active(j.ray[1 - c].f[6], j.ray[1 - c].f[7], j.n, count, s);
-            items.push_back(Item{j.sc, j.ray[1 - c], j.hit[1 - c], nullptr, j.n, j.width, false,
-                                 false});
+            items.push_back(Item{j.sc, j.ray[1 - c], j.hit[1 - c], nullptr, j.n, j.width, false});
         }
     }
     return hip_check(e, "serve: ray generation");
@@ -303,8 +301,8 @@ int grt_serve_run(grt_context ctx, const grt_request* reqs, uint32_t n_reqs,
             if ((rc = grt_batch_trace(batch)) != GRT_OK)
                 break;
         } else {
-            // Plain path: one launch per ray batch, with the stand-alone
-            // schedule (coherent primary rays one per lane, the rest persistent).
+            // Plain path: one launch per ray batch, with the context's
+            // stand-alone schedule.
             for (const Item& it : items) {
                 TraceArgs a;
                 a.fr = it.sc->frame.as<DevFrame>();
@@ -317,13 +315,16 @@ int grt_serve_run(grt_context ctx, const grt_request* reqs, uint32_t n_reqs,
                 a.counters = nullptr;
                 a.coop = ctx->coop;
                 hipError_t e;
-                if (it.primary)
-                    e = launch_trace_closest(a, it.rays.in(), it.hits.out(), ctx->block_closest,
-                                             s);
-                else
+                if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
                     e = launch_trace_persistent(a, it.rays.in(), it.hits.out(), it.occ, it.any,
                                                 it.sc->small_counters.as<uint32_t>() + 2,
-                                                ctx->block_any, ctx->refill, s);
+                                                it.any ? ctx->block_any : ctx->block_closest,
+                                                ctx->refill, s);
+                else if (it.any)
+                    e = launch_trace_any(a, it.rays.in(), it.occ, ctx->block_any, s);
+                else
+                    e = launch_trace_closest(a, it.rays.in(), it.hits.out(), ctx->block_closest,
+                                             s);
                 if ((rc = hip_check(e, "serve: trace")) != GRT_OK)
                     break;
             }

@@ -263,170 +263,6 @@ __device__ __forceinline__ float4 kload4(KF4 p, uint64_t i)
 }
 
 // ---------------------------------------------------------------------------
-// One ray per lane
-// ---------------------------------------------------------------------------
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a, RaySoA rays, HitSoA hits)
-{
-    bool valid;
-    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
-    if (!valid)
-        return;
-    const DevFrame* __restrict__ fr = a.fr;
-    const uint32_t* __restrict__ offsets = a.offsets;
-    const float4* __restrict__ trir = a.trir;
-    const KF4 ktr = (KF4)a.trir;
-    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
-    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
-    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
-    int32_t id = -1;
-    float best = INFINITY, bu = 0.0f, bv = 0.0f;
-    uint32_t cells = 0, tests = 0;
-    Dda s;
-    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
-        const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
-        for (;;) {
-            if (COUNT)
-                ++cells;
-            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
-            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            const uint32_t b = offsets[ci], e = offsets[ci + 1];
-            if (b < e) {
-                const float bound = exit_t * kExitScale;
-                const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
-                const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
-                if (__ballot(b != b0 || e != e0) == 0) {
-                    // Every active lane is in this cell: wave-uniform records.
-                    float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
-                           C = kload4(ktr, 3ull * b0 + 2);
-                    for (uint32_t k = b0; k < e0; ++k) {
-                        const float4 cA = A, cB = B, cC = C;
-                        if (k + 1 < e0) {
-                            const uint64_t q = 3ull * (k + 1);
-                            A = kload4(ktr, q);
-                            B = kload4(ktr, q + 1);
-                            C = kload4(ktr, q + 2);
-                        }
-                        closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id,
-                                     best, bu, bv);
-                    }
-                } else {
-                    float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
-                    for (uint32_t k = b; k < e; ++k) {
-                        const float4 cA = A, cB = B, cC = C;
-                        if (k + 1 < e) {
-                            const uint64_t q = 3ull * (k + 1);
-                            A = trir[q];
-                            B = trir[q + 1];
-                            C = trir[q + 2];
-                        }
-                        closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id,
-                                     best, bu, bv);
-                    }
-                }
-                if (COUNT)
-                    tests += e - b;
-                if (id >= 0)
-                    break;
-            }
-            if (exit_t > tmax)
-                break;
-            if (!dda_step(s))
-                break;
-        }
-    }
-    hits.id[i] = id;
-    hits.t[i] = best;
-    hits.u[i] = bu;
-    hits.v[i] = bv;
-    if (COUNT) {
-        a.counters[2ull * i] = cells;
-        a.counters[2ull * i + 1] = tests;
-    }
-}
-
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_trace_any(TraceArgs a, RaySoA rays,
-                                                   uint8_t* __restrict__ occluded)
-{
-    bool valid;
-    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
-    if (!valid)
-        return;
-    const DevFrame* __restrict__ fr = a.fr;
-    const uint32_t* __restrict__ offsets = a.offsets;
-    const float4* __restrict__ trir = a.trir;
-    const KF4 ktr = (KF4)a.trir;
-    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
-    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
-    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
-    bool occ = false;
-    uint32_t cells = 0, tests = 0;
-    Dda s;
-    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
-        const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
-        for (;;) {
-            if (COUNT)
-                ++cells;
-            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
-            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            const uint32_t b = offsets[ci], e = offsets[ci + 1];
-            if (b < e) {
-                const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
-                const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
-                if (__ballot(b != b0 || e != e0) == 0) {
-                    float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
-                           C = kload4(ktr, 3ull * b0 + 2);
-                    for (uint32_t k = b0; k < e0; ++k) {
-                        const float4 cA = A, cB = B, cC = C;
-                        if (k + 1 < e0) {
-                            const uint64_t q = 3ull * (k + 1);
-                            A = kload4(ktr, q);
-                            B = kload4(ktr, q + 1);
-                            C = kload4(ktr, q + 2);
-                        }
-                        if (COUNT)
-                            ++tests;
-                        if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax)) {
-                            occ = true;
-                            break;
-                        }
-                    }
-                } else {
-                    float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
-                    for (uint32_t k = b; k < e; ++k) {
-                        const float4 cA = A, cB = B, cC = C;
-                        if (k + 1 < e) {
-                            const uint64_t q = 3ull * (k + 1);
-                            A = trir[q];
-                            B = trir[q + 1];
-                            C = trir[q + 2];
-                        }
-                        if (COUNT)
-                            ++tests;
-                        if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax)) {
-                            occ = true;
-                            break;
-                        }
-                    }
-                }
-            }
-            if (occ)
-                break;
-            if (exit_t > tmax)
-                break;
-            if (!dda_step(s))
-                break;
-        }
-    }
-    occluded[i] = occ ? 1 : 0;
-    if (COUNT) {
-        a.counters[2ull * i] = cells;
-        a.counters[2ull * i + 1] = tests;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Wave-cooperative cell tests
 // ---------------------------------------------------------------------------
 //
@@ -572,6 +408,281 @@ __device__ __forceinline__ uint64_t coop_lanes(bool busy, bool drained_lane, uin
     const uint32_t nb = (uint32_t)__popcll(__ballot(busy));
     const uint32_t thr = coop > 2u * nb ? coop : 2u * nb;
     return __ballot(busy && k < e && e - k >= thr);
+}
+
+// ---------------------------------------------------------------------------
+// One ray per lane
+// ---------------------------------------------------------------------------
+//
+// Each iteration of the wave-uniform loop handles one cell per busy lane:
+//   1. the cooperative pass (cells with many references, when the busy lanes
+//      of the wave are not all in one cell),
+//   2. the remaining references of each lane's cell -- with wave-uniform
+//      (scalar) record loads when every busy lane is in the same cell,
+//   3. finish (hit in this cell / ray leaves its interval or the grid) or step
+//      to the next cell.
+
+// Lanes for a cooperative pass in the one-ray-per-lane kernels: none when all
+// busy lanes share one cell (the scalar path serves them better), else those
+// with at least max(coop, 2 x busy lanes) references left.
+__device__ __forceinline__ uint64_t coop_lanes_simple(bool busy, uint32_t k, uint32_t e,
+                                                      uint32_t coop)
+{
+    if (coop == 0)
+        return 0;
+    const uint64_t bm = __ballot(busy);
+    const uint32_t f = (uint32_t)__builtin_ctzll(bm);
+    const uint32_t k0 = bcast_u(k, f), e0 = bcast_u(e, f);
+    if (__ballot(busy && (k != k0 || e != e0)) == 0)
+        return 0;
+    const uint32_t nb = (uint32_t)__popcll(bm);
+    const uint32_t thr = coop > 2u * nb ? coop : 2u * nb;
+    return __ballot(busy && k < e && e - k >= thr);
+}
+
+// Current cell of a lane: exit distance, reference range, hit bound.
+__device__ __forceinline__ void load_cell(const Dda& s, const uint32_t* __restrict__ offsets,
+                                          uint32_t nx, uint32_t ny, float& exit_t, float& bound,
+                                          uint32_t& k, uint32_t& e)
+{
+    exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+    const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+    k = offsets[ci];
+    e = offsets[ci + 1];
+    bound = exit_t * kExitScale;
+}
+
+// Closest hit of ray i (valid lanes) against one grid; every lane of the wave
+// calls it together.
+template <bool COUNT>
+__device__ __forceinline__ void trace_closest_lane(const DevFrame* __restrict__ fr,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const float4* __restrict__ trir,
+                                                   const RaySoA& rays, const HitSoA& hits,
+                                                   uint32_t* __restrict__ counters, uint32_t coop,
+                                                   uint32_t i, bool valid)
+{
+    const KF4 ktr = (KF4)trir;
+    LaneRay r = {};
+    r.id = -1;
+    r.best = INFINITY;
+    uint32_t cells = 0;
+    Dda s = {};
+    bool busy = false;
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    if (valid) {
+        r.o0 = rays.ox[i];
+        r.o1 = rays.oy[i];
+        r.o2 = rays.oz[i];
+        r.d0 = rays.dx[i];
+        r.d1 = rays.dy[i];
+        r.d2 = rays.dz[i];
+        r.tmin = rays.tmin[i];
+        r.tmax = rays.tmax[i];
+        busy = grid_ok(fr) && dda_setup(fr, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, s);
+    }
+    float exit_t = 0.0f;
+    if (busy) {
+        load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
+        cells = 1;
+    }
+    while (__ballot(busy) != 0) {
+        if (const uint64_t hv0 = coop_lanes_simple(busy, r.k, r.e, coop))
+            for (uint64_t hv = hv0; hv != 0; hv &= hv - 1)
+                coop_cell((uint32_t)__builtin_ctzll(hv), false, trir, r);
+        if (busy && r.k < r.e) {
+            const uint32_t b = r.k, e = r.e;
+            const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+            const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
+            if (__ballot(b != b0 || e != e0) == 0) {
+                // Every lane here is in this cell: wave-uniform records.
+                float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
+                       C = kload4(ktr, 3ull * b0 + 2);
+                for (uint32_t k = b0; k < e0; ++k) {
+                    const float4 cA = A, cB = B, cC = C;
+                    if (k + 1 < e0) {
+                        const uint64_t q = 3ull * (k + 1);
+                        A = kload4(ktr, q);
+                        B = kload4(ktr, q + 1);
+                        C = kload4(ktr, q + 2);
+                    }
+                    closest_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax,
+                                 r.bound, r.id, r.best, r.bu, r.bv);
+                }
+            } else {
+                float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
+                for (uint32_t k = b; k < e; ++k) {
+                    const float4 cA = A, cB = B, cC = C;
+                    if (k + 1 < e) {
+                        const uint64_t q = 3ull * (k + 1);
+                        A = trir[q];
+                        B = trir[q + 1];
+                        C = trir[q + 2];
+                    }
+                    closest_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax,
+                                 r.bound, r.id, r.best, r.bu, r.bv);
+                }
+            }
+            r.tests += e - b;
+            r.k = e;
+        }
+        if (busy) {
+            if (r.id >= 0 || exit_t > r.tmax || !dda_step(s)) {
+                busy = false;
+            } else {
+                load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
+                ++cells;
+            }
+        }
+    }
+    if (valid) {
+        hits.id[i] = r.id;
+        hits.t[i] = r.best;
+        hits.u[i] = r.bu;
+        hits.v[i] = r.bv;
+        if (COUNT) {
+            counters[2ull * i] = cells;
+            counters[2ull * i + 1] = r.tests;
+        }
+    }
+}
+
+// Any hit of ray i (valid lanes) against one grid; every lane of the wave calls
+// it together.
+template <bool COUNT>
+__device__ __forceinline__ void trace_any_lane(const DevFrame* __restrict__ fr,
+                                               const uint32_t* __restrict__ offsets,
+                                               const float4* __restrict__ trir,
+                                               const RaySoA& rays, uint8_t* __restrict__ occluded,
+                                               uint32_t* __restrict__ counters, uint32_t coop,
+                                               uint32_t i, bool valid)
+{
+    const KF4 ktr = (KF4)trir;
+    LaneRay r = {};
+    r.id = -1;
+    uint32_t cells = 0;
+    Dda s = {};
+    bool busy = false;
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    if (valid) {
+        r.o0 = rays.ox[i];
+        r.o1 = rays.oy[i];
+        r.o2 = rays.oz[i];
+        r.d0 = rays.dx[i];
+        r.d1 = rays.dy[i];
+        r.d2 = rays.dz[i];
+        r.tmin = rays.tmin[i];
+        r.tmax = rays.tmax[i];
+        busy = grid_ok(fr) && dda_setup(fr, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, s);
+    }
+    float exit_t = 0.0f;
+    if (busy) {
+        load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
+        cells = 1;
+    }
+    while (__ballot(busy) != 0) {
+        if (const uint64_t hv0 = coop_lanes_simple(busy, r.k, r.e, coop))
+            for (uint64_t hv = hv0; hv != 0; hv &= hv - 1)
+                coop_cell((uint32_t)__builtin_ctzll(hv), true, trir, r);
+        if (busy && r.k < r.e) {
+            const uint32_t b = r.k, e = r.e;
+            const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+            const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
+            uint32_t k = b;
+            if (__ballot(b != b0 || e != e0) == 0) {
+                float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
+                       C = kload4(ktr, 3ull * b0 + 2);
+                for (k = b0; k < e0; ++k) {
+                    const float4 cA = A, cB = B, cC = C;
+                    if (k + 1 < e0) {
+                        const uint64_t q = 3ull * (k + 1);
+                        A = kload4(ktr, q);
+                        B = kload4(ktr, q + 1);
+                        C = kload4(ktr, q + 2);
+                    }
+                    if (any_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                                 r.tmax)) {
+                        r.id = 1;
+                        break;
+                    }
+                }
+            } else {
+                float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
+                for (; k < e; ++k) {
+                    const float4 cA = A, cB = B, cC = C;
+                    if (k + 1 < e) {
+                        const uint64_t q = 3ull * (k + 1);
+                        A = trir[q];
+                        B = trir[q + 1];
+                        C = trir[q + 2];
+                    }
+                    if (any_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                                 r.tmax)) {
+                        r.id = 1;
+                        break;
+                    }
+                }
+            }
+            // references tested: up to and including an occluder
+            r.tests += (r.id >= 0 ? k + 1 : e) - b;
+            r.k = e;
+        }
+        if (busy) {
+            if (r.id >= 0 || exit_t > r.tmax || !dda_step(s)) {
+                busy = false;
+            } else {
+                load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
+                ++cells;
+            }
+        }
+    }
+    if (valid) {
+        occluded[i] = r.id >= 0 ? 1 : 0;
+        if (COUNT) {
+            counters[2ull * i] = cells;
+            counters[2ull * i + 1] = r.tests;
+        }
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a, RaySoA rays, HitSoA hits)
+{
+    bool valid;
+    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
+    trace_closest_lane<COUNT>(a.fr, a.offsets, a.trir, rays, hits, a.counters, a.coop, i, valid);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_any(TraceArgs a, RaySoA rays,
+                                                   uint8_t* __restrict__ occluded)
+{
+    bool valid;
+    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
+    trace_any_lane<COUNT>(a.fr, a.offsets, a.trir, rays, occluded, a.counters, a.coop, i, valid);
+}
+
+// Batched multi-scene traversal, one ray per lane: every 64-slot group belongs
+// to one task (map[slot / 64]), so each wave runs the single-scene loop of its
+// task with wave-uniform scene data.
+__global__ __launch_bounds__(256) void k_trace_batch_simple(const BatchTask* __restrict__ tasks,
+                                                            const uint32_t* __restrict__ map,
+                                                            uint32_t total_slots, uint32_t coop)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t group = __builtin_amdgcn_readfirstlane(slot >> 6);
+    if ((group << 6) >= total_slots)
+        return;   // whole waves only: total_slots is a multiple of 64
+    const uint32_t task = __builtin_amdgcn_readfirstlane(map[group]);
+    const BatchTask& T = tasks[task];
+    bool valid;
+    const uint32_t i = slot_ray(slot - T.slot_begin, T.image_width, T.n, valid);
+    if (T.any)
+        trace_any_lane<false>(T.fr, T.offsets, T.trir, T.rays, T.occluded, nullptr, coop, i,
+                              valid);
+    else
+        trace_closest_lane<false>(T.fr, T.offsets, T.trir, T.rays, T.hits, nullptr, coop, i,
+                                  valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -1024,11 +1135,16 @@ hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* m
 }
 
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
-                              uint32_t* slot_counter, uint32_t block, uint32_t refill, uint32_t coop,
-                              hipStream_t s)
+                              uint32_t* slot_counter, bool persistent, uint32_t block,
+                              uint32_t refill, uint32_t coop, hipStream_t s)
 {
     if (total_slots == 0)
         return hipSuccess;
+    if (!persistent) {
+        hipLaunchKernelGGL(k_trace_batch_simple, dim3((total_slots + block - 1) / block),
+                           dim3(block), 0, s, tasks, map, total_slots, coop);
+        return hipGetLastError();
+    }
     refill = refill < 1 ? 1 : (refill > 64 ? 64 : refill);
     hipLaunchKernelGGL(k_fill_u32_one, dim3(1), dim3(64), 0, s, slot_counter);
     uint32_t g = resident_blocks(k_trace_batch, block);

@@ -142,21 +142,16 @@ class BatchedJobs:
             self.batches[lvl].trace()
 
     def run_plain(self):
-        """One job after another, one launch per ray batch (the stand-alone
-        schedule: primary rays one per lane, the rest persistent)."""
+        """One job after another, one launch per ray batch, with the context's
+        current schedule."""
         for j in self.jobs:
-            sc = j["scene"]
             for lvl in range(j["depth"] + 1):
                 self._gen(j, lvl)
                 for (s, rays, hits, occ, width) in self._items(j, lvl):
-                    self.ctx.set_trace_mode(lvl != 0, self.ctx_refill)
                     if hits is not None:
                         s.trace_closest(rays, hits, image_width=width)
                     else:
                         s.trace_any(rays, occ, image_width=width)
-            self.ctx.set_trace_mode(True, self.ctx_refill)
-
-    ctx_refill = 16
 
     def active_rays(self):
         torch = _torch()
@@ -190,8 +185,9 @@ def _time(fn, reps, torch):
 
 
 def batch_config(reps: int = 5):
-    """Batched (one persistent launch per level, hipGraph) vs plain; the
-    autotuner picks lambda in {2,5,8}, block size and refill threshold."""
+    """Batched (one launch per level, hipGraph) vs plain; the autotuner picks
+    the schedule (one ray per lane or persistent), lambda in {2,5,8}, block
+    size and refill threshold."""
     torch = _torch()
     ctx = Context(0)
     stream = stream_create()
@@ -200,8 +196,10 @@ def batch_config(reps: int = 5):
     jobs = W.batch_jobs()
     bj = BatchedJobs(ctx, jobs, 5.0)
     tuner = Autotuner()
-    cands = [{"lam": lam, "block": blk, "refill": rf}
-             for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256) for rf in (8, 16, 32)]
+    cands = [{"lam": lam, "block": blk, "persistent": False, "refill": 16}
+             for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256)]
+    cands += [{"lam": lam, "block": blk, "persistent": True, "refill": rf}
+              for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256) for rf in (8, 16, 32)]
     cur = {"lam": 5.0}
 
     def measure(c):
@@ -209,7 +207,7 @@ def batch_config(reps: int = 5):
             bj.set_lambda(c["lam"])
             cur["lam"] = c["lam"]
         ctx.set_block_size(c["block"], c["block"])
-        ctx.set_trace_mode(True, c["refill"])
+        ctx.set_trace_mode(c["persistent"], c["refill"])
         bj.run_batched()
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -221,8 +219,7 @@ def batch_config(reps: int = 5):
     if best["lam"] != cur["lam"]:
         bj.set_lambda(best["lam"])
     ctx.set_block_size(best["block"], best["block"])
-    ctx.set_trace_mode(True, best["refill"])
-    BatchedJobs.ctx_refill = best["refill"]
+    ctx.set_trace_mode(best["persistent"], best["refill"])
     rays = bj.active_rays()
     g = Graph(ctx)
     g.begin()
@@ -237,13 +234,21 @@ def batch_config(reps: int = 5):
 
     t_graph = _time(replay, reps, torch)
     t_eager = _time(lambda: (bj.run_batched(), ctx.synchronize()), reps, torch)
-    t_plain = _time(lambda: (bj.run_plain(), ctx.synchronize()), reps, torch)
+    # the plain path at the same lambda, with its better schedule
+    t_plain_by = {}
+    for persistent in (False, True):
+        ctx.set_trace_mode(persistent, best["refill"])
+        t_plain_by["persistent" if persistent else "simple"] = \
+            _time(lambda: (bj.run_plain(), ctx.synchronize()), reps, torch)
+    t_plain = min(t_plain_by.values())
+    ctx.set_trace_mode(best["persistent"], best["refill"])
     g.close()
     out = {"jobs": len(jobs), "triangles": int(sum(j["tris"] for j in bj.jobs)),
            "primary_rays": int(sum(j["n"] for j in bj.jobs)), "rays_per_run": rays,
            "tuned": best,
            "batched_graph_ms": round(t_graph * 1e3, 3), "batched_eager_ms": round(t_eager * 1e3, 3),
            "plain_ms": round(t_plain * 1e3, 3),
+           "plain_ms_by_schedule": {k: round(v * 1e3, 3) for k, v in t_plain_by.items()},
            "mrays_s": {"batched_graph": round(rays / t_graph / 1e6, 2),
                        "batched_eager": round(rays / t_eager / 1e6, 2),
                        "plain": round(rays / t_plain / 1e6, 2)},

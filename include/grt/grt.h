@@ -83,10 +83,12 @@ int grt_memory_used(grt_context ctx, uint64_t* bytes);
  * cudaSetBlockSize (RayTracing.h:39) was a no-op in the reference. */
 int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
 
-/* Traversal scheduling.  GRT_TRACE_SIMPLE: one ray per lane for the whole
- * launch.  GRT_TRACE_PERSISTENT (default): persistent waves whose idle lanes
- * fetch new rays once `refill` (1..64) lanes of a wave are idle.  Results are
- * identical; only the schedule differs. */
+/* Traversal scheduling.  GRT_TRACE_SIMPLE (default): one ray per lane for the
+ * whole launch.  GRT_TRACE_PERSISTENT: persistent waves whose idle lanes fetch
+ * new rays once `refill` (1..64) lanes of a wave are idle.  Both finish dense
+ * cells wave-cooperatively (grt_set_coop).  Results are identical; only the
+ * schedule differs.  Batches (grt_batch_trace) and the serving loop follow the
+ * same setting. */
 #define GRT_TRACE_SIMPLE     0
 #define GRT_TRACE_PERSISTENT 1
 int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill);

@@ -69,20 +69,36 @@ def test_batch_items_match_oracle(gpu_ctx, three_scenes):
             items.append((sc, rb, hb, None, width))
             outs.append(hb)
             want.append(orc.trace_closest(fr, offs, refs, v, rays))
-    for order in (list(range(len(items))), list(reversed(range(len(items))))):
+    # both batched kernels (one ray per lane, persistent), cooperative pass on
+    # and off, two item orders
+    for persistent, coop, rev in ((False, 16, False), (False, 0, True), (True, 16, True),
+                                  (True, 1, False)):
+        gpu_ctx.set_trace_mode(persistent, 16)
+        gpu_ctx.set_coop(coop)
+        order = list(range(len(items)))
+        if rev:
+            order.reverse()
+        for out in outs:   # stale results must not pass for new ones
+            if isinstance(out, grt.HitBatch):
+                out.f["id"].fill_(-3)
+            else:
+                out.fill_(7)
         batch = grt.Batch(gpu_ctx, 16)
         batch.set([items[i] for i in order])
         assert batch.slots() % 64 == 0
         batch.trace()
         torch.cuda.synchronize()
+        tag = f"[persistent={persistent} coop={coop}]"
         for k, (spec, out, w) in enumerate(zip(specs, outs, want)):
             if spec[2]:
-                assert_same(out.cpu().numpy(), w, f"item {k} occluded")
+                assert_same(out.cpu().numpy(), w, f"item {k} occluded {tag}")
             else:
                 g = out.numpy()
                 for f in ("id", "t", "u", "v"):
-                    assert_same(g[f], w[f], f"item {k} hit.{f}")
+                    assert_same(g[f], w[f], f"item {k} hit.{f} {tag}")
         batch.close()
+    gpu_ctx.set_trace_mode(False, 16)
+    gpu_ctx.set_coop(16)
 
 
 def test_batch_rejects_bad_items(gpu_ctx, three_scenes):
@@ -164,6 +180,18 @@ def test_serve_result_same_alone_and_in_mix(gpu_ctx, three_scenes):
 
     mix, res_mix = make(reqs, True, [0.0] * len(reqs))
     assert np.all(res_mix[2] >= res_mix[1]) and np.all(res_mix[3] > 0)
+    # the persistent batched kernel serves the same mix identically
+    gpu_ctx.set_trace_mode(True, 16)
+    try:
+        mixp, _ = make(reqs, True, [0.0] * len(reqs))
+    finally:
+        gpu_ctx.set_trace_mode(False, 16)
+    for k in range(len(reqs)):
+        a, b = mix[k][0].numpy(), mixp[k][0].numpy()
+        for f in ("id", "t", "u", "v"):
+            assert_same(a[f], b[f], f"request {k} primary hit.{f} (persistent batch)")
+        assert_same(mix[k][1].cpu().numpy(), mixp[k][1].cpu().numpy(),
+                    f"request {k} shadow (persistent batch)")
     for k, r in enumerate(reqs):
         alone, res_alone = make([r], False, [0.0])
         assert res_alone[3][0] == res_mix[3][k], "same active-ray count alone and in the mix"
