@@ -327,7 +327,8 @@ def main():
         if args.mode == "serve":
             ctx = grt.Context(0)
             scenes, tris = perf.build_serve_scenes(ctx)
-            res = perf.serve_vs_plain(ctx, scenes, n_requests=args.serve_requests)
+            res = perf.serve_vs_plain(ctx, scenes, n_requests=args.serve_requests,
+                                      loads=(0.5, 0.8, 0.95, 1.1))
             res["triangles"] = tris
         elif args.mode == "batch":
             res = perf.batch_config()

@@ -116,6 +116,8 @@ int grid_reserve_refs(grt_scene_t* sc, uint32_t refs)
     const uint64_t list_cap = cap / (kSortSmallCells + 1) + 1;
     if ((rc = sc->sort_list.reserve(sc->ctx, list_cap * 4, false)) != GRT_OK)
         return rc;
+    if ((rc = sc->sort_list_small.reserve(sc->ctx, (cap / 2 + 1) * 4, false)) != GRT_OK)
+        return rc;
     if ((rc = sc->trir.reserve(sc->ctx, cap * 48, false)) != GRT_OK)
         return rc;
     sc->refs_cap = (uint32_t)cap;
@@ -146,7 +148,25 @@ static void frame_to_info(const DevFrame& f, grt_grid_info* out)
 }
 
 // Count -> scan -> scatter -> sort, all on the stream.
-static int enqueue_build_passes(grt_scene_t* sc, uint32_t refs_cap)
+// The per-cell sort lists (empty before the reference buffers exist).
+static SortLists sort_lists(grt_scene_t* sc)
+{
+    if (!sc->sort_list_small.ptr || !sc->sort_list.ptr)
+        return SortLists{};
+    return SortLists{sc->sort_list_small.as<uint32_t>(), sc->sort_list.as<uint32_t>(),
+                     sc->small_counters.as<uint32_t>() + 4, sc->refs_cap / 2 + 1, sc->list_cap};
+}
+
+static int enqueue_scan(grt_scene_t* sc, uint32_t refs_cap, bool with_lists)
+{
+    return hip_check(launch_grid_scan(sc->frame.as<DevFrame>(), sc->counts.as<uint32_t>(),
+                                      sc->offsets.as<uint32_t>(), sc->block_sums.as<uint32_t>(),
+                                      sc->cells_cap, refs_cap,
+                                      with_lists ? sort_lists(sc) : SortLists{}, sc->ctx->stream),
+                     "grid scan");
+}
+
+static int enqueue_build_passes(grt_scene_t* sc, uint32_t refs_cap, bool with_lists)
 {
     hipStream_t s = sc->ctx->stream;
     DevFrame* fr = sc->frame.as<DevFrame>();
@@ -157,9 +177,7 @@ static int enqueue_build_passes(grt_scene_t* sc, uint32_t refs_cap)
                                           sc->small_counters.as<uint32_t>(), s),
                         "grid count")) != GRT_OK)
         return rc;
-    return hip_check(launch_grid_scan(fr, sc->counts.as<uint32_t>(), sc->offsets.as<uint32_t>(),
-                                      sc->block_sums.as<uint32_t>(), sc->cells_cap, refs_cap, s),
-                     "grid scan");
+    return enqueue_scan(sc, refs_cap, with_lists);
 }
 
 static int enqueue_scatter_sort(grt_scene_t* sc)
@@ -174,8 +192,7 @@ static int enqueue_scatter_sort(grt_scene_t* sc)
                         "grid scatter")) != GRT_OK)
         return rc;
     if ((rc = hip_check(launch_grid_sort(fr, sc->offsets.as<uint32_t>(), sc->refs.as<uint32_t>(),
-                                         nullptr, sc->sort_list.as<uint32_t>(), sc->list_cap,
-                                         sc->small_counters.as<uint32_t>() + 1, sc->cells_cap, s),
+                                         sort_lists(sc), s),
                         "grid sort")) != GRT_OK)
         return rc;
     return hip_check(launch_gather_refs(fr, sc->refs.as<uint32_t>(), sc->tri.as<float4>(),
@@ -538,13 +555,17 @@ int grt_grid_build(grt_scene sc, float lambda)
                          host.n[1], host.n[2]);
     if ((rc = grid_reserve_cells(sc, host.m)) != GRT_OK)
         return rc;
-    if ((rc = enqueue_build_passes(sc, 0xffffffffu)) != GRT_OK)
+    // Count and scan to learn the reference total, size the reference
+    // buffers, then scan again to collect the sort lists into them.
+    if ((rc = enqueue_build_passes(sc, 0xffffffffu, false)) != GRT_OK)
         return rc;
     if ((rc = fetch_frame(sc, &host)) != GRT_OK)
         return rc;
     if (host.status != 0)
         return set_error(GRT_CAPACITY, "grid build status 0x%x", host.status);
     if ((rc = grid_reserve_refs(sc, host.r > 0 ? host.r : 1)) != GRT_OK)
+        return rc;
+    if ((rc = enqueue_scan(sc, sc->refs_cap, true)) != GRT_OK)
         return rc;
     if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
         return rc;
@@ -586,7 +607,7 @@ int grt_grid_rebuild_async(grt_scene sc, float lambda)
                                                 sc->frame.as<DevFrame>(), s),
                         "grid frame")) != GRT_OK)
         return rc;
-    if ((rc = enqueue_build_passes(sc, sc->refs_cap)) != GRT_OK)
+    if ((rc = enqueue_build_passes(sc, sc->refs_cap, true)) != GRT_OK)
         return rc;
     if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
         return rc;

@@ -457,12 +457,17 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* __restrict__ bsu
     }
 }
 
+// With lists.small set, also appends every cell with 2..kSortSmallCells
+// references to lists.small and every larger cell to lists.large (the cells
+// the sort pass has to visit), one atomic per block and list.
 __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n_host,
                                                     const DevFrame* fr,
                                                     const uint32_t* __restrict__ bsum,
-                                                    uint32_t* __restrict__ out)
+                                                    uint32_t* __restrict__ out, SortLists lists)
 {
     __shared__ uint32_t lds[8];
+    __shared__ uint32_t lds2[8];
+    __shared__ uint32_t list_base[2];
     uint32_t n;
     if (!scan_len(fr, n_host, n))
         return;
@@ -476,6 +481,36 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
 #pragma unroll
     for (int k = 0; k < 16; ++k)
         s += v[k];
+    if (lists.small && !(fr->status & kStatusRefsOver)) {
+        uint32_t ns = 0, nl = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            ns += (v[k] >= 2u && v[k] <= kSortSmallCells) ? 1u : 0u;
+            nl += (v[k] > kSortSmallCells) ? 1u : 0u;
+        }
+        uint32_t ts, tl;
+        uint32_t ps = block_excl_scan<256>(ns, lds, ts);
+        uint32_t pl = block_excl_scan<256>(nl, lds2, tl);
+        if (threadIdx.x == 0) {
+            list_base[0] = ts ? atomicAdd(lists.counts, ts) : 0u;
+            list_base[1] = tl ? atomicAdd(lists.counts + 1, tl) : 0u;
+        }
+        __syncthreads();
+        ps += list_base[0];
+        pl += list_base[1];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (v[k] >= 2u && v[k] <= kSortSmallCells) {
+                if (ps < lists.small_cap)
+                    lists.small[ps] = i0 + (uint32_t)k;
+                ++ps;
+            } else if (v[k] > kSortSmallCells) {
+                if (pl < lists.large_cap)
+                    lists.large[pl] = i0 + (uint32_t)k;
+                ++pl;
+            }
+        }
+    }
     uint32_t total;
     uint32_t p = block_excl_scan<256>(s, lds, total) + bsum[blockIdx.x];
     uint32_t o[16];
@@ -508,29 +543,11 @@ __device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
     b = hi;
 }
 
-// Cells of 2..8 references: Batcher odd-even merge network in registers.
-// Larger cells go to `list` for k_sort_large.
-__global__ __launch_bounds__(256) void k_sort_small(const DevFrame* __restrict__ fr,
-                                                    const uint32_t* __restrict__ offsets,
-                                                    uint32_t* __restrict__ refs,
-                                                    uint32_t* __restrict__ list, uint32_t list_cap,
-                                                    uint32_t* __restrict__ list_count)
+__device__ __forceinline__ void sort_small_cell(const uint32_t* __restrict__ offsets,
+                                                uint32_t* __restrict__ refs, uint32_t c)
 {
-    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
-        return;
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= fr->m)
-        return;
     const uint32_t b = offsets[c];
     const uint32_t cnt = offsets[c + 1] - b;
-    if (cnt <= 1)
-        return;
-    if (cnt > kSortSmallCells) {
-        const uint32_t slot = atomicAdd(list_count, 1u);
-        if (slot < list_cap)
-            list[slot] = c;
-        return;
-    }
     uint32_t a[8];
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k)
@@ -551,6 +568,21 @@ __global__ __launch_bounds__(256) void k_sort_small(const DevFrame* __restrict__
     for (uint32_t k = 0; k < 8; ++k)
         if (k < cnt)
             refs[b + k] = a[k];
+}
+
+// Cells of 2..8 references (lists.small): Batcher odd-even merge network in
+// registers, one thread per cell.
+__global__ __launch_bounds__(256) void k_sort_small(const DevFrame* __restrict__ fr,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ refs, SortLists lists)
+{
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    uint32_t nl = lists.counts[0];
+    nl = nl < lists.small_cap ? nl : lists.small_cap;
+    for (uint32_t li = blockIdx.x * blockDim.x + threadIdx.x; li < nl;
+         li += gridDim.x * blockDim.x)
+        sort_small_cell(offsets, refs, lists.small[li]);
 }
 
 // Bitonic sort written with ascending comparators only (the first comparator
@@ -615,6 +647,106 @@ __device__ void global_pass(uint32_t* __restrict__ a, uint32_t n, uint32_t p, ui
     __syncthreads();
 }
 
+// One wave sorts one cell of up to 64*E references in registers: element
+// i = e*64 + lane sits in lane `lane`, register e; the bitonic network compares
+// i with i^j ascending when (i & k) == 0 -- across lanes by shuffles (j < 64),
+// within a lane between registers (j >= 64).  Slots past n hold 0xffffffff,
+// which sorts last.
+// One comparator layer (k = 2^LK, j = 2^LJ), then the next smaller j.
+template <int E, int LK, int LJ>
+__device__ __forceinline__ void bitonic_layer(uint32_t (&x)[E], uint32_t lane)
+{
+    constexpr uint32_t k = 1u << LK, j = 1u << LJ;
+    if constexpr (j >= 64u) {
+        constexpr uint32_t je = j >> 6;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (((uint32_t)e & je) == 0) {
+                const int f = e ^ (int)je;
+                const bool up = (((uint32_t)e * 64u) & k) == 0;
+                const uint32_t lo = x[e] < x[f] ? x[e] : x[f];
+                const uint32_t hi = x[e] < x[f] ? x[f] : x[e];
+                x[e] = up ? lo : hi;
+                x[f] = up ? hi : lo;
+            }
+        }
+    } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)x[e], (int)j, 64);
+            const bool up = ((((uint32_t)e * 64u) | lane) & k) == 0;
+            const uint32_t lo = x[e] < o ? x[e] : o;
+            const uint32_t hi = x[e] < o ? o : x[e];
+            x[e] = (lower == up) ? lo : hi;
+        }
+    }
+    if constexpr (LJ > 0)
+        bitonic_layer<E, LK, LJ - 1>(x, lane);
+}
+
+// Merge stages k = 2^LK .. 64*E.
+template <int E, int LK>
+__device__ __forceinline__ void bitonic_stages(uint32_t (&x)[E], uint32_t lane)
+{
+    bitonic_layer<E, LK, LK - 1>(x, lane);
+    if constexpr ((1u << LK) < 64u * E)
+        bitonic_stages<E, LK + 1>(x, lane);
+}
+
+// One wave sorts one cell of up to 64*E references in registers: element
+// i = e*64 + lane sits in lane `lane`, register e; the bitonic network compares
+// i with i^j ascending when (i & k) == 0 -- across lanes by shuffles (j < 64),
+// within a lane between registers (j >= 64).  Slots past n hold 0xffffffff,
+// which sorts last.
+template <int E>
+__device__ __forceinline__ void wave_sort(uint32_t* __restrict__ seg, uint32_t n)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = (uint32_t)e * 64u + lane;
+        x[e] = i < n ? seg[i] : 0xffffffffu;
+    }
+    bitonic_stages<E, 1>(x, lane);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = (uint32_t)e * 64u + lane;
+        if (i < n)
+            seg[i] = x[e];
+    }
+}
+
+// Cells on lists.large of at most kSortWaveMax references, one wave each.
+constexpr uint32_t kSortWaveMax = 4096;
+
+__global__ __launch_bounds__(256) void k_sort_wave(const DevFrame* __restrict__ fr,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   uint32_t* __restrict__ refs, SortLists lists)
+{
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    uint32_t cnt = lists.counts[1];
+    cnt = cnt < lists.large_cap ? cnt : lists.large_cap;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t li = wave; li < cnt; li += n_waves) {
+        const uint32_t c = lists.large[li];
+        const uint32_t b = offsets[c];
+        const uint32_t n = offsets[c + 1] - b;
+        uint32_t* seg = refs + b;
+        if (n <= 64)
+            wave_sort<1>(seg, n);
+        else if (n <= 256)
+            wave_sort<4>(seg, n);
+        else if (n <= 1024)
+            wave_sort<16>(seg, n);
+        else if (n <= kSortWaveMax)
+            wave_sort<64>(seg, n);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__ fr,
                                                     const uint32_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ refs,
@@ -632,6 +764,8 @@ __global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__
         const uint32_t b = offsets[c];
         const uint32_t n = offsets[c + 1] - b;
         uint32_t* seg = refs + b;
+        if (n <= kSortWaveMax)
+            continue;   // sorted by k_sort_wave
         if (n <= kSortTile) {
             for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
                 lds[i] = seg[i];
@@ -825,15 +959,17 @@ hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_
 
 hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
                             uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
-                            hipStream_t s)
+                            const SortLists& lists, hipStream_t s)
 {
     const uint32_t nb = grid_for(cells_cap, kScanTile);
+    if (lists.small)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, lists.counts, 0u, (uint64_t)2);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
                        block_sums);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, 0u, fr, offsets,
                        refs_cap);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
-                       (const uint32_t*)block_sums, offsets);
+                       (const uint32_t*)block_sums, offsets, lists);
     return hipGetLastError();
 }
 
@@ -847,7 +983,7 @@ hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, n, (DevFrame*)nullptr,
                        out, 0xffffffffu);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n, (const DevFrame*)nullptr,
-                       (const uint32_t*)block_sums, out);
+                       (const uint32_t*)block_sums, out, SortLists{});
     return hipGetLastError();
 }
 
@@ -866,15 +1002,13 @@ hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t 
 }
 
 hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_t* refs,
-                            uint32_t* scratch, uint32_t* list, uint32_t list_cap,
-                            uint32_t* list_count, uint32_t cells_cap, hipStream_t s)
+                            const SortLists& lists, hipStream_t s)
 {
-    (void)scratch;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, list_count, 0u, (uint64_t)1);
-    hipLaunchKernelGGL(k_sort_small, dim3(grid_for(cells_cap, 256)), dim3(256), 0, s, fr, offsets,
-                       refs, list, list_cap, list_count);
+    hipLaunchKernelGGL(k_sort_small, dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
+    hipLaunchKernelGGL(k_sort_wave, dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
     hipLaunchKernelGGL(k_sort_large, dim3(1024), dim3(256), 0, s, fr, offsets, refs,
-                       (const uint32_t*)list, list_cap, (const uint32_t*)list_count);
+                       (const uint32_t*)lists.large, lists.large_cap,
+                       (const uint32_t*)(lists.counts + 1));
     return hipGetLastError();
 }
 

@@ -28,22 +28,32 @@ hipError_t launch_frame_from_box(const float bmin[3], const float bmax[3], uint3
 hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_tris,
                              uint32_t* counts, uint32_t* big_list, uint32_t* big_count,
                              hipStream_t s);
+// Cells the per-cell sort has to visit, collected by the scan: `small` holds
+// the cells of 2..kSortSmallCells references, `large` the larger ones;
+// counts[0], counts[1] their lengths (zeroed by launch_grid_scan).
+struct SortLists {
+    uint32_t* small;
+    uint32_t* large;
+    uint32_t* counts;
+    uint32_t small_cap;
+    uint32_t large_cap;
+};
+
 // Exclusive scan counts[0..m) -> offsets[0..m], total also to fr->r; flags
 // kStatusRefsOver when the total exceeds refs_cap.  block_sums holds
-// ceil(cells_cap/kScanTile) entries.
+// ceil(cells_cap/kScanTile) entries.  With lists.small set, also fills the
+// sort lists.
 hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
                             uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
-                            hipStream_t s);
+                            const SortLists& lists, hipStream_t s);
 // Scatter pass: refs[offsets[c] + k] = t; returns counts to zero.
 hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t n_tris,
                                uint32_t* counts, const uint32_t* offsets, uint32_t* refs,
                                const uint32_t* big_list, const uint32_t* big_count,
                                hipStream_t s);
-// Sort every cell's references ascending; `scratch` holds refs_cap entries,
-// `list` holds list_cap cell indices, `list_count` one counter.
+// Sort the references of every cell on the sort lists ascending.
 hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_t* refs,
-                            uint32_t* scratch, uint32_t* list, uint32_t list_cap,
-                            uint32_t* list_count, uint32_t cells_cap, hipStream_t s);
+                            const SortLists& lists, hipStream_t s);
 
 // trir[k] = tri[refs[k]] for k < fr->r (3 float4 per reference).
 hipError_t launch_gather_refs(const DevFrame* fr, const uint32_t* refs, const float4* tri,

@@ -48,10 +48,17 @@ def _summary(arrival, start, finish, rays, wall):
             "wall_s": round(wall, 3), "requests": int(len(lat))}
 
 
-def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120, load: float = 0.8,
-                   budget: int = 8 << 30, max_active: int = 64):
-    """Saturation throughput (all requests at t=0) and open-loop latency at
-    `load` x the plain path's capacity, for both paths."""
+def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120,
+                   loads=(0.5, 0.8, 0.95), n_open: int = 400, budget: int = 8 << 30,
+                   max_active: int = 64, slo_ms: float = 50.0):
+    """Serving (continuous batching) against the plain path on the mixed
+    request stream.
+
+    Saturation: n_requests all arriving at t = 0, for both paths -> each
+    path's capacity.  Open loop: n_open Poisson arrivals at each `load` x the
+    plain path's capacity (requests/s), the same stream for both paths ->
+    p50/p99 latency per load, the p99 ratio plain/serve, and the highest load
+    of the sweep at which each path keeps p99 within slo_ms."""
     torch = _torch()
     warm = _requests(scenes, W.serve_requests(6, 0.0, len(scenes), seed=W.SEED_SERVE + 7))
     serve_run(ctx, warm, True, budget, max_active)
@@ -60,18 +67,31 @@ def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120, load: float = 0.
     closed = _requests(scenes, W.serve_requests(n_requests, 0.0, len(scenes)))
     plain_sat = _summary(*serve_run(ctx, closed, False, budget, max_active))
     serve_sat = _summary(*serve_run(ctx, closed, True, budget, max_active))
-    rate = load * n_requests / plain_sat["wall_s"]
-    spec = W.serve_requests(n_requests, rate, len(scenes))
-    opened = _requests(scenes, spec)
-    plain = _summary(*serve_run(ctx, opened, False, budget, max_active))
-    served = _summary(*serve_run(ctx, opened, True, budget, max_active))
+    capacity = n_requests / plain_sat["wall_s"]
+    sweep = []
+    for load in loads:
+        rate = load * capacity
+        opened = _requests(scenes, W.serve_requests(n_open, rate, len(scenes)))
+        plain = _summary(*serve_run(ctx, opened, False, budget, max_active))
+        served = _summary(*serve_run(ctx, opened, True, budget, max_active))
+        sweep.append({"load_vs_plain_capacity": load, "arrival_rate_hz": round(rate, 2),
+                      "serve": served, "plain": plain,
+                      "p99_ratio": round(plain["p99_ms"] / max(served["p99_ms"], 1e-9), 2)})
+
+    def max_load(path):
+        ok = [p["load_vs_plain_capacity"] for p in sweep if p[path]["p99_ms"] <= slo_ms]
+        return max(ok) if ok else 0.0
+
+    top = sweep[-1]
     return {
-        "arrival_rate_hz": round(rate, 2),
-        "load_vs_plain_capacity": load,
-        "serve": served, "plain": plain,
+        "plain_capacity_req_s": round(capacity, 2),
         "saturation": {"serve": serve_sat, "plain": plain_sat},
-        "p99_headroom_x": round(plain["p99_ms"] / max(served["p99_ms"], 1e-9), 2),
         "saturation_mrays_x": round(serve_sat["mrays_s"] / max(plain_sat["mrays_s"], 1e-9), 2),
+        "sweep": sweep,
+        "p99_headroom_x": top["p99_ratio"],
+        "p99_headroom_load": top["load_vs_plain_capacity"],
+        "slo_p99_ms": slo_ms,
+        "max_load_within_slo": {"serve": max_load("serve"), "plain": max_load("plain")},
     }
 
 
