@@ -118,6 +118,8 @@ int grid_reserve_refs(grt_scene_t* sc, uint32_t refs)
         return rc;
     if ((rc = sc->sort_list_small.reserve(sc->ctx, (cap / 2 + 1) * 4, false)) != GRT_OK)
         return rc;
+    if ((rc = sc->sort_classes.reserve(sc->ctx, list_cap * 4 * kSortClasses, false)) != GRT_OK)
+        return rc;
     if ((rc = sc->trir.reserve(sc->ctx, cap * 48, false)) != GRT_OK)
         return rc;
     sc->refs_cap = (uint32_t)cap;
@@ -151,10 +153,11 @@ static void frame_to_info(const DevFrame& f, grt_grid_info* out)
 // The per-cell sort lists (empty before the reference buffers exist).
 static SortLists sort_lists(grt_scene_t* sc)
 {
-    if (!sc->sort_list_small.ptr || !sc->sort_list.ptr)
+    if (!sc->sort_list_small.ptr || !sc->sort_list.ptr || !sc->sort_classes.ptr)
         return SortLists{};
     return SortLists{sc->sort_list_small.as<uint32_t>(), sc->sort_list.as<uint32_t>(),
-                     sc->small_counters.as<uint32_t>() + 4, sc->refs_cap / 2 + 1, sc->list_cap};
+                     sc->small_counters.as<uint32_t>() + 4, sc->refs_cap / 2 + 1, sc->list_cap,
+                     sc->sort_classes.as<uint32_t>()};
 }
 
 static int enqueue_scan(grt_scene_t* sc, uint32_t refs_cap, bool with_lists)

@@ -19,7 +19,7 @@
 
 namespace grt {
 
-constexpr uint32_t kSmallCells = 16;
+constexpr uint32_t kSmallCells = 64;
 constexpr uint32_t kHugeCells = 65536;
 constexpr uint32_t kSortTile = 16384;   // LDS tile of the per-cell sort (64 KB)
 
@@ -254,6 +254,53 @@ __device__ __forceinline__ void cell_op(uint32_t c, uint32_t t, uint32_t* counts
     }
 }
 
+// Per-block LDS hash of the cells touched by the block's triangles (key: cell
+// index, or kHashEmpty).  Keys are written once and never removed, so a cell
+// that could not be inserted within kHashProbes probes is never found later
+// either, and such (triangle, cell) pairs fall back to global atomics.
+constexpr uint32_t kHashBits = 11;
+constexpr uint32_t kHashSlots = 1u << kHashBits;
+constexpr uint32_t kHashEmpty = 0xffffffffu;
+constexpr uint32_t kHashProbes = 16;
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t c)
+{
+    return (c * 2654435761u) >> (32 - kHashBits);
+}
+
+__device__ __forceinline__ uint32_t hash_insert(uint32_t* keys, uint32_t c)
+{
+    uint32_t h = hash_slot(c);
+    for (uint32_t p = 0; p < kHashProbes; ++p) {
+        const uint32_t prev = atomicCAS(&keys[h], kHashEmpty, c);
+        if (prev == kHashEmpty || prev == c)
+            return h;
+        h = (h + 1) & (kHashSlots - 1);
+    }
+    return kHashSlots;
+}
+
+__device__ __forceinline__ uint32_t hash_find(const uint32_t* keys, uint32_t c)
+{
+    uint32_t h = hash_slot(c);
+    for (uint32_t p = 0; p < kHashProbes; ++p) {
+        const uint32_t k = keys[h];
+        if (k == c)
+            return h;
+        if (k == kHashEmpty)
+            return kHashSlots;
+        h = (h + 1) & (kHashSlots - 1);
+    }
+    return kHashSlots;
+}
+
+// Count (SCATTER = false) or scatter pass for triangles of at most kSmallCells
+// cells, one thread per triangle.  Consecutive triangles are neighbours in
+// space, so a block's (triangle, cell) pairs are first combined per cell in
+// LDS and each distinct cell costs one global atomic per block: the count
+// adds the block's total; the scatter reserves a range of the cell's slots
+// (counting down, which returns counts[] to zero) and hands positions out of
+// it with LDS atomics.  The order within a cell is fixed by the sort pass.
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ fr,
                                                    const float* __restrict__ verts, uint32_t n_tris,
@@ -263,38 +310,84 @@ __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ 
                                                    uint32_t* __restrict__ big_list,
                                                    uint32_t* __restrict__ big_count)
 {
+    __shared__ uint32_t keys[kHashSlots];
+    __shared__ uint32_t slot_n[kHashSlots];   // count, then the next absolute position
     const uint32_t bad = SCATTER ? (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)
                                  : (kStatusFrameInvalid | kStatusCellsOver);
     if (fr->status & bad)
-        return;
+        return;   // uniform over the block
+    for (uint32_t i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+        keys[i] = kHashEmpty;
+        slot_n[i] = 0;
+    }
+    __syncthreads();
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_tris)
-        return;
-    const CellBox b = tri_cells(fr, verts + 9ull * t);
-    const uint32_t k = (uint32_t)b.w[0] * (uint32_t)b.w[1] * (uint32_t)b.w[2];
-    if (k > kSmallCells) {
-        // Medium triangles fill the list from the front, huge ones from the
-        // back (each triangle is on at most one list, so they never meet).
-        if (!SCATTER) {
-            if (k > kHugeCells)
-                big_list[n_tris - 1u - atomicAdd(big_count + 3, 1u)] = t;
-            else
-                big_list[atomicAdd(big_count, 1u)] = t;
+    CellBox b = {};
+    bool small = false;
+    if (t < n_tris) {
+        b = tri_cells(fr, verts + 9ull * t);
+        const uint32_t k = (uint32_t)b.w[0] * (uint32_t)b.w[1] * (uint32_t)b.w[2];
+        if (k > kSmallCells) {
+            // Medium triangles fill the list from the front, huge ones from
+            // the back (each triangle is on at most one list).
+            if (!SCATTER) {
+                if (k > kHugeCells)
+                    big_list[n_tris - 1u - atomicAdd(big_count + 3, 1u)] = t;
+                else
+                    big_list[atomicAdd(big_count, 1u)] = t;
+            }
+        } else {
+            small = true;
         }
-        return;
     }
     const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
-    for (int32_t z = 0; z < b.w[2]; ++z)
-        for (int32_t y = 0; y < b.w[1]; ++y) {
-            const uint32_t row = (uint32_t)b.lo[0] +
-                                 nx * ((uint32_t)(b.lo[1] + y) + ny * (uint32_t)(b.lo[2] + z));
-            for (int32_t x = 0; x < b.w[0]; ++x)
-                cell_op<SCATTER>(row + (uint32_t)x, t, counts, offsets, refs);
-        }
+    if (small) {
+        for (int32_t z = 0; z < b.w[2]; ++z)
+            for (int32_t y = 0; y < b.w[1]; ++y) {
+                const uint32_t row =
+                    (uint32_t)b.lo[0] + nx * ((uint32_t)(b.lo[1] + y) + ny * (uint32_t)(b.lo[2] + z));
+                for (int32_t x = 0; x < b.w[0]; ++x) {
+                    const uint32_t c = row + (uint32_t)x;
+                    const uint32_t h = hash_insert(keys, c);
+                    if (h < kHashSlots) {
+                        atomicAdd(&slot_n[h], 1u);
+                    } else if (SCATTER) {
+                        const uint32_t slot = atomicSub(&counts[c], 1u) - 1u;
+                        refs[offsets[c] + slot] = t;
+                    } else {
+                        atomicAdd(&counts[c], 1u);
+                    }
+                }
+            }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+        const uint32_t c = keys[i];
+        if (c == kHashEmpty)
+            continue;
+        const uint32_t n = slot_n[i];
+        if (SCATTER)
+            slot_n[i] = offsets[c] + atomicSub(&counts[c], n) - n;
+        else
+            atomicAdd(&counts[c], n);
+    }
+    if (!SCATTER)
+        return;
+    __syncthreads();
+    if (small) {
+        for (int32_t z = 0; z < b.w[2]; ++z)
+            for (int32_t y = 0; y < b.w[1]; ++y) {
+                const uint32_t row =
+                    (uint32_t)b.lo[0] + nx * ((uint32_t)(b.lo[1] + y) + ny * (uint32_t)(b.lo[2] + z));
+                for (int32_t x = 0; x < b.w[0]; ++x) {
+                    const uint32_t h = hash_find(keys, row + (uint32_t)x);
+                    if (h < kHashSlots)
+                        refs[atomicAdd(&slot_n[h], 1u)] = t;
+                }
+            }
+    }
 }
 
-// Huge triangles (more than kHugeCells cells, e.g. ground planes): the whole
-// grid strides over the cells of each one in turn.
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_grid_pass_huge(const DevFrame* __restrict__ fr,
                                                         const float* __restrict__ verts,
@@ -543,35 +636,67 @@ __device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
     b = hi;
 }
 
+// Bitonic layer (k = 2^LK, j = 2^LJ) over N registers of one thread, then the
+// next smaller j: element i is compared with i^j, ascending when (i & k) == 0.
+template <int N, int LK, int LJ>
+__device__ __forceinline__ void thread_bitonic_layer(uint32_t (&a)[N])
+{
+    constexpr int k = 1 << LK, j = 1 << LJ;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if ((i & j) == 0) {
+            if ((i & k) == 0)
+                cswap(a[i], a[i | j]);
+            else
+                cswap(a[i | j], a[i]);
+        }
+    }
+    if constexpr (LJ > 0)
+        thread_bitonic_layer<N, LK, LJ - 1>(a);
+}
+
+template <int N, int LK>
+__device__ __forceinline__ void thread_bitonic(uint32_t (&a)[N])
+{
+    thread_bitonic_layer<N, LK, LK - 1>(a);
+    if constexpr ((1 << LK) < N)
+        thread_bitonic<N, LK + 1>(a);
+}
+
+// A cell of 2..16 references sorted by one thread in registers (slots past
+// the count hold 0xffffffff, which sorts last).
+template <int N>
+__device__ __forceinline__ void sort_cell_thread(uint32_t* __restrict__ seg, uint32_t cnt)
+{
+    uint32_t a[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        a[k] = (uint32_t)k < cnt ? seg[k] : 0xffffffffu;
+    bool sorted = true;
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+        sorted = sorted && (a[k - 1] <= a[k]);
+    if (sorted)
+        return;
+    thread_bitonic<N, 1>(a);
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        if ((uint32_t)k < cnt)
+            seg[k] = a[k];
+}
+
 __device__ __forceinline__ void sort_small_cell(const uint32_t* __restrict__ offsets,
                                                 uint32_t* __restrict__ refs, uint32_t c)
 {
     const uint32_t b = offsets[c];
     const uint32_t cnt = offsets[c + 1] - b;
-    uint32_t a[8];
-#pragma unroll
-    for (uint32_t k = 0; k < 8; ++k)
-        a[k] = k < cnt ? refs[b + k] : 0xffffffffu;
-    bool sorted = true;
-#pragma unroll
-    for (int k = 1; k < 8; ++k)
-        sorted = sorted && (a[k - 1] <= a[k]);
-    if (sorted)
-        return;
-    cswap(a[0], a[1]); cswap(a[2], a[3]); cswap(a[4], a[5]); cswap(a[6], a[7]);
-    cswap(a[0], a[2]); cswap(a[1], a[3]); cswap(a[4], a[6]); cswap(a[5], a[7]);
-    cswap(a[1], a[2]); cswap(a[5], a[6]);
-    cswap(a[0], a[4]); cswap(a[1], a[5]); cswap(a[2], a[6]); cswap(a[3], a[7]);
-    cswap(a[2], a[4]); cswap(a[3], a[5]);
-    cswap(a[1], a[2]); cswap(a[3], a[4]); cswap(a[5], a[6]);
-#pragma unroll
-    for (uint32_t k = 0; k < 8; ++k)
-        if (k < cnt)
-            refs[b + k] = a[k];
+    if (cnt <= 8)
+        sort_cell_thread<8>(refs + b, cnt);
+    else
+        sort_cell_thread<16>(refs + b, cnt);
 }
 
-// Cells of 2..8 references (lists.small): Batcher odd-even merge network in
-// registers, one thread per cell.
+// Cells of 2..kSortSmallCells references (lists.small), one thread per cell.
 __global__ __launch_bounds__(256) void k_sort_small(const DevFrame* __restrict__ fr,
                                                     const uint32_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ refs, SortLists lists)
@@ -718,94 +843,125 @@ __device__ __forceinline__ void wave_sort(uint32_t* __restrict__ seg, uint32_t n
     }
 }
 
-// Cells on lists.large of at most kSortWaveMax references, one wave each.
 constexpr uint32_t kSortWaveMax = 4096;
 
+// Split lists.large by size class (see SortLists), one atomic per wave and
+// class.
+__global__ __launch_bounds__(256) void k_sort_classify(const DevFrame* __restrict__ fr,
+                                                       const uint32_t* __restrict__ offsets,
+                                                       SortLists lists)
+{
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    uint32_t cnt = lists.counts[1];
+    cnt = cnt < lists.large_cap ? cnt : lists.large_cap;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // the loop runs over whole waves (li0 is the wave's first entry), so every
+    // lane takes part in the ballots
+    for (uint32_t li0 = blockIdx.x * blockDim.x + threadIdx.x - lane; li0 < cnt; li0 += stride) {
+        const uint32_t li = li0 + lane;
+        int cls = -1;
+        uint32_t c = 0;
+        if (li < cnt) {
+            c = lists.large[li];
+            const uint32_t n = offsets[c + 1] - offsets[c];
+            cls = n <= 64u ? 0 : n <= 256u ? 1 : n <= 1024u ? 2 : n <= kSortWaveMax ? 3 : 4;
+        }
+#pragma unroll
+        for (int q = 0; q < kSortClasses; ++q) {
+            const uint64_t m = __ballot(cls == q);
+            if (m == 0)
+                continue;
+            const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+            uint32_t at = 0;
+            if (lane == leader)
+                at = atomicAdd(lists.counts + 2 + q, (uint32_t)__popcll(m));
+            at = (uint32_t)__builtin_amdgcn_readlane((int)at, (int)leader);
+            if (cls == q)
+                lists.cls[(size_t)q * lists.large_cap + at + lanes_below(m)] = c;
+        }
+    }
+}
+
+// Cells of size class Q (E = 4^Q: 16E < n <= 64E references), one wave each.
+template <int E, int Q>
 __global__ __launch_bounds__(256) void k_sort_wave(const DevFrame* __restrict__ fr,
                                                    const uint32_t* __restrict__ offsets,
                                                    uint32_t* __restrict__ refs, SortLists lists)
 {
     if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
         return;
-    uint32_t cnt = lists.counts[1];
-    cnt = cnt < lists.large_cap ? cnt : lists.large_cap;
+    const uint32_t cnt = lists.counts[2 + Q];
+    const uint32_t* __restrict__ list = lists.cls + (size_t)Q * lists.large_cap;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t li = wave; li < cnt; li += n_waves) {
-        const uint32_t c = lists.large[li];
+        const uint32_t c = list[li];
         const uint32_t b = offsets[c];
-        const uint32_t n = offsets[c + 1] - b;
-        uint32_t* seg = refs + b;
-        if (n <= 64)
-            wave_sort<1>(seg, n);
-        else if (n <= 256)
-            wave_sort<4>(seg, n);
-        else if (n <= 1024)
-            wave_sort<16>(seg, n);
-        else if (n <= kSortWaveMax)
-            wave_sort<64>(seg, n);
+        wave_sort<E>(refs + b, offsets[c + 1] - b);
     }
 }
 
-__global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__ fr,
-                                                    const uint32_t* __restrict__ offsets,
-                                                    uint32_t* __restrict__ refs,
-                                                    const uint32_t* __restrict__ list,
-                                                    uint32_t list_cap,
-                                                    const uint32_t* __restrict__ list_count)
+// Sort of one cell segment seg[0..n), n > kSortWaveMax, by the whole block:
+// LDS bitonic sort when it fits a tile; otherwise each tile is sorted in LDS
+// and the merge stages with k > kSortTile run their j >= kSortTile
+// comparators in global memory and the rest tile by tile in LDS.
+__device__ void block_sort_cell(uint32_t* __restrict__ seg, uint32_t n, uint32_t* lds)
 {
-    __shared__ uint32_t lds[kSortTile];
-    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+    if (n <= kSortTile) {
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+            lds[i] = seg[i];
+        __syncthreads();
+        lds_sort(lds, n);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+            seg[i] = lds[i];
+        __syncthreads();
         return;
-    uint32_t cnt = *list_count;
-    cnt = cnt < list_cap ? cnt : list_cap;
-    for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {
-        const uint32_t c = list[li];
-        const uint32_t b = offsets[c];
-        const uint32_t n = offsets[c + 1] - b;
-        uint32_t* seg = refs + b;
-        if (n <= kSortWaveMax)
-            continue;   // sorted by k_sort_wave
-        if (n <= kSortTile) {
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-                lds[i] = seg[i];
-            __syncthreads();
-            lds_sort(lds, n);
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-                seg[i] = lds[i];
-            __syncthreads();
-            continue;
-        }
-        // n > kSortTile: sort each tile in LDS, then merge stages with
-        // k > kSortTile run their j >= kSortTile comparators in global memory
-        // and the rest tile by tile in LDS.
-        uint32_t p = kSortTile;
-        while (p < n)
-            p <<= 1;
+    }
+    uint32_t p = kSortTile;
+    while (p < n)
+        p <<= 1;
+    for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
+        const uint32_t len = (n - t0) < kSortTile ? (n - t0) : kSortTile;
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
+            lds[i] = seg[t0 + i];
+        __syncthreads();
+        lds_sort(lds, len);
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
+            seg[t0 + i] = lds[i];
+        __syncthreads();
+    }
+    for (uint32_t k = 2 * kSortTile; k <= p; k <<= 1) {
+        for (uint32_t j = k >> 1; j >= kSortTile; j >>= 1)
+            global_pass(seg, n, p, k, j);
         for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
             const uint32_t len = (n - t0) < kSortTile ? (n - t0) : kSortTile;
             for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
                 lds[i] = seg[t0 + i];
             __syncthreads();
-            lds_sort(lds, len);
+            lds_half_cleaners(lds, len, kSortTile >> 1);
             for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
                 seg[t0 + i] = lds[i];
             __syncthreads();
         }
-        for (uint32_t k = 2 * kSortTile; k <= p; k <<= 1) {
-            for (uint32_t j = k >> 1; j >= kSortTile; j >>= 1)
-                global_pass(seg, n, p, k, j);
-            for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
-                const uint32_t len = (n - t0) < kSortTile ? (n - t0) : kSortTile;
-                for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
-                    lds[i] = seg[t0 + i];
-                __syncthreads();
-                lds_half_cleaners(lds, len, kSortTile >> 1);
-                for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
-                    seg[t0 + i] = lds[i];
-                __syncthreads();
-            }
-        }
+    }
+}
+
+// Cells of more than kSortWaveMax references (size class 4), one block each.
+__global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__ fr,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ refs, SortLists lists)
+{
+    __shared__ uint32_t lds[kSortTile];
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    const uint32_t cnt = lists.counts[2 + 4];
+    const uint32_t* __restrict__ list = lists.cls + (size_t)4 * lists.large_cap;
+    for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const uint32_t c = list[li];
+        const uint32_t b = offsets[c];
+        block_sort_cell(refs + b, offsets[c + 1] - b, lds);
     }
 }
 
@@ -963,7 +1119,8 @@ hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offs
 {
     const uint32_t nb = grid_for(cells_cap, kScanTile);
     if (lists.small)
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, lists.counts, 0u, (uint64_t)2);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, lists.counts, 0u,
+                           (uint64_t)(2 + kSortClasses));
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
                        block_sums);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, 0u, fr, offsets,
@@ -1005,10 +1162,12 @@ hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_
                             const SortLists& lists, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sort_small, dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
-    hipLaunchKernelGGL(k_sort_wave, dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
-    hipLaunchKernelGGL(k_sort_large, dim3(1024), dim3(256), 0, s, fr, offsets, refs,
-                       (const uint32_t*)lists.large, lists.large_cap,
-                       (const uint32_t*)(lists.counts + 1));
+    hipLaunchKernelGGL(k_sort_classify, dim3(512), dim3(256), 0, s, fr, offsets, lists);
+    hipLaunchKernelGGL((k_sort_wave<1, 0>), dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
+    hipLaunchKernelGGL((k_sort_wave<4, 1>), dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
+    hipLaunchKernelGGL((k_sort_wave<16, 2>), dim3(1024), dim3(256), 0, s, fr, offsets, refs, lists);
+    hipLaunchKernelGGL((k_sort_wave<64, 3>), dim3(512), dim3(256), 0, s, fr, offsets, refs, lists);
+    hipLaunchKernelGGL(k_sort_large, dim3(512), dim3(256), 0, s, fr, offsets, refs, lists);
     return hipGetLastError();
 }
 

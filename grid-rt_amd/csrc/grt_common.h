@@ -101,6 +101,13 @@ __device__ __forceinline__ int32_t vox(float val, float bmin, float inv, int32_t
     return (int32_t)x;
 }
 
+// Number of set bits of a wave mask below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace grt

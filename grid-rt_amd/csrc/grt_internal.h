@@ -67,7 +67,7 @@ struct grt_scene_t {
     size_t xf_host_bytes = 0;
     // Grid.
     grt::DevMem frame, partial, counts, offsets, refs, block_sums, big_list, sort_list,
-        sort_list_small, small_counters, trir;
+        sort_list_small, sort_classes, small_counters, trir;
     uint32_t cells_cap = 0, refs_cap = 0, list_cap = 0;
     bool grid_valid = false;
     bool info_stale = false;
@@ -78,7 +78,8 @@ struct grt_scene_t {
     {
         return {&verts,  &normals,    &rest_verts, &rest_normals, &tri,       &xf,
                 &frame,  &partial,    &counts,     &offsets,      &refs,      &block_sums,
-                &big_list, &sort_list, &sort_list_small, &small_counters, &trir};
+                &big_list, &sort_list, &sort_list_small, &sort_classes, &small_counters,
+                &trir};
     }
     ~grt_scene_t()
     {

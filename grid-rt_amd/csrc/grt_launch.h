@@ -10,7 +10,7 @@ namespace grt {
 // ---- grid build (grt_build.hip) --------------------------------------------
 constexpr uint32_t kScanTile = 4096;    // elements per scan block (256 threads x 16)
 constexpr uint32_t kBBoxMaxBlocks = 1024;
-constexpr uint32_t kSortSmallCells = 8;  // cells up to this size sort in registers
+constexpr uint32_t kSortSmallCells = 16;  // cells up to this size sort in one thread
 
 // Frame from the vertices: block partial boxes, then one block computes the
 // frame.  `partial` holds 6*kBBoxMaxBlocks floats.
@@ -31,12 +31,17 @@ hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_
 // Cells the per-cell sort has to visit, collected by the scan: `small` holds
 // the cells of 2..kSortSmallCells references, `large` the larger ones;
 // counts[0], counts[1] their lengths (zeroed by launch_grid_scan).
+// `cls` (large_cap entries per class) receives the large list split by size
+// for the sort kernels: (16,64], (64,256], (256,1024], (1024,4096], larger;
+// counts[2..6] their lengths.
+constexpr int kSortClasses = 5;
 struct SortLists {
     uint32_t* small;
     uint32_t* large;
     uint32_t* counts;
     uint32_t small_cap;
     uint32_t large_cap;
+    uint32_t* cls;
 };
 
 // Exclusive scan counts[0..m) -> offsets[0..m], total also to fr->r; flags

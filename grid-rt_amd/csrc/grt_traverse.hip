@@ -58,12 +58,6 @@ static uint64_t slot_count(uint32_t n, uint32_t img_w)
     return (uint64_t)((img_w + 7u) >> 3) * ((h + 7u) >> 3) * 64u;
 }
 
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
 // ---------------------------------------------------------------------------
 // DDA
 // ---------------------------------------------------------------------------

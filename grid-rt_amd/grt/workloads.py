@@ -89,5 +89,12 @@ def ao_scene(seed: int = SEED_AO, clusters: int = 4096, per_cluster: int = 16):
     return gen_sphere_clusters(seed, clusters, per_cluster, 0.02, 0.003, 0.012)
 
 
+def ao_camera(width: int, height: int):
+    """From the middle of the scene box's front face (z = 0) looking in: every
+    primary ray crosses the whole box, so every pixel has a hit to start AO rays
+    from."""
+    return camera_lookat((0.5, 0.5, 0.0), (0.5, 0.5, 1.0), UP, FOVY, width, height)
+
+
 def percentile_ms(lat_s, q):
     return float(np.percentile(np.asarray(lat_s) * 1e3, q)) if len(lat_s) else float("nan")
