@@ -91,7 +91,8 @@ def kernel_name(closest, sched_char):
     """The traversal kernel a stage launches under a schedule character."""
     if sched_char == "p":
         return "k_trace_persistent<%s, false>" % ("false" if closest else "true")
-    return "k_trace_closest<false>" if closest else "k_trace_any<false>"
+    pooled = "true" if sched_char == "w" else "false"
+    return ("k_trace_closest<false, %s>" if closest else "k_trace_any<false, %s>") % pooled
 
 
 class Frame:
@@ -186,9 +187,12 @@ class Frame:
         sc.trace_any(self.shadow2, self.occ2, image_width=W)
         mark(10)
 
+    SCHEDULES = {"s": "simple", "w": "pooled", "p": "persistent"}
+
     def mode(self, stage):
-        """Traversal schedule of a stage: 's' one ray per lane, 'p' persistent."""
-        self.ctx.set_trace_mode(self.sched[stage] == "p", self.refill)
+        """Traversal schedule of a stage: 's' one ray per lane, 'w' one ray per
+        lane with the wave's tests pooled, 'p' persistent."""
+        self.ctx.set_trace_mode(self.SCHEDULES[self.sched[stage]], self.refill)
 
     def active(self, rb):
         f = rb.f
@@ -197,8 +201,8 @@ class Frame:
 
 def autotune(frame, lams, sched):
     """Pick lambda and the per-stage traversal schedule on the frame itself
-    (outside the timed region): for each lambda, time every stage under both
-    schedules and keep the faster per stage."""
+    (outside the timed region): for each lambda, time every stage under each
+    schedule and keep the fastest per stage."""
     from grt.autotune import Autotuner, per_stage_schedule
     tuner = Autotuner()
     table = {}
@@ -210,13 +214,13 @@ def autotune(frame, lams, sched):
             ms = frame.time_stages()
             table[c["lam"]] = {"sched": sched, "frame_ms": round(sum(ms), 3)}
             return sum(ms)
-        frame.sched = "ssss"
-        ms_s = frame.time_stages()
-        frame.sched = "pppp"
-        ms_p = frame.time_stages()
-        best = per_stage_schedule(ms_s[2:], ms_p[2:])
-        total = sum(ms_s[:2]) / 2 + sum(ms_p[:2]) / 2 + sum(min(a, b) for a, b in
-                                                             zip(ms_s[2:], ms_p[2:]))
+        by = {}
+        for ch in Frame.SCHEDULES:
+            frame.sched = ch * 4
+            by[ch] = frame.time_stages()
+        best = per_stage_schedule({ch: ms[2:] for ch, ms in by.items()})
+        total = sum(sum(ms[:2]) for ms in by.values()) / len(by) + \
+            sum(by[ch][2 + i] for i, ch in enumerate(best))
         table[c["lam"]] = {"sched": best, "frame_ms": round(total, 3)}
         return total
 
@@ -449,8 +453,9 @@ def main():
         # p50/p99 request latency, continuous batching vs the plain path, on
         # the serving workload (16 resident sphere scenes, mixed requests).
         from grt import perf
-        scenes, tris = perf.build_serve_scenes(frame.ctx)
-        lat = perf.serve_vs_plain(frame.ctx, scenes, n_requests=args.serve_requests)
+        sctx = grt.Context(0)   # the serving run's own schedule and buffers
+        scenes, tris = perf.build_serve_scenes(sctx)
+        lat = perf.serve_vs_plain(sctx, scenes, n_requests=args.serve_requests)
         lat["triangles"] = tris
         out["latency"] = lat
     if not args.no_cpu_baseline:

@@ -304,7 +304,7 @@ int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill)
 {
     if (!ctx)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: NULL context");
-    if (mode != GRT_TRACE_SIMPLE && mode != GRT_TRACE_PERSISTENT)
+    if (mode != GRT_TRACE_SIMPLE && mode != GRT_TRACE_PERSISTENT && mode != GRT_TRACE_POOLED)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: unknown mode %d", mode);
     if (refill < 1 || refill > 64)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: refill must be 1..64");
@@ -779,6 +779,7 @@ static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint
     a.image_width = image_width;
     a.counters = counters;
     a.coop = sc->ctx->coop;
+    a.pool = sc->ctx->trace_mode == GRT_TRACE_POOLED ? 1u : 0u;
     return a;
 }
 

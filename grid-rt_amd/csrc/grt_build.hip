@@ -27,20 +27,6 @@ constexpr uint32_t kSortTile = 16384;   // LDS tile of the per-cell sort (64 KB)
 // Wave64 / block primitives
 // ---------------------------------------------------------------------------
 
-// Inclusive prefix sum over a full wave64 with DPP: row_shr 1,2,4,8 inside each
-// row of 16 lanes, then row_bcast15 (rows 1,3) and row_bcast31 (rows 2,3).
-// All 64 lanes must be active.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
-{
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
-    return v;
-}
-
 // Exclusive block scan (BLOCK threads, BLOCK/64 waves); lds needs BLOCK/64+1
 // words.  Returns the exclusive prefix; `total` receives the block sum.
 template <int BLOCK>

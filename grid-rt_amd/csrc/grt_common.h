@@ -101,6 +101,20 @@ __device__ __forceinline__ int32_t vox(float val, float bmin, float inv, int32_t
     return (int32_t)x;
 }
 
+// Inclusive prefix sum over a full wave64 with DPP: row_shr 1,2,4,8 inside each
+// row of 16 lanes, then row_bcast15 (rows 1,3) and row_bcast31 (rows 2,3).
+// All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+
 // Number of set bits of a wave mask below the calling lane.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 {

@@ -3,6 +3,7 @@
 // synchronises.  Buffer sizes are checked by the callers in grt_api.cpp.
 #pragma once
 
+#include "grt/grt.h"
 #include "grt_common.h"
 
 namespace grt {
@@ -88,9 +89,10 @@ struct TraceArgs {
     uint32_t n;
     uint32_t image_width;   // 0: lane i traces ray i; else 8x8 pixel tiles per wave
     uint32_t* counters;     // NULL or 2 per ray {cells, tests}
-    uint32_t coop;          // persistent kernels: wave-cooperative cell tests once the
-                            // ray queue is drained, for cells with at least this many
+    uint32_t coop;          // wave-cooperative tests of dense cells: at least this many
                             // references left (0: off)
+    uint32_t pool;          // one-ray-per-lane kernels: pool the wave's tests (1) or
+                            // test per lane (0)
 };
 hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
                                 uint32_t block, hipStream_t s);
@@ -122,11 +124,11 @@ struct alignas(16) BatchTask {
 // Map each 64-slot block to its task (n_blocks entries).
 hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* map,
                            hipStream_t s);
-// Multi-scene traversal over all tasks (mixed closest / any hit): one ray per
-// lane (each wave in one task), or persistent waves refilled from the slot
-// counter.
+// Multi-scene traversal over all tasks (mixed closest / any hit) with the
+// schedule `mode` (GRT_TRACE_*): one ray per lane (each wave in one task), with
+// per-lane or pooled tests, or persistent waves refilled from the slot counter.
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
-                              uint32_t* slot_counter, bool persistent, uint32_t block,
+                              uint32_t* slot_counter, int mode, uint32_t block,
                               uint32_t refill, uint32_t coop,
                               hipStream_t s);
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s);

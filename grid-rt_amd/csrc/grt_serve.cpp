@@ -314,6 +314,7 @@ int grt_serve_run(grt_context ctx, const grt_request* reqs, uint32_t n_reqs,
                 a.image_width = it.width;
                 a.counters = nullptr;
                 a.coop = ctx->coop;
+                a.pool = ctx->trace_mode == GRT_TRACE_POOLED ? 1u : 0u;
                 hipError_t e;
                 if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
                     e = launch_trace_persistent(a, it.rays.in(), it.hits.out(), it.occ, it.any,

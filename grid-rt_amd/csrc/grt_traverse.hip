@@ -405,6 +405,138 @@ __device__ __forceinline__ uint64_t coop_lanes(bool busy, bool drained_lane, uin
 }
 
 // ---------------------------------------------------------------------------
+// Pooled tests (one ray per lane)
+// ---------------------------------------------------------------------------
+//
+// With rays in different cells, a per-lane loop over each lane's references
+// leaves most lanes idle (the wave runs as long as its fullest cell).  Pooled,
+// the (lane, reference) pairs of all the wave's current cells are numbered
+// through an exclusive scan of the lanes' counts and tested 64 at a time, one
+// pair per lane, the owner's ray read from LDS.  Closest hit keeps, per owner,
+// the minimum of (t, reference) in an LDS 64-bit atomic -- t mapped to an
+// order-preserving integer with -0 folded into +0 -- which is the reference
+// the serial loop would keep (it replaces only on a strictly smaller t); the
+// winner's u, v are then recomputed by the owner with the same arithmetic.
+// Any hit keeps the first occluding offset; the test counter counts what the
+// serial loop would have tested.
+
+// LDS staging of one wave.
+struct WavePool {
+    float o0[64], o1[64], o2[64], d0[64], d1[64], d2[64], tmin[64], tmax[64], bound[64];
+    uint32_t pre[64];                // exclusive prefix of the lanes' reference counts
+    uint32_t kb[64];                 // first reference of each lane's cell
+    unsigned long long win[64];      // per owner: (t, reference) minimum / first occluder
+};
+
+// Lanes of one wave exchange data through LDS: order the accesses around this
+// point (a wave's LDS operations execute in program order).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t ordered_bits(float t)
+{
+    const uint32_t b = __float_as_uint(t == 0.0f ? 0.0f : t);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ void pool_store_ray(WavePool& P, const LaneRay& r)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    P.o0[lane] = r.o0;
+    P.o1[lane] = r.o1;
+    P.o2[lane] = r.o2;
+    P.d0[lane] = r.d0;
+    P.d1[lane] = r.d1;
+    P.d2[lane] = r.d2;
+    P.tmin[lane] = r.tmin;
+    P.tmax[lane] = r.tmax;
+}
+
+// Owner of pair p: the last lane whose prefix is <= p.
+__device__ __forceinline__ uint32_t pool_owner(const WavePool& P, uint32_t p)
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (uint32_t step = 32; step > 0; step >>= 1)
+        if (P.pre[o + step] <= p)
+            o += step;
+    return o;
+}
+
+// The whole wave tests every lane's references [k, k + c) (c = 0 for lanes
+// without work); each lane's hit and test count are updated as by the serial
+// loop and its cell is consumed.
+template <bool ANY>
+__device__ __forceinline__ void pool_cells(WavePool& P, const float4* __restrict__ trir,
+                                           LaneRay& r, uint32_t c)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t total = bcast_u(incl, 63);
+    P.pre[lane] = incl - c;
+    P.kb[lane] = r.k;
+    P.bound[lane] = r.bound;
+    P.win[lane] = ~0ull;
+    wave_lds_sync();
+    for (uint32_t base = 0; base < total; base += 64u) {
+        const uint32_t p = base + lane;
+        if (p < total) {
+            const uint32_t o = pool_owner(P, p);
+            const uint32_t j = p - P.pre[o];
+            const uint32_t ref = P.kb[o] + j;
+            if (ANY && P.win[o] < j)
+                continue;   // the owner has an earlier occluder already
+            const float4 A = trir[3ull * ref], B = trir[3ull * ref + 1], C = trir[3ull * ref + 2];
+            const float o0 = P.o0[o], o1 = P.o1[o], o2 = P.o2[o];
+            const float d0 = P.d0[o], d1 = P.d1[o], d2 = P.d2[o];
+            const float tmin = P.tmin[o], tmax = P.tmax[o];
+            if (ANY) {
+                if (any_test(A, B, C, o0, o1, o2, d0, d1, d2, tmin, tmax))
+                    atomicMin(&P.win[o], (unsigned long long)j);
+            } else {
+                const float bound = P.bound[o];
+                const MtPre m = mt_pre(A, B, C, o0, o1, o2, d0, d1, d2);
+                if (mt_maybe(m, tmin, tmin_f(tmax, bound))) {
+                    float t, u, v;
+                    if (mt_exact(m, t, u, v) && t > tmin && t <= tmax && t <= bound)
+                        atomicMin(&P.win[o], ((unsigned long long)ordered_bits(t) << 32) | ref);
+                }
+            }
+        }
+    }
+    wave_lds_sync();
+    if (c == 0)
+        return;
+    const unsigned long long w = P.win[lane];
+    if (ANY) {
+        if (w != ~0ull) {
+            r.id = 1;
+            r.tests += (uint32_t)w + 1u;
+        } else {
+            r.tests += c;
+        }
+    } else {
+        if (w != ~0ull) {
+            const uint32_t ref = (uint32_t)w;
+            const float4 A = trir[3ull * ref], B = trir[3ull * ref + 1], C = trir[3ull * ref + 2];
+            const MtPre m = mt_pre(A, B, C, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2);
+            float t, u, v;
+            mt_exact(m, t, u, v);
+            r.best = t;
+            r.bu = u;
+            r.bv = v;
+            r.id = (int32_t)__float_as_uint(C.y);
+        }
+        r.tests += c;
+    }
+    r.k = r.e;
+}
+
+// ---------------------------------------------------------------------------
 // One ray per lane
 // ---------------------------------------------------------------------------
 //
@@ -448,13 +580,13 @@ __device__ __forceinline__ void load_cell(const Dda& s, const uint32_t* __restri
 
 // Closest hit of ray i (valid lanes) against one grid; every lane of the wave
 // calls it together.
-template <bool COUNT>
+template <bool COUNT, bool POOL>
 __device__ __forceinline__ void trace_closest_lane(const DevFrame* __restrict__ fr,
                                                    const uint32_t* __restrict__ offsets,
                                                    const float4* __restrict__ trir,
                                                    const RaySoA& rays, const HitSoA& hits,
                                                    uint32_t* __restrict__ counters, uint32_t coop,
-                                                   uint32_t i, bool valid)
+                                                   uint32_t i, bool valid, WavePool* pool)
 {
     const KF4 ktr = (KF4)trir;
     LaneRay r = {};
@@ -475,16 +607,31 @@ __device__ __forceinline__ void trace_closest_lane(const DevFrame* __restrict__ 
         r.tmax = rays.tmax[i];
         busy = grid_ok(fr) && dda_setup(fr, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, s);
     }
+    if (POOL)
+        pool_store_ray(*pool, r);
     float exit_t = 0.0f;
     if (busy) {
         load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
         cells = 1;
     }
     while (__ballot(busy) != 0) {
-        if (const uint64_t hv0 = coop_lanes_simple(busy, r.k, r.e, coop))
+        bool pooled = false;
+        if (POOL) {
+            const uint32_t c = (busy && r.k < r.e) ? r.e - r.k : 0u;
+            const uint64_t cm = __ballot(c != 0);
+            if (cm != 0) {
+                const uint32_t f = (uint32_t)__builtin_ctzll(cm);
+                const uint32_t k0 = bcast_u(r.k, f), e0 = bcast_u(r.e, f);
+                if (__ballot(c != 0 && (r.k != k0 || r.e != e0)) != 0) {
+                    pool_cells<false>(*pool, trir, r, c);
+                    pooled = true;
+                }
+            }
+        } else if (const uint64_t hv0 = coop_lanes_simple(busy, r.k, r.e, coop)) {
             for (uint64_t hv = hv0; hv != 0; hv &= hv - 1)
                 coop_cell((uint32_t)__builtin_ctzll(hv), false, trir, r);
-        if (busy && r.k < r.e) {
+        }
+        if (!pooled && busy && r.k < r.e) {
             const uint32_t b = r.k, e = r.e;
             const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
             const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
@@ -543,13 +690,13 @@ __device__ __forceinline__ void trace_closest_lane(const DevFrame* __restrict__ 
 
 // Any hit of ray i (valid lanes) against one grid; every lane of the wave calls
 // it together.
-template <bool COUNT>
+template <bool COUNT, bool POOL>
 __device__ __forceinline__ void trace_any_lane(const DevFrame* __restrict__ fr,
                                                const uint32_t* __restrict__ offsets,
                                                const float4* __restrict__ trir,
                                                const RaySoA& rays, uint8_t* __restrict__ occluded,
                                                uint32_t* __restrict__ counters, uint32_t coop,
-                                               uint32_t i, bool valid)
+                                               uint32_t i, bool valid, WavePool* pool)
 {
     const KF4 ktr = (KF4)trir;
     LaneRay r = {};
@@ -569,16 +716,31 @@ __device__ __forceinline__ void trace_any_lane(const DevFrame* __restrict__ fr,
         r.tmax = rays.tmax[i];
         busy = grid_ok(fr) && dda_setup(fr, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, s);
     }
+    if (POOL)
+        pool_store_ray(*pool, r);
     float exit_t = 0.0f;
     if (busy) {
         load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
         cells = 1;
     }
     while (__ballot(busy) != 0) {
-        if (const uint64_t hv0 = coop_lanes_simple(busy, r.k, r.e, coop))
+        bool pooled = false;
+        if (POOL) {
+            const uint32_t c = (busy && r.k < r.e) ? r.e - r.k : 0u;
+            const uint64_t cm = __ballot(c != 0);
+            if (cm != 0) {
+                const uint32_t f = (uint32_t)__builtin_ctzll(cm);
+                const uint32_t k0 = bcast_u(r.k, f), e0 = bcast_u(r.e, f);
+                if (__ballot(c != 0 && (r.k != k0 || r.e != e0)) != 0) {
+                    pool_cells<true>(*pool, trir, r, c);
+                    pooled = true;
+                }
+            }
+        } else if (const uint64_t hv0 = coop_lanes_simple(busy, r.k, r.e, coop)) {
             for (uint64_t hv = hv0; hv != 0; hv &= hv - 1)
                 coop_cell((uint32_t)__builtin_ctzll(hv), true, trir, r);
-        if (busy && r.k < r.e) {
+        }
+        if (!pooled && busy && r.k < r.e) {
             const uint32_t b = r.k, e = r.e;
             const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
             const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
@@ -639,30 +801,37 @@ __device__ __forceinline__ void trace_any_lane(const DevFrame* __restrict__ fr,
     }
 }
 
-template <bool COUNT>
+template <bool COUNT, bool POOL>
 __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a, RaySoA rays, HitSoA hits)
 {
+    __shared__ WavePool pools[POOL ? 4 : 1];
     bool valid;
     const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
-    trace_closest_lane<COUNT>(a.fr, a.offsets, a.trir, rays, hits, a.counters, a.coop, i, valid);
+    trace_closest_lane<COUNT, POOL>(a.fr, a.offsets, a.trir, rays, hits, a.counters, a.coop, i,
+                                    valid, POOL ? &pools[threadIdx.x >> 6] : nullptr);
 }
 
-template <bool COUNT>
+template <bool COUNT, bool POOL>
 __global__ __launch_bounds__(256) void k_trace_any(TraceArgs a, RaySoA rays,
                                                    uint8_t* __restrict__ occluded)
 {
+    __shared__ WavePool pools[POOL ? 4 : 1];
     bool valid;
     const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
-    trace_any_lane<COUNT>(a.fr, a.offsets, a.trir, rays, occluded, a.counters, a.coop, i, valid);
+    trace_any_lane<COUNT, POOL>(a.fr, a.offsets, a.trir, rays, occluded, a.counters, a.coop, i,
+                                valid, POOL ? &pools[threadIdx.x >> 6] : nullptr);
 }
 
 // Batched multi-scene traversal, one ray per lane: every 64-slot group belongs
 // to one task (map[slot / 64]), so each wave runs the single-scene loop of its
 // task with wave-uniform scene data.
+template <bool POOL>
 __global__ __launch_bounds__(256) void k_trace_batch_simple(const BatchTask* __restrict__ tasks,
                                                             const uint32_t* __restrict__ map,
                                                             uint32_t total_slots, uint32_t coop)
 {
+    __shared__ WavePool pools[POOL ? 4 : 1];
+    WavePool* pool = POOL ? &pools[threadIdx.x >> 6] : nullptr;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t group = __builtin_amdgcn_readfirstlane(slot >> 6);
     if ((group << 6) >= total_slots)
@@ -672,11 +841,11 @@ __global__ __launch_bounds__(256) void k_trace_batch_simple(const BatchTask* __r
     bool valid;
     const uint32_t i = slot_ray(slot - T.slot_begin, T.image_width, T.n, valid);
     if (T.any)
-        trace_any_lane<false>(T.fr, T.offsets, T.trir, T.rays, T.occluded, nullptr, coop, i,
-                              valid);
+        trace_any_lane<false, POOL>(T.fr, T.offsets, T.trir, T.rays, T.occluded, nullptr, coop, i,
+                                    valid, pool);
     else
-        trace_closest_lane<false>(T.fr, T.offsets, T.trir, T.rays, T.hits, nullptr, coop, i,
-                                  valid);
+        trace_closest_lane<false, POOL>(T.fr, T.offsets, T.trir, T.rays, T.hits, nullptr, coop, i,
+                                        valid, pool);
 }
 
 // ---------------------------------------------------------------------------
@@ -1045,10 +1214,17 @@ hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const Hi
     if (a.n == 0)
         return hipSuccess;
     const uint32_t g = ray_grid(a.n, a.image_width, block);
-    if (a.counters)
-        hipLaunchKernelGGL(k_trace_closest<true>, dim3(g), dim3(block), 0, s, a, rays, hits);
+    if (a.counters && a.pool)
+        hipLaunchKernelGGL((k_trace_closest<true, true>), dim3(g), dim3(block), 0, s, a, rays, hits);
+    else if (a.counters)
+        hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(g), dim3(block), 0, s, a, rays,
+                           hits);
+    else if (a.pool)
+        hipLaunchKernelGGL((k_trace_closest<false, true>), dim3(g), dim3(block), 0, s, a, rays,
+                           hits);
     else
-        hipLaunchKernelGGL(k_trace_closest<false>, dim3(g), dim3(block), 0, s, a, rays, hits);
+        hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(g), dim3(block), 0, s, a, rays,
+                           hits);
     return hipGetLastError();
 }
 
@@ -1058,10 +1234,17 @@ hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occ
     if (a.n == 0)
         return hipSuccess;
     const uint32_t g = ray_grid(a.n, a.image_width, block);
-    if (a.counters)
-        hipLaunchKernelGGL(k_trace_any<true>, dim3(g), dim3(block), 0, s, a, rays, occluded);
+    if (a.counters && a.pool)
+        hipLaunchKernelGGL((k_trace_any<true, true>), dim3(g), dim3(block), 0, s, a, rays, occluded);
+    else if (a.counters)
+        hipLaunchKernelGGL((k_trace_any<true, false>), dim3(g), dim3(block), 0, s, a, rays,
+                           occluded);
+    else if (a.pool)
+        hipLaunchKernelGGL((k_trace_any<false, true>), dim3(g), dim3(block), 0, s, a, rays,
+                           occluded);
     else
-        hipLaunchKernelGGL(k_trace_any<false>, dim3(g), dim3(block), 0, s, a, rays, occluded);
+        hipLaunchKernelGGL((k_trace_any<false, false>), dim3(g), dim3(block), 0, s, a, rays,
+                           occluded);
     return hipGetLastError();
 }
 
@@ -1129,14 +1312,19 @@ hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* m
 }
 
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
-                              uint32_t* slot_counter, bool persistent, uint32_t block,
+                              uint32_t* slot_counter, int mode, uint32_t block,
                               uint32_t refill, uint32_t coop, hipStream_t s)
 {
     if (total_slots == 0)
         return hipSuccess;
-    if (!persistent) {
-        hipLaunchKernelGGL(k_trace_batch_simple, dim3((total_slots + block - 1) / block),
-                           dim3(block), 0, s, tasks, map, total_slots, coop);
+    if (mode != GRT_TRACE_PERSISTENT) {
+        const dim3 g((total_slots + block - 1) / block);
+        if (mode == GRT_TRACE_POOLED)
+            hipLaunchKernelGGL(k_trace_batch_simple<true>, g, dim3(block), 0, s, tasks, map,
+                               total_slots, coop);
+        else
+            hipLaunchKernelGGL(k_trace_batch_simple<false>, g, dim3(block), 0, s, tasks, map,
+                               total_slots, coop);
         return hipGetLastError();
     }
     refill = refill < 1 ? 1 : (refill > 64 ? 64 : refill);

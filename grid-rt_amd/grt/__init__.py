@@ -324,8 +324,15 @@ class Context:
     def set_block_size(self, closest: int, any_hit: int):
         check(lib().grt_set_block_size(self.h, closest, any_hit))
 
-    def set_trace_mode(self, persistent: bool = True, refill: int = 16):
-        check(lib().grt_set_trace_mode(self.h, 1 if persistent else 0, refill))
+    TRACE_MODES = {"simple": 0, "persistent": 1, "pooled": 2}
+
+    def set_trace_mode(self, mode="simple", refill: int = 16):
+        """Traversal schedule: "simple" (one ray per lane), "pooled" (one ray
+        per lane, the wave's triangle tests pooled), "persistent" (refilled
+        persistent waves); a bool selects persistent (True) or simple."""
+        if isinstance(mode, bool):
+            mode = "persistent" if mode else "simple"
+        check(lib().grt_set_trace_mode(self.h, self.TRACE_MODES[mode], refill))
 
     def set_coop(self, min_refs: int = 16):
         """Cooperative cell tests in drained persistent waves (0: off)."""

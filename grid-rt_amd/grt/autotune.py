@@ -5,7 +5,8 @@ that depend on the grid):
   lam     grid resolution factor (cells ~ lam * triangles; the reference fixed
           k = 6 in UniformGridAccStructBuilder.cpp:91 and exposed it in its GUI,
           WdgTransformEdit.cpp:49-69)
-  sched   per traversal stage, 's' one ray per lane or 'p' persistent refill
+  sched   per traversal stage, 's' one ray per lane, 'w' one ray per lane with
+          the wave's triangle tests pooled, or 'p' persistent refill
   refill  idle lanes that trigger a refill in the persistent schedule
   block   threads per block of the traversal kernels
 
@@ -55,6 +56,9 @@ class Autotuner:
         return best
 
 
-def per_stage_schedule(stage_ms_simple: list, stage_ms_persistent: list) -> str:
-    """Pick, stage by stage, the faster of the two traversal schedules."""
-    return "".join("s" if a <= b else "p" for a, b in zip(stage_ms_simple, stage_ms_persistent))
+def per_stage_schedule(stage_ms: dict) -> str:
+    """Pick, stage by stage, the fastest traversal schedule: stage_ms maps a
+    schedule character to its list of per-stage milliseconds."""
+    chars = list(stage_ms)
+    n = len(stage_ms[chars[0]])
+    return "".join(min(chars, key=lambda ch: stage_ms[ch][i]) for i in range(n))

@@ -60,6 +60,8 @@ def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120,
     p50/p99 latency per load, the p99 ratio plain/serve, and the highest load
     of the sweep at which each path keeps p99 within slo_ms."""
     torch = _torch()
+    ctx.set_trace_mode(False, 16)   # both paths: one ray per lane, cooperative tails
+    ctx.set_coop(16)
     warm = _requests(scenes, W.serve_requests(6, 0.0, len(scenes), seed=W.SEED_SERVE + 7))
     serve_run(ctx, warm, True, budget, max_active)
     serve_run(ctx, warm, False, budget, max_active)
@@ -216,9 +218,9 @@ def batch_config(reps: int = 5):
     jobs = W.batch_jobs()
     bj = BatchedJobs(ctx, jobs, 5.0)
     tuner = Autotuner()
-    cands = [{"lam": lam, "block": blk, "persistent": False, "refill": 16}
-             for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256)]
-    cands += [{"lam": lam, "block": blk, "persistent": True, "refill": rf}
+    cands = [{"lam": lam, "block": blk, "mode": m, "refill": 16}
+             for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256) for m in ("simple", "pooled")]
+    cands += [{"lam": lam, "block": blk, "mode": "persistent", "refill": rf}
               for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256) for rf in (8, 16, 32)]
     cur = {"lam": 5.0}
 
@@ -227,7 +229,7 @@ def batch_config(reps: int = 5):
             bj.set_lambda(c["lam"])
             cur["lam"] = c["lam"]
         ctx.set_block_size(c["block"], c["block"])
-        ctx.set_trace_mode(c["persistent"], c["refill"])
+        ctx.set_trace_mode(c["mode"], c["refill"])
         bj.run_batched()
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -239,7 +241,7 @@ def batch_config(reps: int = 5):
     if best["lam"] != cur["lam"]:
         bj.set_lambda(best["lam"])
     ctx.set_block_size(best["block"], best["block"])
-    ctx.set_trace_mode(best["persistent"], best["refill"])
+    ctx.set_trace_mode(best["mode"], best["refill"])
     rays = bj.active_rays()
     g = Graph(ctx)
     g.begin()
@@ -256,12 +258,11 @@ def batch_config(reps: int = 5):
     t_eager = _time(lambda: (bj.run_batched(), ctx.synchronize()), reps, torch)
     # the plain path at the same lambda, with its better schedule
     t_plain_by = {}
-    for persistent in (False, True):
-        ctx.set_trace_mode(persistent, best["refill"])
-        t_plain_by["persistent" if persistent else "simple"] = \
-            _time(lambda: (bj.run_plain(), ctx.synchronize()), reps, torch)
+    for mode in ("simple", "pooled", "persistent"):
+        ctx.set_trace_mode(mode, best["refill"])
+        t_plain_by[mode] = _time(lambda: (bj.run_plain(), ctx.synchronize()), reps, torch)
     t_plain = min(t_plain_by.values())
-    ctx.set_trace_mode(best["persistent"], best["refill"])
+    ctx.set_trace_mode(best["mode"], best["refill"])
     g.close()
     out = {"jobs": len(jobs), "triangles": int(sum(j["tris"] for j in bj.jobs)),
            "primary_rays": int(sum(j["n"] for j in bj.jobs)), "rays_per_run": rays,
@@ -331,20 +332,20 @@ def ao_config(lams=(4.0, 8.0, 16.0), reps: int = 3, width: int = 1024, height: i
     def measure(c):
         setup(c["lam"])
         best = {}
-        for persistent in (False, True):
-            ctx.set_trace_mode(persistent, 16)
+        for mode in ("simple", "pooled", "persistent"):
+            ctx.set_trace_mode(mode, 16)
             frame(c["lam"], state["extent"])
             frame(c["lam"], state["extent"], record=True)
             torch.cuda.synchronize()
-            best[persistent] = sum(ev[i].elapsed_time(ev[i + 1]) for i in range(4))
-        c["persistent"] = best[True] < best[False]
-        return min(best.values()) * 1e-3
+            best[mode] = sum(ev[i].elapsed_time(ev[i + 1]) for i in range(4))
+        c["mode"] = min(best, key=best.get)
+        return best[c["mode"]] * 1e-3
 
     tuner = Autotuner()
     cands = [{"lam": lam} for lam in lams]
     pick = tuner.tune("ao-16M", cands, measure)
     setup(pick["lam"])
-    ctx.set_trace_mode(pick["persistent"], 16)
+    ctx.set_trace_mode(pick["mode"], 16)
     frame(pick["lam"], state["extent"])
     torch.cuda.synchronize()
     stage = np.zeros(4)
@@ -368,7 +369,7 @@ def ao_config(lams=(4.0, 8.0, 16.0), reps: int = 3, width: int = 1024, height: i
     build_ms, prim_ms, gen_ms, ao_ms = (float(x) for x in stage)
     gbs = active * bytes_ray / (ao_ms * 1e-3) / 1e9
     return {"triangles": n_tris, "ao_rays": active, "spp": spp, "lambda": pick["lam"],
-            "schedule": "persistent" if pick["persistent"] else "one ray per lane",
+            "schedule": pick["mode"],
             "primary_hit_rate": round(float((hits.f["id"] >= 0).float().mean().item()), 4),
             "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"]},
             "frame_ms": round(float(stage.sum()), 3),

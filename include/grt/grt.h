@@ -84,13 +84,17 @@ int grt_memory_used(grt_context ctx, uint64_t* bytes);
 int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
 
 /* Traversal scheduling.  GRT_TRACE_SIMPLE (default): one ray per lane for the
- * whole launch.  GRT_TRACE_PERSISTENT: persistent waves whose idle lanes fetch
- * new rays once `refill` (1..64) lanes of a wave are idle.  Both finish dense
- * cells wave-cooperatively (grt_set_coop).  Results are identical; only the
- * schedule differs.  Batches (grt_batch_trace) and the serving loop follow the
- * same setting. */
+ * whole launch, each lane testing its own cell's triangles.  GRT_TRACE_POOLED:
+ * one ray per lane, the triangle tests of all the wave's current cells pooled
+ * and spread over the wave's lanes.  GRT_TRACE_PERSISTENT: persistent waves
+ * whose idle lanes fetch new rays once `refill` (1..64) lanes of a wave are
+ * idle.  The per-lane schedules finish dense cells wave-cooperatively
+ * (grt_set_coop).  Results and counters are identical; only the schedule
+ * differs.  Batches (grt_batch_trace) and the serving loop follow the same
+ * setting. */
 #define GRT_TRACE_SIMPLE     0
 #define GRT_TRACE_PERSISTENT 1
+#define GRT_TRACE_POOLED     2
 int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill);
 /* Persistent and batched traversal: once a wave finds the ray queue drained,
  * the wave tests the rest of a lane's cell 64 references at a time when at

@@ -71,9 +71,9 @@ def test_batch_items_match_oracle(gpu_ctx, three_scenes):
             want.append(orc.trace_closest(fr, offs, refs, v, rays))
     # both batched kernels (one ray per lane, persistent), cooperative pass on
     # and off, two item orders
-    for persistent, coop, rev in ((False, 16, False), (False, 0, True), (True, 16, True),
-                                  (True, 1, False)):
-        gpu_ctx.set_trace_mode(persistent, 16)
+    for mode, coop, rev in (("simple", 16, False), ("simple", 0, True), ("pooled", 16, True),
+                            ("persistent", 16, True), ("persistent", 1, False)):
+        gpu_ctx.set_trace_mode(mode, 16)
         gpu_ctx.set_coop(coop)
         order = list(range(len(items)))
         if rev:
@@ -88,7 +88,7 @@ def test_batch_items_match_oracle(gpu_ctx, three_scenes):
         assert batch.slots() % 64 == 0
         batch.trace()
         torch.cuda.synchronize()
-        tag = f"[persistent={persistent} coop={coop}]"
+        tag = f"[{mode} coop={coop}]"
         for k, (spec, out, w) in enumerate(zip(specs, outs, want)):
             if spec[2]:
                 assert_same(out.cpu().numpy(), w, f"item {k} occluded {tag}")
@@ -180,18 +180,19 @@ def test_serve_result_same_alone_and_in_mix(gpu_ctx, three_scenes):
 
     mix, res_mix = make(reqs, True, [0.0] * len(reqs))
     assert np.all(res_mix[2] >= res_mix[1]) and np.all(res_mix[3] > 0)
-    # the persistent batched kernel serves the same mix identically
-    gpu_ctx.set_trace_mode(True, 16)
-    try:
-        mixp, _ = make(reqs, True, [0.0] * len(reqs))
-    finally:
-        gpu_ctx.set_trace_mode(False, 16)
-    for k in range(len(reqs)):
-        a, b = mix[k][0].numpy(), mixp[k][0].numpy()
-        for f in ("id", "t", "u", "v"):
-            assert_same(a[f], b[f], f"request {k} primary hit.{f} (persistent batch)")
-        assert_same(mix[k][1].cpu().numpy(), mixp[k][1].cpu().numpy(),
-                    f"request {k} shadow (persistent batch)")
+    # the persistent and pooled batched kernels serve the same mix identically
+    for mode in ("persistent", "pooled"):
+        gpu_ctx.set_trace_mode(mode, 16)
+        try:
+            mixp, _ = make(reqs, True, [0.0] * len(reqs))
+        finally:
+            gpu_ctx.set_trace_mode("simple", 16)
+        for k in range(len(reqs)):
+            a, b = mix[k][0].numpy(), mixp[k][0].numpy()
+            for f in ("id", "t", "u", "v"):
+                assert_same(a[f], b[f], f"request {k} primary hit.{f} ({mode} batch)")
+            assert_same(mix[k][1].cpu().numpy(), mixp[k][1].cpu().numpy(),
+                        f"request {k} shadow ({mode} batch)")
     for k, r in enumerate(reqs):
         alone, res_alone = make([r], False, [0.0])
         assert res_alone[3][0] == res_mix[3][k], "same active-ray count alone and in the mix"

@@ -56,10 +56,12 @@ def check_grid(ctx, v, n, lam):
 
 
 # Every traversal schedule must give identical per-ray results and counters:
-# (persistent, refill, cooperative threshold).  The persistent waves finish
-# dense cells cooperatively once the queue is drained; thresholds 1 and 2 make
-# nearly every tail cell cooperative, 0 turns the pass off.
-MODES = [(False, 16, 16), (True, 16, 0), (True, 16, 16), (True, 1, 1), (True, 64, 2)]
+# (schedule, refill, cooperative threshold).  The per-lane and persistent
+# schedules finish dense cells cooperatively; thresholds 1 and 2 make nearly
+# every such cell cooperative, 0 turns the pass off.  The pooled schedule
+# spreads every wave's tests over its lanes.
+MODES = [("simple", 16, 16), ("simple", 16, 0), ("pooled", 16, 16), ("persistent", 16, 0),
+         ("persistent", 16, 16), ("persistent", 1, 1), ("persistent", 64, 2)]
 
 
 def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
@@ -67,20 +69,20 @@ def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
     n = rays["ox"].shape[0]
     o = orc.trace_closest(fr, offs, refs, v, rays, counters=True)
     rb = grt.RayBatch.from_numpy(rays)
-    for persistent, refill, coop in MODES:
-        sc.ctx.set_trace_mode(persistent, refill)
+    for mode, refill, coop in MODES:
+        sc.ctx.set_trace_mode(mode, refill)
         sc.ctx.set_coop(coop)
         hb = grt.HitBatch.empty(n)
         cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
         sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
         g = hb.numpy()
         gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-        tag = f"[persistent={persistent} refill={refill} coop={coop}]"
+        tag = f"[{mode} refill={refill} coop={coop}]"
         for k in ("id", "t", "u", "v"):
             assert_same(g[k], o[k], f"hit.{k} {tag}")
         assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
         assert_same(gc[:, 1], o["tests"], f"triangle tests {tag}")
-    sc.ctx.set_trace_mode(True, 16)
+    sc.ctx.set_trace_mode("simple", 16)
     sc.ctx.set_coop(16)
     return g, o
 
@@ -90,18 +92,18 @@ def check_any(sc, fr, offs, refs, v, rays):
     n = rays["ox"].shape[0]
     o = orc.trace_any(fr, offs, refs, v, rays, counters=True)
     rb = grt.RayBatch.from_numpy(rays)
-    for persistent, refill, coop in MODES:
-        sc.ctx.set_trace_mode(persistent, refill)
+    for mode, refill, coop in MODES:
+        sc.ctx.set_trace_mode(mode, refill)
         sc.ctx.set_coop(coop)
         occ = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
         cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
         sc.trace_any(rb, occ, counters=cnt)
         gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-        tag = f"[persistent={persistent} refill={refill} coop={coop}]"
+        tag = f"[{mode} refill={refill} coop={coop}]"
         assert_same(occ.cpu().numpy(), o["occluded"], f"occluded {tag}")
         assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
         assert_same(gc[:, 1], o["tests"], f"triangle tests {tag}")
-    sc.ctx.set_trace_mode(True, 16)
+    sc.ctx.set_trace_mode("simple", 16)
     sc.ctx.set_coop(16)
     return o
 
