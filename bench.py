@@ -58,7 +58,8 @@ def parse():
     p.add_argument("--block", type=int, default=0, help="traversal block size (0 = default)")
     p.add_argument("--sched", default=None,
                    help="traversal schedule per stage (primary, shadow, bounce, bounce "
-                        "shadow): s = one ray per lane, p = persistent refill; default: tuned")
+                        "shadow): s = one ray per lane, w = one ray per lane with pooled "
+                        "tests, p = persistent refill; default: tuned")
     p.add_argument("--refill", type=int, default=16)
     p.add_argument("--coop", type=int, default=16,
                    help="cooperative cell tests in drained persistent waves: minimum "

@@ -304,7 +304,7 @@ int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill)
 {
     if (!ctx)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: NULL context");
-    if (mode != GRT_TRACE_SIMPLE && mode != GRT_TRACE_PERSISTENT && mode != GRT_TRACE_POOLED)
+    if (mode < GRT_TRACE_SIMPLE || mode > GRT_TRACE_POOLED)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: unknown mode %d", mode);
     if (refill < 1 || refill > 64)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: refill must be 1..64");

@@ -329,7 +329,8 @@ class Context:
     def set_trace_mode(self, mode="simple", refill: int = 16):
         """Traversal schedule: "simple" (one ray per lane), "pooled" (one ray
         per lane, the wave's triangle tests pooled), "persistent" (refilled
-        persistent waves); a bool selects persistent (True) or simple."""
+        persistent waves, one test or cell per iteration); a bool selects
+        persistent (True) or simple."""
         if isinstance(mode, bool):
             mode = "persistent" if mode else "simple"
         check(lib().grt_set_trace_mode(self.h, self.TRACE_MODES[mode], refill))

@@ -60,22 +60,32 @@ def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120,
     p50/p99 latency per load, the p99 ratio plain/serve, and the highest load
     of the sweep at which each path keeps p99 within slo_ms."""
     torch = _torch()
-    ctx.set_trace_mode(False, 16)   # both paths: one ray per lane, cooperative tails
     ctx.set_coop(16)
     warm = _requests(scenes, W.serve_requests(6, 0.0, len(scenes), seed=W.SEED_SERVE + 7))
     serve_run(ctx, warm, True, budget, max_active)
     serve_run(ctx, warm, False, budget, max_active)
     torch.cuda.synchronize()
     closed = _requests(scenes, W.serve_requests(n_requests, 0.0, len(scenes)))
-    plain_sat = _summary(*serve_run(ctx, closed, False, budget, max_active))
-    serve_sat = _summary(*serve_run(ctx, closed, True, budget, max_active))
+    # Each path runs with the traversal schedule that saturates it best.
+    sat = {True: {}, False: {}}
+    for mode in ("simple", "pooled"):
+        ctx.set_trace_mode(mode, 16)
+        for batched in (False, True):
+            sat[batched][mode] = _summary(*serve_run(ctx, closed, batched, budget, max_active))
+    mode_of = {b: max(sat[b], key=lambda m: sat[b][m]["mrays_s"]) for b in sat}
+    plain_sat, serve_sat = sat[False][mode_of[False]], sat[True][mode_of[True]]
+
+    def run(reqs, batched):
+        ctx.set_trace_mode(mode_of[batched], 16)
+        return _summary(*serve_run(ctx, reqs, batched, budget, max_active))
+
     capacity = n_requests / plain_sat["wall_s"]
     sweep = []
     for load in loads:
         rate = load * capacity
         opened = _requests(scenes, W.serve_requests(n_open, rate, len(scenes)))
-        plain = _summary(*serve_run(ctx, opened, False, budget, max_active))
-        served = _summary(*serve_run(ctx, opened, True, budget, max_active))
+        plain = run(opened, False)
+        served = run(opened, True)
         sweep.append({"load_vs_plain_capacity": load, "arrival_rate_hz": round(rate, 2),
                       "serve": served, "plain": plain,
                       "p99_ratio": round(plain["p99_ms"] / max(served["p99_ms"], 1e-9), 2)})
@@ -87,6 +97,7 @@ def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120,
     top = sweep[-1]
     return {
         "plain_capacity_req_s": round(capacity, 2),
+        "schedule": {"serve": mode_of[True], "plain": mode_of[False]},
         "saturation": {"serve": serve_sat, "plain": plain_sat},
         "saturation_mrays_x": round(serve_sat["mrays_s"] / max(plain_sat["mrays_s"], 1e-9), 2),
         "sweep": sweep,

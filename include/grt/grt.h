@@ -88,8 +88,8 @@ int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
  * one ray per lane, the triangle tests of all the wave's current cells pooled
  * and spread over the wave's lanes.  GRT_TRACE_PERSISTENT: persistent waves
  * whose idle lanes fetch new rays once `refill` (1..64) lanes of a wave are
- * idle.  The per-lane schedules finish dense cells wave-cooperatively
- * (grt_set_coop).  Results and counters are identical; only the schedule
+ * idle, advancing each lane by one triangle test or cell per iteration.  The
+ * per-lane schedules finish dense cells wave-cooperatively (grt_set_coop).  Results and counters are identical; only the schedule
  * differs.  Batches (grt_batch_trace) and the serving loop follow the same
  * setting. */
 #define GRT_TRACE_SIMPLE     0
