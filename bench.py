@@ -59,7 +59,7 @@ def parse():
     p.add_argument("--sched", default=None,
                    help="traversal schedule per stage (primary, shadow, bounce, bounce "
                         "shadow): s = one ray per lane, w = one ray per lane with pooled "
-                        "tests, p = persistent refill; default: tuned")
+                        "tests, p = persistent refill, 2/4/8 = lanes per ray; default: tuned")
     p.add_argument("--refill", type=int, default=16)
     p.add_argument("--coop", type=int, default=16,
                    help="cooperative cell tests in drained persistent waves: minimum "
@@ -92,6 +92,8 @@ def kernel_name(closest, sched_char):
     """The traversal kernel a stage launches under a schedule character."""
     if sched_char == "p":
         return "k_trace_persistent<%s, false>" % ("false" if closest else "true")
+    if sched_char in "248":
+        return "k_trace_split<%s, false>" % ("false" if closest else "true")
     pooled = "true" if sched_char == "w" else "false"
     return ("k_trace_closest<false, %s>" if closest else "k_trace_any<false, %s>") % pooled
 
@@ -188,12 +190,15 @@ class Frame:
         sc.trace_any(self.shadow2, self.occ2, image_width=W)
         mark(10)
 
-    SCHEDULES = {"s": "simple", "w": "pooled", "p": "persistent"}
+    SCHEDULES = {"s": "simple", "w": "pooled", "p": "persistent", "2": "split", "4": "split",
+                 "8": "split"}
 
     def mode(self, stage):
         """Traversal schedule of a stage: 's' one ray per lane, 'w' one ray per
-        lane with the wave's tests pooled, 'p' persistent."""
-        self.ctx.set_trace_mode(self.SCHEDULES[self.sched[stage]], self.refill)
+        lane with the wave's tests pooled, 'p' persistent, '2'/'4'/'8' that
+        many lanes per ray."""
+        ch = self.sched[stage]
+        self.ctx.set_trace_mode(self.SCHEDULES[ch], int(ch) if ch in "248" else self.refill)
 
     def active(self, rb):
         f = rb.f

@@ -300,16 +300,23 @@ int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any)
     return GRT_OK;
 }
 
-int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill)
+int grt_set_trace_mode(grt_context ctx, int mode, uint32_t param)
 {
     if (!ctx)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: NULL context");
-    if (mode < GRT_TRACE_SIMPLE || mode > GRT_TRACE_POOLED)
+    if (mode < GRT_TRACE_SIMPLE || mode > GRT_TRACE_SPLIT)
         return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: unknown mode %d", mode);
-    if (refill < 1 || refill > 64)
-        return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: refill must be 1..64");
+    if (mode == GRT_TRACE_SPLIT) {
+        if (param != 2 && param != 4 && param != 8 && param != 16)
+            return set_error(GRT_INVALID_PARAM,
+                             "grt_set_trace_mode: lanes per ray must be 2, 4, 8 or 16");
+        ctx->split_lanes = param;
+    } else {
+        if (param < 1 || param > 64)
+            return set_error(GRT_INVALID_PARAM, "grt_set_trace_mode: refill must be 1..64");
+        ctx->refill = param;
+    }
     ctx->trace_mode = mode;
-    ctx->refill = refill;
     return GRT_OK;
 }
 
@@ -799,6 +806,10 @@ int grt_trace_closest(grt_scene sc, const grt_rays* rays, const grt_hits* hits, 
                                                  sc->small_counters.as<uint32_t>() + 2,
                                                  ctx->block_closest, ctx->refill, ctx->stream),
                          "trace_closest");
+    if (ctx->trace_mode == GRT_TRACE_SPLIT)
+        return hip_check(launch_trace_split(a, to_soa(rays), h, nullptr, false, ctx->split_lanes,
+                                            ctx->block_closest, ctx->stream),
+                         "trace_closest");
     return hip_check(launch_trace_closest(a, to_soa(rays), h, ctx->block_closest, ctx->stream),
                      "trace_closest");
 }
@@ -817,6 +828,10 @@ int grt_trace_any(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev, uin
         return hip_check(launch_trace_persistent(a, to_soa(rays), HitSoA{}, occluded_dev, true,
                                                  sc->small_counters.as<uint32_t>() + 2,
                                                  ctx->block_any, ctx->refill, ctx->stream),
+                         "trace_any");
+    if (ctx->trace_mode == GRT_TRACE_SPLIT)
+        return hip_check(launch_trace_split(a, to_soa(rays), HitSoA{}, occluded_dev, true,
+                                            ctx->split_lanes, ctx->block_any, ctx->stream),
                          "trace_any");
     return hip_check(launch_trace_any(a, to_soa(rays), occluded_dev, ctx->block_any, ctx->stream),
                      "trace_any");

@@ -52,6 +52,7 @@ struct grt_context_t {
     uint32_t block_any = 128;
     int trace_mode = GRT_TRACE_SIMPLE;
     uint32_t refill = 16;
+    uint32_t split_lanes = 4;   // GRT_TRACE_SPLIT: lanes per ray
     uint32_t coop = 16;   // cooperative cell tests in drained persistent waves (0: off)
     std::set<grt_scene_t*> scenes;
 };

@@ -106,6 +106,12 @@ hipError_t launch_trace_persistent(const TraceArgs& a, const RaySoA& rays, const
                                    uint32_t block, uint32_t refill, hipStream_t s);
 // One item of a batched launch: a ray batch against one scene's grid.  The
 // slot range [slot_begin, slot_begin + slots) is padded to whole waves (64).
+// Several lanes per ray (2, 4, 8 or 16): lane j of a ray's group visits the
+// ray's cells j, j + lanes, ... of the serial order (schedule GRT_TRACE_SPLIT).
+hipError_t launch_trace_split(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                              uint8_t* occluded, bool any, uint32_t lanes, uint32_t block,
+                              hipStream_t s);
+
 struct alignas(16) BatchTask {
     const DevFrame* fr;
     const uint32_t* offsets;

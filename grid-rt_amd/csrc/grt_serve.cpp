@@ -321,6 +321,10 @@ int grt_serve_run(grt_context ctx, const grt_request* reqs, uint32_t n_reqs,
                                                 it.sc->small_counters.as<uint32_t>() + 2,
                                                 it.any ? ctx->block_any : ctx->block_closest,
                                                 ctx->refill, s);
+                else if (ctx->trace_mode == GRT_TRACE_SPLIT)
+                    e = launch_trace_split(a, it.rays.in(), it.hits.out(), it.occ, it.any,
+                                           ctx->split_lanes,
+                                           it.any ? ctx->block_any : ctx->block_closest, s);
                 else if (it.any)
                     e = launch_trace_any(a, it.rays.in(), it.occ, ctx->block_any, s);
                 else

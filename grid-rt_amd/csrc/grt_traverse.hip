@@ -849,6 +849,173 @@ __global__ __launch_bounds__(256) void k_trace_batch_simple(const BatchTask* __r
 }
 
 // ---------------------------------------------------------------------------
+// Several lanes per ray
+// ---------------------------------------------------------------------------
+//
+// A launch with few rays (secondary rays of a few pixels, a small request)
+// leaves most SIMDs with one wave or none, and each lane then pays the memory
+// latency of every cell and record of its ray in series.  Here a group of
+// `lanes` (2..16, a power of two) adjacent lanes shares one ray: lane j of the
+// group visits the ray's cells j, j + lanes, j + 2 lanes, ... of the serial
+// DDA order, stepping the DDA on registers in between.  After each round the
+// group keeps the earliest cell (lowest ordinal) that holds a hit -- the cell
+// where the serial loop would have stopped -- and discards later cells, so
+// hits, occlusion flags and the {cells, tests} counters equal the serial
+// ones.  Dense cells get the wave-cooperative pass as in the one-ray-per-lane
+// kernels.
+
+// Advance by `count` cells of the serial order: false when the ray's interval
+// ends in the current cell (its exit beyond tmax) or the ray leaves the grid.
+__device__ __forceinline__ bool dda_skip(Dda& s, uint32_t count, float tmax)
+{
+    for (uint32_t q = 0; q < count; ++q)
+        if (tmin_f(tmin_f(s.tx, s.ty), s.tz) > tmax || !dda_step(s))
+            return false;
+    return true;
+}
+
+// Serial tests of the rest of a lane's current cell, per-lane (vector) loads.
+template <bool ANY>
+__device__ __forceinline__ void lane_cell(const float4* __restrict__ trir, LaneRay& r)
+{
+    const uint32_t b = r.k, e = r.e;
+    float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
+    uint32_t k = b;
+    for (; k < e; ++k) {
+        const float4 cA = A, cB = B, cC = C;
+        if (k + 1 < e) {
+            const uint64_t q = 3ull * (k + 1);
+            A = trir[q];
+            B = trir[q + 1];
+            C = trir[q + 2];
+        }
+        if (ANY) {
+            if (any_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax)) {
+                r.id = 1;
+                break;
+            }
+        } else {
+            closest_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, r.bound,
+                         r.id, r.best, r.bu, r.bv);
+        }
+    }
+    r.tests += ((ANY && r.id >= 0) ? k + 1 : e) - b;
+    r.k = e;
+}
+
+// Sum over the `lanes`-lane group (every lane of the wave calls it).
+__device__ __forceinline__ uint32_t group_sum(uint32_t v, uint32_t lanes)
+{
+    for (uint32_t o = 1; o < lanes; o <<= 1)
+        v += (uint32_t)__shfl_xor((int)v, (int)o, 64);
+    return v;
+}
+
+// Ray i (valid: the whole group) traced by its group of `lanes` lanes.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ void trace_split_lane(const DevFrame* __restrict__ fr,
+                                                 const uint32_t* __restrict__ offsets,
+                                                 const float4* __restrict__ trir,
+                                                 const RaySoA& rays, const HitSoA& hits,
+                                                 uint8_t* __restrict__ occluded,
+                                                 uint32_t* __restrict__ counters, uint32_t coop,
+                                                 uint32_t lanes, uint32_t i, bool valid)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t j = lane & (lanes - 1u);
+    const uint32_t g0 = lane - j;
+    const uint64_t gmask = (lanes >= 64u ? ~0ull : ((1ull << lanes) - 1ull)) << g0;
+    LaneRay r = {};
+    r.id = -1;
+    r.best = INFINITY;
+    uint32_t cells = 0;
+    Dda s = {};
+    bool alive = false;
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    if (valid) {
+        r.o0 = rays.ox[i];
+        r.o1 = rays.oy[i];
+        r.o2 = rays.oz[i];
+        r.d0 = rays.dx[i];
+        r.d1 = rays.dy[i];
+        r.d2 = rays.dz[i];
+        r.tmin = rays.tmin[i];
+        r.tmax = rays.tmax[i];
+        alive = grid_ok(fr) &&
+                dda_setup(fr, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, s) &&
+                dda_skip(s, j, r.tmax);
+    }
+    float exit_t = 0.0f;
+    if (alive) {
+        load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
+        cells = 1;
+    }
+    bool winner = false;
+    while (__ballot(alive) != 0) {
+        const uint32_t t0 = r.tests;
+        if (const uint64_t hv0 = coop_lanes_simple(alive, r.k, r.e, coop)) {
+            for (uint64_t hv = hv0; hv != 0; hv &= hv - 1)
+                coop_cell((uint32_t)__builtin_ctzll(hv), ANY, trir, r);
+        }
+        if (alive && r.k < r.e)
+            lane_cell<ANY>(trir, r);
+        // The group's earliest cell with a hit ends the ray; cells after it
+        // (higher lanes of this round) are not part of the serial walk.
+        const uint64_t hm = __ballot(alive && r.id >= 0) & gmask;
+        if (hm != 0) {
+            const uint32_t jw = (uint32_t)__builtin_ctzll(hm) - g0;
+            if (alive && j > jw) {
+                --cells;
+                r.tests = t0;
+            }
+            winner = alive && j == jw;
+            alive = false;
+        } else if (alive) {
+            if (dda_skip(s, lanes, r.tmax)) {
+                load_cell(s, offsets, nx, ny, exit_t, r.bound, r.k, r.e);
+                ++cells;
+            } else {
+                alive = false;
+            }
+        }
+    }
+    const bool group_hit = (__ballot(winner) & gmask) != 0;
+    uint32_t sc = 0, st = 0;
+    if (COUNT) {
+        sc = group_sum(cells, lanes);
+        st = group_sum(r.tests, lanes);
+    }
+    if (!valid)
+        return;
+    if (ANY) {
+        if (j == 0)
+            occluded[i] = group_hit ? 1 : 0;
+    } else if (winner || (j == 0 && !group_hit)) {
+        hits.id[i] = r.id;
+        hits.t[i] = r.best;
+        hits.u[i] = r.bu;
+        hits.v[i] = r.bv;
+    }
+    if (COUNT && j == 0) {
+        counters[2ull * i] = sc;
+        counters[2ull * i + 1] = st;
+    }
+}
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(256) void k_trace_split(TraceArgs a, RaySoA rays, HitSoA hits,
+                                                     uint8_t* __restrict__ occluded,
+                                                     uint32_t lanes)
+{
+    const uint32_t shift = (uint32_t)__builtin_ctz(lanes);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid;
+    const uint32_t i = slot_ray(t >> shift, a.image_width, a.n, valid);
+    trace_split_lane<ANY, COUNT>(a.fr, a.offsets, a.trir, rays, hits, occluded, a.counters,
+                                 a.coop, lanes, i, valid);
+}
+
+// ---------------------------------------------------------------------------
 // Persistent waves with lane refill
 // ---------------------------------------------------------------------------
 
@@ -1245,6 +1412,36 @@ hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occ
     else
         hipLaunchKernelGGL((k_trace_any<false, false>), dim3(g), dim3(block), 0, s, a, rays,
                            occluded);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_split(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                              uint8_t* occluded, bool any, uint32_t lanes, uint32_t block,
+                              hipStream_t s)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    if (lanes < 2 || lanes > 16 || (lanes & (lanes - 1)) != 0)
+        return hipErrorInvalidValue;
+    const uint64_t threads = slot_count(a.n, a.image_width) * lanes;
+    if (threads > 0xffffffffull - block)
+        return hipErrorInvalidValue;
+    const dim3 g((uint32_t)((threads + block - 1) / block));
+    if (any) {
+        if (a.counters)
+            hipLaunchKernelGGL((k_trace_split<true, true>), g, dim3(block), 0, s, a, rays, hits,
+                               occluded, lanes);
+        else
+            hipLaunchKernelGGL((k_trace_split<true, false>), g, dim3(block), 0, s, a, rays, hits,
+                               occluded, lanes);
+    } else {
+        if (a.counters)
+            hipLaunchKernelGGL((k_trace_split<false, true>), g, dim3(block), 0, s, a, rays, hits,
+                               occluded, lanes);
+        else
+            hipLaunchKernelGGL((k_trace_split<false, false>), g, dim3(block), 0, s, a, rays, hits,
+                               occluded, lanes);
+    }
     return hipGetLastError();
 }
 

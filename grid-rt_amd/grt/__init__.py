@@ -324,16 +324,18 @@ class Context:
     def set_block_size(self, closest: int, any_hit: int):
         check(lib().grt_set_block_size(self.h, closest, any_hit))
 
-    TRACE_MODES = {"simple": 0, "persistent": 1, "pooled": 2}
+    TRACE_MODES = {"simple": 0, "persistent": 1, "pooled": 2, "split": 3}
 
-    def set_trace_mode(self, mode="simple", refill: int = 16):
+    def set_trace_mode(self, mode="simple", param: int = 16):
         """Traversal schedule: "simple" (one ray per lane), "pooled" (one ray
         per lane, the wave's triangle tests pooled), "persistent" (refilled
-        persistent waves, one test or cell per iteration); a bool selects
+        persistent waves, one test or cell per iteration; param = refill
+        threshold 1..64), "split" (param = 2, 4, 8 or 16 lanes per ray, each
+        lane visiting every param-th cell of the ray); a bool selects
         persistent (True) or simple."""
         if isinstance(mode, bool):
             mode = "persistent" if mode else "simple"
-        check(lib().grt_set_trace_mode(self.h, self.TRACE_MODES[mode], refill))
+        check(lib().grt_set_trace_mode(self.h, self.TRACE_MODES[mode], param))
 
     def set_coop(self, min_refs: int = 16):
         """Cooperative cell tests in drained persistent waves (0: off)."""

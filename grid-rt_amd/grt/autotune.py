@@ -6,7 +6,8 @@ that depend on the grid):
           k = 6 in UniformGridAccStructBuilder.cpp:91 and exposed it in its GUI,
           WdgTransformEdit.cpp:49-69)
   sched   per traversal stage, 's' one ray per lane, 'w' one ray per lane with
-          the wave's triangle tests pooled, or 'p' persistent refill
+          the wave's triangle tests pooled, 'p' persistent refill, or '2', '4',
+          '8' lanes per ray
   refill  idle lanes that trigger a refill in the persistent schedule
   block   threads per block of the traversal kernels
 

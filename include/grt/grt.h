@@ -87,15 +87,21 @@ int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
  * whole launch, each lane testing its own cell's triangles.  GRT_TRACE_POOLED:
  * one ray per lane, the triangle tests of all the wave's current cells pooled
  * and spread over the wave's lanes.  GRT_TRACE_PERSISTENT: persistent waves
- * whose idle lanes fetch new rays once `refill` (1..64) lanes of a wave are
- * idle, advancing each lane by one triangle test or cell per iteration.  The
- * per-lane schedules finish dense cells wave-cooperatively (grt_set_coop).  Results and counters are identical; only the schedule
- * differs.  Batches (grt_batch_trace) and the serving loop follow the same
- * setting. */
+ * whose idle lanes fetch new rays once `param` (1..64) lanes of a wave are
+ * idle, advancing each lane by one triangle test or cell per iteration.
+ * GRT_TRACE_SPLIT: `param` (2, 4, 8 or 16) lanes per ray, lane j visiting
+ * the ray's cells j, j + param, ... of the serial order; for launches with
+ * too few rays to fill the GPU.  For SIMPLE and POOLED `param` is kept as the
+ * refill threshold (1..64) of later persistent launches.  The per-lane
+ * schedules finish dense cells wave-cooperatively (grt_set_coop).  Results and
+ * counters are identical; only the schedule differs.  The serving loop follows
+ * the setting; batches (grt_batch_trace) follow it too, except that they run
+ * GRT_TRACE_SPLIT as GRT_TRACE_SIMPLE. */
 #define GRT_TRACE_SIMPLE     0
 #define GRT_TRACE_PERSISTENT 1
 #define GRT_TRACE_POOLED     2
-int grt_set_trace_mode(grt_context ctx, int mode, uint32_t refill);
+#define GRT_TRACE_SPLIT      3
+int grt_set_trace_mode(grt_context ctx, int mode, uint32_t param);
 /* Persistent and batched traversal: once a wave finds the ray queue drained,
  * the wave tests the rest of a lane's cell 64 references at a time when at
  * least max(min_refs, 2 x its busy lanes) are left (0 = never; default 16).

@@ -200,6 +200,15 @@ def test_serve_result_same_alone_and_in_mix(gpu_ctx, three_scenes):
         for f in ("id", "t", "u", "v"):
             assert_same(a[f], b[f], f"request {k} primary hit.{f}")
         assert_same(mix[k][1].cpu().numpy(), alone[0][1].cpu().numpy(), f"request {k} shadow")
+    # the plain path under the split schedule (several lanes per ray)
+    gpu_ctx.set_trace_mode("split", 4)
+    try:
+        alone, _ = make([reqs[1]], False, [0.0])
+    finally:
+        gpu_ctx.set_trace_mode("simple", 16)
+    for f in ("id", "t", "u", "v"):
+        assert_same(mix[1][0].numpy()[f], alone[0][0].numpy()[f], f"request 1 hit.{f} (split)")
+    assert_same(mix[1][1].cpu().numpy(), alone[0][1].cpu().numpy(), "request 1 shadow (split)")
     # and request 0 against the oracle
     sc, v, nrm, (fr, offs, refs) = three_scenes[0]
     rays = orc.primary_rays(grt.camera_dict(mix[0][2]))

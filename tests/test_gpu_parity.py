@@ -56,12 +56,14 @@ def check_grid(ctx, v, n, lam):
 
 
 # Every traversal schedule must give identical per-ray results and counters:
-# (schedule, refill, cooperative threshold).  The per-lane and persistent
-# schedules finish dense cells cooperatively; thresholds 1 and 2 make nearly
-# every such cell cooperative, 0 turns the pass off.  The pooled schedule
-# spreads every wave's tests over its lanes.
+# (schedule, refill threshold or lanes per ray, cooperative threshold).  The
+# per-lane, split and persistent schedules finish dense cells cooperatively;
+# thresholds 1 and 2 make nearly every such cell cooperative, 0 turns the pass
+# off.  The pooled schedule spreads every wave's tests over its lanes; the
+# split schedule gives each ray 2..16 lanes that take its cells in turn.
 MODES = [("simple", 16, 16), ("simple", 16, 0), ("pooled", 16, 16), ("persistent", 16, 0),
-         ("persistent", 16, 16), ("persistent", 1, 1), ("persistent", 64, 2)]
+         ("persistent", 16, 16), ("persistent", 1, 1), ("persistent", 64, 2),
+         ("split", 2, 16), ("split", 4, 0), ("split", 8, 1), ("split", 16, 16)]
 
 
 def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
@@ -77,7 +79,7 @@ def check_closest(sc, fr, offs, refs, v, rays, image_width=0):
         sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
         g = hb.numpy()
         gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-        tag = f"[{mode} refill={refill} coop={coop}]"
+        tag = f"[{mode} param={refill} coop={coop}]"
         for k in ("id", "t", "u", "v"):
             assert_same(g[k], o[k], f"hit.{k} {tag}")
         assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
@@ -99,13 +101,25 @@ def check_any(sc, fr, offs, refs, v, rays):
         cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
         sc.trace_any(rb, occ, counters=cnt)
         gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-        tag = f"[{mode} refill={refill} coop={coop}]"
+        tag = f"[{mode} param={refill} coop={coop}]"
         assert_same(occ.cpu().numpy(), o["occluded"], f"occluded {tag}")
         assert_same(gc[:, 0], o["cells"], f"cells visited {tag}")
         assert_same(gc[:, 1], o["tests"], f"triangle tests {tag}")
     sc.ctx.set_trace_mode("simple", 16)
     sc.ctx.set_coop(16)
     return o
+
+
+def test_trace_mode_validation(gpu_ctx):
+    for bad in (0, 1, 3, 32):
+        with pytest.raises(grt.GrtError):
+            gpu_ctx.set_trace_mode("split", bad)
+    for bad in (0, 65):
+        with pytest.raises(grt.GrtError):
+            gpu_ctx.set_trace_mode("persistent", bad)
+    with pytest.raises(grt.GrtError):
+        grt.check(grt.lib().grt_set_trace_mode(gpu_ctx.h, 4, 16))
+    gpu_ctx.set_trace_mode("simple", 16)
 
 
 # ---------------------------------------------------------------------------
