@@ -121,6 +121,9 @@ class Frame:
         self.rhits = grt.HitBatch.empty(self.n)
         self.shadow2 = grt.RayBatch.empty(self.n)
         self.occ2 = torch.empty(self.n, dtype=torch.uint8, device="cuda:0")
+        self.rgb = torch.empty((self.n, 3), dtype=torch.float32, device="cuda:0")
+        self.rgba = torch.empty(self.n, dtype=torch.int32, device="cuda:0")
+        self.phong = grt.phong({"light": LIGHT})
         # Grid buffers sized with headroom by one synchronous build of frame 0.
         self.xf = {}
         self.set_lambda(lam)
@@ -136,7 +139,7 @@ class Frame:
         info = self.scene.grid_info()
         self.scene.reserve_grid(int(info["m"] * 1.25) + 4096, int(info["r"] * 1.5) + 4096)
 
-    N_EVENTS = 11
+    N_EVENTS = 12
 
     def time_stages(self, frames=2):
         """Milliseconds of transform, build and the four ray stages (ray
@@ -155,8 +158,9 @@ class Frame:
 
     def run(self, frame, events=None):
         """One frame.  Events (if given, N_EVENTS of them on the tracing
-        stream) bracket every launch group: transform, build, and for each of
-        the four ray stages its generation kernel and its traversal kernel."""
+        stream) bracket every launch group: transform, build, for each of the
+        four ray stages its generation kernel and its traversal kernel, and the
+        Phong shading of both levels into the RGBA8 frame."""
         sc, W = self.scene, WIDTH
         ev = events if events is not None else [None] * self.N_EVENTS
 
@@ -189,6 +193,9 @@ class Frame:
         mark(9)
         sc.trace_any(self.shadow2, self.occ2, image_width=W)
         mark(10)
+        sc.shade_phong(self.prim, self.hits, self.rgb, self.occ, 0, self.phong)
+        sc.shade_phong(self.refl, self.rhits, self.rgb, self.occ2, 1, self.phong, rgba=self.rgba)
+        mark(11)
 
     SCHEDULES = {"s": "simple", "w": "pooled", "p": "persistent", "2": "split", "4": "split",
                  "8": "split"}
@@ -337,8 +344,7 @@ def main():
         if args.mode == "serve":
             ctx = grt.Context(0)
             scenes, tris = perf.build_serve_scenes(ctx)
-            res = perf.serve_vs_plain(ctx, scenes, n_requests=args.serve_requests,
-                                      loads=(0.5, 0.8, 0.95, 1.1))
+            res = perf.serve_vs_plain(ctx, scenes, n_requests=args.serve_requests)
             res["triangles"] = tris
         elif args.mode == "batch":
             res = perf.batch_config()
@@ -376,7 +382,7 @@ def main():
     d = np.mean([intervals(ev) for ev in events], axis=0)
     trace_ms = {"primary": d[3], "shadow": d[5], "bounce": d[7], "bounce_shadow": d[9]}
     stage_ms = {"transform": d[0], "build": d[1], "raygen": d[2] + d[4] + d[6] + d[8],
-                **trace_ms}
+                **trace_ms, "shade": d[10]}
     if args.profile:
         print(json.dumps({"mode": "profile", "lambda": lam, "schedule": sched, "coop": args.coop,
                           "ms_per_step": round(ms_per_step, 3),
@@ -433,6 +439,7 @@ def main():
         "config": {
             "workload": "stadium-1M: 2^20 tris (256 Gaussian clusters + 2-tri ground plane), "
                         "1920x1080 primary + shadow + 1 mirror bounce + bounce shadow, "
+                        "Phong shading of both levels into an RGBA8 frame, "
                         f"per-frame transform + device grid rebuild (lambda={lam:g}, autotuned)",
             "rays_per_frame": rays,
             "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"]},

@@ -914,6 +914,47 @@ int grt_shade_headlight(grt_scene sc, const grt_rays* rays, const grt_hits* hits
                      "shade_headlight");
 }
 
+void grt_phong_defaults(grt_phong* p)
+{
+    if (!p)
+        return;
+    for (int c = 0; c < 3; ++c) {
+        p->light[c] = 1e10f;
+        p->light_color[c] = 1.0f;
+        p->ambient[c] = 0.0f;
+        p->specular[c] = 0.5f;
+        p->background[c] = 1.0f;
+    }
+    p->spec_exp = 50;
+    p->shadow_coeff = 0.2f;
+    p->reflect_coeff = 0.4f;
+}
+
+int grt_shade_phong(grt_scene sc, const grt_rays* rays, const grt_hits* hits,
+                    const uint8_t* occluded_dev, uint32_t n, const float* diffuse_dev,
+                    const grt_phong* params, uint32_t depth, float* rgb_dev, uint32_t* rgba8_dev)
+{
+    if (!sc || !rays_ok(rays) || !hits_ok(hits) || !params || !rgb_dev)
+        return set_error(GRT_INVALID_PARAM, "grt_shade_phong: bad argument");
+    if (!sc->has_normals)
+        return set_error(GRT_NOT_READY, "grt_shade_phong: scene has no normals");
+    DevPhong p;
+    for (int c = 0; c < 3; ++c) {
+        p.light[c] = params->light[c];
+        p.light_color[c] = params->light_color[c];
+        p.ambient[c] = params->ambient[c];
+        p.specular[c] = params->specular[c];
+        p.background[c] = params->background[c];
+    }
+    p.spec_exp = params->spec_exp;
+    p.shadow_coeff = params->shadow_coeff;
+    p.reflect_coeff = params->reflect_coeff;
+    return hip_check(launch_shade_phong(n, to_soa(rays), to_hits_in(hits, sc->n_tris),
+                                        occluded_dev, sc->normals.as<float>(), diffuse_dev, p,
+                                        depth, rgb_dev, rgba8_dev, sc->ctx->stream),
+                     "shade_phong");
+}
+
 int grt_debug_scan(grt_context ctx, const uint32_t* in_dev, uint32_t* out_dev, uint32_t n)
 {
     if (!ctx || !in_dev || !out_dev || n == 0)

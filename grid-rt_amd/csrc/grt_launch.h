@@ -149,6 +149,23 @@ hipError_t launch_gen_ao(uint32_t n, uint32_t spp, uint64_t seed, const RaySoA& 
                          float ao_tmax, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_count_active(const float* tmin, const float* tmax, uint32_t n,
                                unsigned long long* count, hipStream_t s);
+// Phong parameters (grt_phong, shading_FRAG.glsl:8-36).
+struct DevPhong {
+    float light[3];
+    float light_color[3];
+    float ambient[3];
+    float specular[3];
+    uint32_t spec_exp;
+    float shadow_coeff;
+    float reflect_coeff;
+    float background[3];
+};
+// Phong shading into rgb (3 floats per ray, and RGBA8 when rgba is set):
+// depth 0 writes, depth d > 0 adds reflect_coeff^d x colour for hits.
+hipError_t launch_shade_phong(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
+                              const uint8_t* occluded, const float* normals,
+                              const float* diffuse, const DevPhong& p, uint32_t depth,
+                              float* rgb, uint32_t* rgba, hipStream_t s);
 hipError_t launch_shade_headlight(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
                                   const float* normals, uint32_t* rgba, hipStream_t s);
 

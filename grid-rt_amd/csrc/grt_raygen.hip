@@ -8,6 +8,8 @@
 //   reflection rays     Sample.cpp:7-11,44-52, traversalIntersection_FRAG.glsl:285-296
 //   shading normal      Sample.cpp:100-114, RayTracing_kernel.cu:92-102
 //   headlight shading   RayTracing_kernel.cu:104-120
+//   Phong shading       shading_FRAG.glsl:116-173, PhongMaterialColor.cpp:19-85,
+//                       reflection blend GpuRenderer.cpp:236-297
 // Every kernel is one output ray (or pixel) per lane, structure-of-arrays in
 // and out, and follows the CPU oracle's operation order bit for bit.
 #include "grt_launch.h"
@@ -262,6 +264,98 @@ __global__ __launch_bounds__(256) void k_shade_headlight(uint32_t n, RaySoA in, 
 }
 
 // ---------------------------------------------------------------------------
+// Phong shading with shadow dimming and reflection accumulation
+// ---------------------------------------------------------------------------
+// x^e by square-and-multiply from the lowest bit of e (the specular power; an
+// integer ladder is reproducible where a library powf is not).
+__device__ __forceinline__ float pow_u32(float x, uint32_t e)
+{
+    float r = 1.0f, b = x;
+    while (e) {
+        if (e & 1u)
+            r = r * b;
+        e >>= 1;
+        if (e)
+            b = b * b;
+    }
+    return r;
+}
+
+// One pixel per lane.  The point is lit when N.L > 0 and its shadow ray is
+// not occluded; lit: ambient + max(0, kd lc N.L) + s lc max(0, S.L)^e with
+// S = normalise(reflect(d, N)); shadowed: ambient + max(0, kd lc N.L) x
+// shadow_coeff.  depth 0 writes the colour (background for a miss); depth
+// d > 0 adds weight x colour for a hit.
+__global__ __launch_bounds__(256) void k_shade_phong(uint32_t n, RaySoA in, HitSoAIn hits,
+                                                     const uint8_t* __restrict__ occluded,
+                                                     const float* __restrict__ normals,
+                                                     const float* __restrict__ diffuse,
+                                                     DevPhong p, uint32_t depth, float weight,
+                                                     float* __restrict__ rgb,
+                                                     uint32_t* __restrict__ rgba)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int32_t id = hits.id[i];
+    float c0, c1, c2;
+    if (id >= 0 && (uint32_t)id < hits.n_tris) {
+        const float d0 = in.dx[i], d1 = in.dy[i], d2 = in.dz[i];
+        float h[3], nn[3];
+        hit_frame(normals, id, hits.t[i], hits.u[i], hits.v[i], in.ox[i], in.oy[i], in.oz[i],
+                  d0, d1, d2, h, nn);
+        const float l0 = p.light[0] - h[0], l1 = p.light[1] - h[1], l2 = p.light[2] - h[2];
+        const bool lit = dot3(nn[0], nn[1], nn[2], l0, l1, l2) > 0.0f &&
+                         !(occluded && occluded[i] != 0);
+        const float dist = sqrtf(dot3(l0, l1, l2, l0, l1, l2));
+        const float inv = 1.0f / dist;
+        const float L0 = l0 * inv, L1 = l1 * inv, L2 = l2 * inv;
+        const float proj = dot3(d0, d1, d2, nn[0], nn[1], nn[2]) * 2.0f;
+        float s0 = d0 - nn[0] * proj, s1 = d1 - nn[1] * proj, s2 = d2 - nn[2] * proj;
+        normalize3(s0, s1, s2);
+        const float sdl = dot3(s0, s1, s2, L0, L1, L2);
+        const float spec = (p.spec_exp > 0u && sdl > 0.0f) ? pow_u32(sdl, p.spec_exp) : 0.0f;
+        const float ndl = dot3(nn[0], nn[1], nn[2], L0, L1, L2);
+        float kd[3] = {1.0f, 1.0f, 1.0f};
+        if (diffuse) {
+            kd[0] = diffuse[3ull * (uint32_t)id];
+            kd[1] = diffuse[3ull * (uint32_t)id + 1];
+            kd[2] = diffuse[3ull * (uint32_t)id + 2];
+        }
+        float col[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float dif = kd[c] * (p.light_color[c] * ndl);
+            dif = dif > 0.0f ? dif : 0.0f;
+            col[c] = lit ? (p.ambient[c] + dif) + p.specular[c] * (p.light_color[c] * spec)
+                         : p.ambient[c] + dif * p.shadow_coeff;
+        }
+        if (depth == 0) {
+            c0 = col[0];
+            c1 = col[1];
+            c2 = col[2];
+        } else {
+            c0 = rgb[3ull * i] + weight * col[0];
+            c1 = rgb[3ull * i + 1] + weight * col[1];
+            c2 = rgb[3ull * i + 2] + weight * col[2];
+        }
+    } else if (depth == 0) {
+        c0 = p.background[0];
+        c1 = p.background[1];
+        c2 = p.background[2];
+    } else {
+        c0 = rgb[3ull * i];
+        c1 = rgb[3ull * i + 1];
+        c2 = rgb[3ull * i + 2];
+    }
+    rgb[3ull * i] = c0;
+    rgb[3ull * i + 1] = c1;
+    rgb[3ull * i + 2] = c2;
+    if (rgba)
+        rgba[i] = to_u8(c0) | (to_u8(c1) << 8) | (to_u8(c2) << 16) | (255u << 24);
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s)
@@ -336,6 +430,21 @@ hipError_t launch_shade_headlight(uint32_t n, const RaySoA& in, const HitSoAIn& 
         return hipSuccess;
     hipLaunchKernelGGL(k_shade_headlight, dim3((n + 255) / 256), dim3(256), 0, s, n, in, hits,
                        normals, rgba);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_phong(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
+                              const uint8_t* occluded, const float* normals,
+                              const float* diffuse, const DevPhong& p, uint32_t depth,
+                              float* rgb, uint32_t* rgba, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    float w = 1.0f;
+    for (uint32_t k = 0; k < depth; ++k)
+        w = w * p.reflect_coeff;
+    hipLaunchKernelGGL(k_shade_phong, dim3((n + 255) / 256), dim3(256), 0, s, n, in, hits,
+                       occluded, normals, diffuse, p, depth, w, rgb, rgba);
     return hipGetLastError();
 }
 

@@ -443,4 +443,145 @@ void HipRenderer::readHits(std::vector<int32_t>& id, std::vector<float>& t,
     hip_ok(hipMemcpy(v.data(), hits_.v, n * 4, hipMemcpyDeviceToHost), "read hits");
 }
 
+// ---------------------------------------------------------------------------
+// HipGridRenderer
+// ---------------------------------------------------------------------------
+HipGridRenderer::HipGridRenderer(grt_context ctx, PinholeCamera* camera, Geometry* geometry,
+                                 HipGridAccStructBuilder* builder)
+    : ctx_(ctx), camera_(camera), geometry_(geometry), builder_(builder)
+{
+    if (!camera || !geometry || !builder)
+        throw Error(GRT_INVALID_PARAM, "HipGridRenderer: camera, geometry and builder are required");
+    grt_phong_defaults(&phong_);
+}
+
+HipGridRenderer::~HipGridRenderer()
+{
+    if (buf_ || diffuse_)
+        grt_synchronize(ctx_);
+    if (buf_)
+        (void)hipFree(buf_);
+    if (diffuse_)
+        (void)hipFree(diffuse_);
+}
+
+void HipGridRenderer::receiveParameter(int paramId, void* paramValue)
+{
+    if (paramId == GRT_PARAM_BLOCK_SIZE && paramValue) {
+        const uint32_t b = *static_cast<const uint32_t*>(paramValue);
+        check(grt_set_block_size(ctx_, b, b));
+    }
+}
+
+void HipGridRenderer::rebuildGrid(bool recomputeResolution)
+{
+    (void)recomputeResolution;
+    need_rebuild_ = true;
+}
+
+void HipGridRenderer::setDiffuseColors(const std::vector<float>& rgb)
+{
+    if (!rgb.empty() && rgb.size() != geometry_->triDesc.size() * 3)
+        throw Error(GRT_INVALID_PARAM, "setDiffuseColors: need 3 floats per triangle");
+    if (diffuse_) {
+        check(grt_synchronize(ctx_));
+        (void)hipFree(diffuse_);
+        diffuse_ = nullptr;
+        diffuse_n_ = 0;
+    }
+    if (rgb.empty())
+        return;
+    hip_ok(hipMalloc(&diffuse_, rgb.size() * 4), "diffuse colours");
+    hip_ok(hipMemcpy(diffuse_, rgb.data(), rgb.size() * 4, hipMemcpyHostToDevice),
+           "diffuse colours");
+    diffuse_n_ = rgb.size() / 3;
+}
+
+// Per pixel: primary, shadow and bounce rays (8 floats each), two hit records
+// (4 words each), rgb (3 floats), the RGBA8 pixel, and the occlusion byte.
+void HipGridRenderer::reserve(uint32_t n)
+{
+    if (n <= cap_)
+        return;
+    if (buf_) {
+        check(grt_synchronize(ctx_));
+        (void)hipFree(buf_);
+        buf_ = nullptr;
+        cap_ = 0;
+    }
+    const size_t N = n;
+    hip_ok(hipMalloc(&buf_, N * 4 * 36 + N), "frame buffers");
+    cap_ = n;
+    float* f = static_cast<float*>(buf_);
+    auto rays_at = [&](size_t k) {
+        float* p = f + k * N;
+        return grt_rays{p, p + N, p + 2 * N, p + 3 * N, p + 4 * N, p + 5 * N, p + 6 * N,
+                        p + 7 * N};
+    };
+    rays_ = rays_at(0);
+    shadow_ = rays_at(8);
+    bounce_ = rays_at(16);
+    hits_ = {reinterpret_cast<int32_t*>(f + 24 * N), f + 25 * N, f + 26 * N, f + 27 * N};
+    bounce_hits_ = {reinterpret_cast<int32_t*>(f + 28 * N), f + 29 * N, f + 30 * N, f + 31 * N};
+    rgb_ = f + 32 * N;
+    frame_ = reinterpret_cast<uint32_t*>(f + 35 * N);
+    occluded_ = reinterpret_cast<uint8_t*>(f + 36 * N);
+}
+
+void HipGridRenderer::newFrame()   // GpuRenderer.cpp:43-92
+{
+    camera_->newFrame();
+    cam_ = camera_->toGrtCamera();
+    if (cam_.width == 0 || cam_.height == 0)
+        throw Error(GRT_INVALID_PARAM, "HipGridRenderer: empty viewport");
+    reserve(cam_.width * cam_.height);
+}
+
+void HipGridRenderer::render()   // GpuRenderer.cpp:94-297
+{
+    if (!geometry_->accStruct || animation_ || need_rebuild_) {
+        builder_->buildGeometry(geometry_.get());
+        need_rebuild_ = false;
+    }
+    HipGridAccStruct* acc = dynamic_cast<HipGridAccStruct*>(geometry_->accStruct.get());
+    if (!acc)
+        throw Error(GRT_NOT_READY, "HipGridRenderer: geometry has no HipGridAccStruct");
+    if (diffuse_ && diffuse_n_ != geometry_->triDesc.size())
+        throw Error(GRT_INVALID_PARAM, "HipGridRenderer: diffuse colours do not match the geometry");
+    if (!frame_)
+        newFrame();
+    const uint32_t n = cam_.width * cam_.height;
+    grt_scene sc = acc->scene();
+    check(grt_gen_primary(ctx_, &cam_, &rays_));
+    acc->traceNearest(rays_, hits_, n, cam_.width);
+    check(grt_gen_shadow(sc, &rays_, &hits_, n, phong_.light, 1e-3f, &shadow_));
+    acc->traceAny(shadow_, occluded_, n, cam_.width);
+    check(grt_shade_phong(sc, &rays_, &hits_, occluded_, n, diffuse_, &phong_, 0, rgb_,
+                          reflection_ ? nullptr : frame_));
+    if (!reflection_)
+        return;
+    check(grt_gen_reflection(sc, &rays_, &hits_, n, 1e-3f, &bounce_));
+    acc->traceNearest(bounce_, bounce_hits_, n, cam_.width);
+    check(grt_gen_shadow(sc, &bounce_, &bounce_hits_, n, phong_.light, 1e-3f, &shadow_));
+    acc->traceAny(shadow_, occluded_, n, cam_.width);
+    check(grt_shade_phong(sc, &bounce_, &bounce_hits_, occluded_, n, diffuse_, &phong_, 1, rgb_,
+                          frame_));
+}
+
+void HipGridRenderer::readFrameBuffer(std::vector<uint32_t>& rgba) const
+{
+    check(grt_synchronize(ctx_));
+    const size_t n = (size_t)cam_.width * cam_.height;
+    rgba.resize(n);
+    hip_ok(hipMemcpy(rgba.data(), frame_, n * 4, hipMemcpyDeviceToHost), "read frame");
+}
+
+void HipGridRenderer::readRgb(std::vector<float>& rgb) const
+{
+    check(grt_synchronize(ctx_));
+    const size_t n = (size_t)cam_.width * cam_.height;
+    rgb.resize(3 * n);
+    hip_ok(hipMemcpy(rgb.data(), rgb_, n * 12, hipMemcpyDeviceToHost), "read rgb");
+}
+
 }  // namespace grt

@@ -56,6 +56,14 @@ class GridInfo(ctypes.Structure):
                 ("n", ctypes.c_int32 * 3), ("m", ctypes.c_uint32), ("r", ctypes.c_uint32)]
 
 
+class Phong(ctypes.Structure):
+    """grt_phong: Phong shading parameters (shading_FRAG.glsl:8-36)."""
+    _fields_ = [("light", ctypes.c_float * 3), ("light_color", ctypes.c_float * 3),
+                ("ambient", ctypes.c_float * 3), ("specular", ctypes.c_float * 3),
+                ("spec_exp", ctypes.c_uint32), ("shadow_coeff", ctypes.c_float),
+                ("reflect_coeff", ctypes.c_float), ("background", ctypes.c_float * 3)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -101,6 +109,9 @@ _SIGS = {
                                   ctypes.c_uint64, _F, _F, ctypes.POINTER(Rays)]),
     "grt_shade_headlight": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32,
                                            _P]),
+    "grt_phong_defaults": (None, [ctypes.POINTER(Phong)]),
+    "grt_shade_phong": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _P, _U32,
+                                       _P, ctypes.POINTER(Phong), _U32, _P, _P]),
     "grt_debug_scan": (ctypes.c_int, [_P, _P, _P, _U32]),
     "grt_batch_create": (ctypes.c_int, [_P, _U32, ctypes.POINTER(_P)]),
     "grt_batch_set": (ctypes.c_int, [_P, _P, _U32]),
@@ -297,6 +308,23 @@ class HitBatch:
 # ---------------------------------------------------------------------------
 # Context / scene handles
 # ---------------------------------------------------------------------------
+def phong(params: dict | None = None) -> Phong:
+    """The shader's defaults (grt_phong_defaults), with `params` overriding
+    fields by name."""
+    p = Phong()
+    lib().grt_phong_defaults(ctypes.byref(p))
+    for k, v in (params or {}).items():
+        if k in ("light", "light_color", "ambient", "specular", "background"):
+            getattr(p, k)[:] = [float(x) for x in v]
+        elif k == "spec_exp":
+            p.spec_exp = int(v)
+        elif k in ("shadow_coeff", "reflect_coeff"):
+            setattr(p, k, float(v))
+        else:
+            raise KeyError(k)
+    return p
+
+
 class Context:
     """One device context; by default enqueues on torch's current stream."""
 
@@ -452,6 +480,19 @@ class Scene:
     def shade_headlight(self, rays: RayBatch, hits: HitBatch, rgba):
         check(lib().grt_shade_headlight(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()),
                                         rays.n, _ptr(rgba)))
+
+    def shade_phong(self, rays: RayBatch, hits: HitBatch, rgb, occluded=None, depth: int = 0,
+                    params: "Phong | dict | None" = None, diffuse=None, rgba=None):
+        """Phong shading into rgb (device float32, 3 per ray): depth 0 writes,
+        depth d > 0 adds reflect_coeff^d x colour for hits.  occluded: the
+        hits' shadow-ray flags; diffuse: 3 floats per triangle; rgba: optional
+        RGBA8 output (int32/uint32 per ray)."""
+        p = params if isinstance(params, Phong) else phong(params)
+        check(lib().grt_shade_phong(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()),
+                                    _ptr(occluded) if occluded is not None else None, rays.n,
+                                    _ptr(diffuse) if diffuse is not None else None,
+                                    ctypes.byref(p), depth, _ptr(rgb),
+                                    _ptr(rgba) if rgba is not None else None))
 
     def close(self):
         if getattr(self, "h", None):

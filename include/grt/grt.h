@@ -193,6 +193,38 @@ int grt_gen_ao(grt_scene scene, const grt_rays* in, const grt_hits* hits, uint32
 int grt_shade_headlight(grt_scene scene, const grt_rays* rays, const grt_hits* hits, uint32_t n,
                         uint32_t* rgba_dev);
 
+/* Phong shading of the GPU renderer (shading_FRAG.glsl:8-36,116-173;
+ * PhongMaterialColor::shade, PhongMaterialColor.cpp:19-85).  A hit is lit
+ * when N.L > 0 and its shadow ray is not occluded:
+ *   lit:      ambient + max(0, kd*lc*N.L) + specular*lc*max(0, S.L)^spec_exp
+ *   shadowed: ambient + max(0, kd*lc*N.L) * shadow_coeff
+ * with N the interpolated normal facing the ray, L the unit vector to the
+ * light and S = normalise(reflect(d, N)); spec_exp is an integer (0: no
+ * specular term), evaluated by square-and-multiply. */
+typedef struct {
+    float light[3];        /* point light position */
+    float light_color[3];
+    float ambient[3];
+    float specular[3];     /* specular colour */
+    uint32_t spec_exp;
+    float shadow_coeff;
+    float reflect_coeff;   /* weight of each reflection level */
+    float background[3];   /* colour of a primary miss */
+} grt_phong;
+/* The shader's constants: light (1e10,1e10,1e10), white; ambient 0;
+ * specular 0.5; exponent 50; shadow 0.2; reflection 0.4; background 1. */
+void grt_phong_defaults(grt_phong* p);
+/* Shade n hit records into rgb_dev (3 floats per ray, interleaved).  depth 0
+ * writes the colour (background for a miss); depth d > 0 adds
+ * reflect_coeff^d x colour for each hit of that reflection level and leaves
+ * misses unchanged (the additive blend of GpuRenderer.cpp:277-297).
+ * occluded_dev: the hits' shadow-ray flags (NULL: unshadowed); diffuse_dev:
+ * 3 floats per triangle (NULL: white); rgba8_dev (may be NULL) receives the
+ * clamped 8-bit colour of the accumulated rgb. */
+int grt_shade_phong(grt_scene scene, const grt_rays* rays, const grt_hits* hits,
+                    const uint8_t* occluded_dev, uint32_t n, const float* diffuse_dev,
+                    const grt_phong* params, uint32_t depth, float* rgb_dev, uint32_t* rgba8_dev);
+
 /* ---- batched multi-scene tracing ------------------------------------- */
 /* The reference traced one scene per launch (cudaRayTrace,
  * RayTracing.cu:227-253).  A batch holds items -- each a ray batch of one

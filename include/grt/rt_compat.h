@@ -335,6 +335,65 @@ private:
     uint32_t* frame_ = nullptr;
 };
 
+// rtgl::GpuRenderer (GpuRenderer.h:18-71, GpuRenderer.cpp:43-297,663-703):
+// render() rebuilds the grid when needed (every frame with animation
+// enabled, as the reference did after each keyframe upload), traces one
+// primary ray per pixel, runs the shadow pass and the Phong shading pass
+// (shading_FRAG.glsl), and with reflections enabled (ENABLE_REFLECTION,
+// GpuRenderer.cpp:24) adds one mirror bounce with its own shadow pass.  The
+// frame is RGBA8 in frameBuffer(), converted from the float colours in rgb().
+// Shadow and reflection rays start 1e-3 off the surface along the normal
+// (shadow_FRAG.glsl:267-268).
+class HipGridRenderer : public IRenderer {
+public:
+    HipGridRenderer(grt_context ctx, PinholeCamera* camera, Geometry* geometry,
+                    HipGridAccStructBuilder* builder);
+    void newFrame() override;
+    void render() override;
+    void receiveParameter(int paramId, void* paramValue) override;
+
+    void setAnimationEnabled(bool enabled) { animation_ = enabled; }
+    void setReflectionEnabled(bool enabled) { reflection_ = enabled; }
+    // Rebuild the geometry's grid with the builder before the next frame.  The
+    // resolution always follows the current box and lambda, so
+    // recomputeResolution = false gives the same grid as true.
+    void rebuildGrid(bool recomputeResolution);
+    // Shading parameters (grt_phong_defaults at construction).
+    grt_phong& phong() { return phong_; }
+    // Per-triangle diffuse colours, 3 floats per triangle (the reference's
+    // material texture); empty = white.
+    void setDiffuseColors(const std::vector<float>& rgb);
+
+    const uint32_t* frameBuffer() const { return frame_; }
+    const float* rgb() const { return rgb_; }
+    void readFrameBuffer(std::vector<uint32_t>& rgba) const;
+    void readRgb(std::vector<float>& rgb) const;
+
+protected:
+    ~HipGridRenderer() override;
+
+private:
+    void reserve(uint32_t n);
+    grt_context ctx_;
+    ref_ptr<PinholeCamera> camera_;
+    ref_ptr<Geometry> geometry_;
+    ref_ptr<HipGridAccStructBuilder> builder_;
+    grt_camera cam_{};
+    grt_phong phong_{};
+    bool animation_ = false;
+    bool reflection_ = false;
+    bool need_rebuild_ = false;
+    void* buf_ = nullptr;
+    uint32_t cap_ = 0;
+    grt_rays rays_{}, shadow_{}, bounce_{};
+    grt_hits hits_{}, bounce_hits_{};
+    uint8_t* occluded_ = nullptr;
+    float* rgb_ = nullptr;
+    uint32_t* frame_ = nullptr;
+    float* diffuse_ = nullptr;
+    size_t diffuse_n_ = 0;
+};
+
 }  // namespace grt
 
 #endif  // GRT_RT_COMPAT_H

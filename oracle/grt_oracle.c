@@ -893,3 +893,84 @@ void orc_headlight_image(uint32_t n, const float* dx, const float* dy, const flo
         rgba[i] = orc_headlight_rgba8(d, n9, id[i], u[i], v[i]);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Phong shading (shading_FRAG.glsl:116-173; PhongMaterialColor.cpp:19-85)   */
+/* ------------------------------------------------------------------------ */
+float orc_pow_u32(float x, uint32_t e)
+{
+    float r = 1.0f, b = x;
+    while (e) {
+        if (e & 1u)
+            r = r * b;
+        e >>= 1;
+        if (e)
+            b = b * b;
+    }
+    return r;
+}
+
+/* Colour of one hit: hit point and shading normal facing the ray as for the
+ * secondary rays; L = normalised vector to the light; the point is lit when
+ * N.L > 0 and its shadow ray is not occluded (a point facing away from the
+ * light gets no shadow ray and counts as shadowed).  Specular vector =
+ * normalise(reflect(d, N)); specular = s x lc x max(0, S.L)^e; diffuse =
+ * max(0, kd x lc x N.L); lit: ambient + diffuse + specular, shadowed:
+ * ambient + diffuse x shadow_coeff. */
+static void phong_colour(const float o[3], const float d[3], float t, const float* n9, float u,
+                         float v, int shadowed, const float kd[3], const orc_phong* p,
+                         float col[3])
+{
+    float h[3], nn[3];
+    hit_frame(o, d, t, n9, u, v, h, nn);
+    const float l[3] = {p->light[0] - h[0], p->light[1] - h[1], p->light[2] - h[2]};
+    const int lit = dot3(nn, l) > 0.0f && !shadowed;
+    const float dist = sqrtf(dot3(l, l));
+    const float inv = 1.0f / dist;
+    const float L[3] = {l[0] * inv, l[1] * inv, l[2] * inv};
+    const float proj = dot3(d, nn) * 2.0f;
+    float s[3] = {d[0] - nn[0] * proj, d[1] - nn[1] * proj, d[2] - nn[2] * proj};
+    normalize3(s);
+    const float sdl = dot3(s, L);
+    const float spec = (p->spec_exp > 0 && sdl > 0.0f) ? orc_pow_u32(sdl, p->spec_exp) : 0.0f;
+    const float ndl = dot3(nn, L);
+    for (int c = 0; c < 3; ++c) {
+        float dif = kd[c] * (p->light_color[c] * ndl);
+        dif = dif > 0.0f ? dif : 0.0f;
+        if (lit)
+            col[c] = (p->ambient[c] + dif) + p->specular[c] * (p->light_color[c] * spec);
+        else
+            col[c] = p->ambient[c] + dif * p->shadow_coeff;
+    }
+}
+
+void orc_shade_phong(uint32_t n, const float* ox, const float* oy, const float* oz,
+                     const float* dx, const float* dy, const float* dz, const int32_t* hit_id,
+                     const float* hit_t, const float* hit_u, const float* hit_v,
+                     const uint8_t* occluded, const float* normals, const float* diffuse_rgb,
+                     const orc_phong* p, uint32_t depth, float* rgb, uint32_t* rgba8)
+{
+    float w = 1.0f;
+    for (uint32_t k = 0; k < depth; ++k)
+        w = w * p->reflect_coeff;
+    static const float white[3] = {1.0f, 1.0f, 1.0f};
+    for (size_t i = 0; i < n; ++i) {
+        float* out = rgb + 3 * i;
+        if (hit_id[i] >= 0) {
+            const uint32_t id = (uint32_t)hit_id[i];
+            const float o[3] = {ox[i], oy[i], oz[i]};
+            const float d[3] = {dx[i], dy[i], dz[i]};
+            const float* kd = diffuse_rgb ? diffuse_rgb + 3ull * id : white;
+            float col[3];
+            phong_colour(o, d, hit_t[i], normals + 9ull * id, hit_u[i], hit_v[i],
+                         occluded ? occluded[i] != 0 : 0, kd, p, col);
+            for (int c = 0; c < 3; ++c)
+                out[c] = depth == 0 ? col[c] : out[c] + w * col[c];
+        } else if (depth == 0) {
+            for (int c = 0; c < 3; ++c)
+                out[c] = p->background[c];
+        }
+        if (rgba8)
+            rgba8[i] = to_u8(out[0]) | (to_u8(out[1]) << 8) | (to_u8(out[2]) << 16) | (255u << 24);
+    }
+}

@@ -23,6 +23,9 @@
  *   - secondary rays:  src/rtcore/Sample.cpp:7-52,95-114,
  *                      shaders/shadow_FRAG.glsl:259-270,
  *                      shaders/traversalIntersection_FRAG.glsl:285-296
+ *   - Phong shading:   shaders/shading_FRAG.glsl:8-36,116-173,
+ *                      src/rtplugins/PhongMaterialColor.cpp:19-85,
+ *                      src/rtgl/GpuRenderer.cpp:236-297 (reflection blend)
  *
  * Floating-point evaluation order is fixed: every fused multiply-add is an
  * explicit fmaf() call, everything else is a single IEEE-754 binary32
@@ -185,6 +188,32 @@ uint32_t orc_headlight_rgba8(const float d[3], const float* n9, int32_t id, floa
 void orc_headlight_image(uint32_t n, const float* dx, const float* dy, const float* dz,
                          const float* normals9, const int32_t* id, const float* u,
                          const float* v, uint32_t* rgba);
+
+/* Phong shading of the GPU renderer (shading_FRAG.glsl:8-36,116-173). */
+typedef struct {
+    float light[3];          /* point light position */
+    float light_color[3];
+    float ambient[3];
+    float specular[3];       /* specular colour */
+    uint32_t spec_exp;       /* integer specular exponent; 0: no specular term */
+    float shadow_coeff;      /* diffuse factor of a shadowed point */
+    float reflect_coeff;     /* weight of each reflection level */
+    float background[3];     /* colour of a primary miss */
+} orc_phong;
+
+/* x^e by square-and-multiply from the lowest bit of e (x^0 = 1). */
+float orc_pow_u32(float x, uint32_t e);
+/* Shade the hits of n rays into rgb (3 floats per ray).  depth 0 writes
+ * rgb = colour (background for misses); depth d > 0 adds reflect_coeff^d x
+ * colour for hits and leaves misses unchanged.  occluded: the flags of the
+ * hits' shadow rays (NULL: none).  diffuse_rgb: 3 floats per triangle (NULL:
+ * white).  rgba8 (may be NULL) receives the clamped 8-bit colour of the
+ * resulting rgb. */
+void orc_shade_phong(uint32_t n, const float* ox, const float* oy, const float* oz,
+                     const float* dx, const float* dy, const float* dz, const int32_t* hit_id,
+                     const float* hit_t, const float* hit_u, const float* hit_v,
+                     const uint8_t* occluded, const float* normals, const float* diffuse_rgb,
+                     const orc_phong* p, uint32_t depth, float* rgb, uint32_t* rgba8);
 
 /* Wall-clock seconds (for the CPU baseline). */
 double orc_now(void);

@@ -30,6 +30,31 @@ class Camera(ctypes.Structure):
                 ("width", _U32), ("height", _U32)]
 
 
+class Phong(ctypes.Structure):
+    _fields_ = [("light", _F * 3), ("light_color", _F * 3), ("ambient", _F * 3),
+                ("specular", _F * 3), ("spec_exp", _U32), ("shadow_coeff", _F),
+                ("reflect_coeff", _F), ("background", _F * 3)]
+
+
+# shading_FRAG.glsl:8-36: background 1, ambient 0, specular 0.5, exponent 50,
+# shadow 0.2, reflection 0.4, white light at (1e10, 1e10, 1e10).
+PHONG_DEFAULTS = {"light": (1e10, 1e10, 1e10), "light_color": (1.0, 1.0, 1.0),
+                  "ambient": (0.0, 0.0, 0.0), "specular": (0.5, 0.5, 0.5), "spec_exp": 50,
+                  "shadow_coeff": 0.2, "reflect_coeff": 0.4, "background": (1.0, 1.0, 1.0)}
+
+
+def phong_struct(params: dict | None = None) -> Phong:
+    d = dict(PHONG_DEFAULTS)
+    d.update(params or {})
+    p = Phong()
+    for k in ("light", "light_color", "ambient", "specular", "background"):
+        getattr(p, k)[:] = [float(x) for x in d[k]]
+    p.spec_exp = int(d["spec_exp"])
+    p.shadow_coeff = float(d["shadow_coeff"])
+    p.reflect_coeff = float(d["reflect_coeff"])
+    return p
+
+
 _lib = None
 
 
@@ -74,6 +99,9 @@ def lib():
             "orc_brute_any_f64": (None, [_P, _U32, _U32] + [_P] * 8 + [_P, _P, ctypes.c_int]),
             "orc_headlight_rgba8": (_U32, [_F * 3, _P, ctypes.c_int32, _F, _F]),
             "orc_headlight_image": (None, [_U32, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "orc_pow_u32": (_F, [_F, _U32]),
+            "orc_shade_phong": (None, [_U32] + [_P] * 6 + [_P] * 4 + [_P, _P, _P,
+                                ctypes.POINTER(Phong), _U32, _P, _P]),
             "orc_now": (ctypes.c_double, []),
             "orc_max_threads": (ctypes.c_int, []),
         }
@@ -349,6 +377,29 @@ def headlight_image(rays: dict, hits: dict, normals) -> np.ndarray:
     lib().orc_headlight_image(n, _p(d[0]), _p(d[1]), _p(d[2]), _p(nrm), _p(ids), _p(uv[0]),
                               _p(uv[1]), _p(out))
     return out
+
+
+def pow_u32(x: float, e: int) -> float:
+    return lib().orc_pow_u32(x, e)
+
+
+def shade_phong(rays: dict, hits: dict, normals, occluded=None, params: dict | None = None,
+                depth: int = 0, rgb=None, diffuse=None):
+    """Phong shading of closest-hit records: returns (rgb float32 [n, 3], rgba8
+    uint32 [n]).  depth 0 writes rgb; depth d > 0 adds reflect_coeff^d x colour
+    for hits into a copy of `rgb`."""
+    n = rays["dx"].shape[0]
+    out = np.zeros((n, 3), np.float32) if rgb is None else np.array(rgb, np.float32).reshape(n, 3)
+    img = np.empty(n, np.uint32)
+    r = [_f32(rays[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz")]
+    h = [np.ascontiguousarray(hits["id"], np.int32)] + [_f32(hits[k]) for k in ("t", "u", "v")]
+    occ = None if occluded is None else np.ascontiguousarray(occluded, np.uint8)
+    kd = None if diffuse is None else _f32(diffuse).reshape(-1)
+    p = phong_struct(params)
+    lib().orc_shade_phong(n, *[_p(a) for a in r], *[_p(a) for a in h], _p(occ),
+                          _p(_f32(normals).reshape(-1)), _p(kd), ctypes.byref(p), depth,
+                          _p(out), _p(img))
+    return out, img
 
 
 def now() -> float:

@@ -42,7 +42,7 @@ def test_library_exports_every_declared_function():
         missing = [n for n in names if n not in exported]
         assert not missing, f"{h}: declared but not exported: {missing}"
         total += len(names)
-    assert total >= 67
+    assert total >= 69
     # every bound signature in the Python package is a real export
     assert not [n for n in grt._SIGS if n not in exported]
 
@@ -52,6 +52,7 @@ def test_cpp_plugin_classes_are_exported():
                          text=True, check=True).stdout
     for sym in ("grt::HipGridAccStruct::traceNearestGeometry", "grt::HipGridAccStruct::traceNearest",
                 "grt::HipGridAccStructBuilder::buildGeometry", "grt::HipRenderer::render",
+                "grt::HipGridRenderer::render", "grt::HipGridRenderer::rebuildGrid",
                 "grt::PinholeCamera::getRayDirection", "grt::IAccStruct::traceAnyInstance",
                 "grt::IPlugin::receiveParameter", "grt::ICamera::setLookAt"):
         assert sym in out, sym
@@ -185,3 +186,15 @@ def test_pinhole_camera_plugin_matches_oracle(tmp_path):
     assert np.allclose(o, (0.5, 0.5, -2.5), atol=1e-5)
     i = 17 * 64 + 31
     assert np.allclose(d, [rays[k][i] for k in ("dx", "dy", "dz")], atol=1e-5)
+
+
+def test_phong_defaults_are_the_shader_constants():
+    """grt_phong_defaults (host only, no GPU) equals the oracle's restatement of
+    shading_FRAG.glsl:8-36."""
+    p = grt.phong()
+    for k, v in orc.PHONG_DEFAULTS.items():
+        got = getattr(p, k)
+        if isinstance(v, tuple):
+            assert list(got) == [float(np.float32(x)) for x in v], k
+        else:
+            assert got == np.float32(v), k

@@ -201,3 +201,43 @@ def test_cpp_plugin_classes_render(gpu_ctx, c1_scene, tmp_path):
     for k in ("id", "t", "u", "v"):
         assert_same(g[k], o[k], f"hit.{k}")
     assert_same(rgba, img, "RGBA8 frame")
+
+
+PHONG = os.path.join(os.path.dirname(grt.__file__), "..", "_build", "render_phong")
+
+
+@pytest.mark.parametrize("colours", [False, True])
+def test_cpp_phong_renderer(gpu_ctx, c1_scene, tmp_path, colours):
+    """grt::HipGridRenderer (the rtgl::GpuRenderer counterpart): primary,
+    shadow, Phong, one reflection with its shadow, against the oracle."""
+    from test_gpu_parity import _phong_frame_oracle
+    v, nrm = c1_scene
+    w, h, lam = 160, 120, 5.0
+    scene = tmp_path / "scene.bin"
+    with open(scene, "wb") as f:
+        np.asarray([v.shape[0]], np.uint32).tofile(f)
+        np.ascontiguousarray(v, np.float32).tofile(f)
+        np.ascontiguousarray(nrm, np.float32).tofile(f)
+    out = tmp_path / "frame.bin"
+    cmd = [PHONG, str(scene), str(out), str(w), str(h), str(lam), *map(str, EYE), *map(str, AT)]
+    diffuse = None
+    if colours:
+        diffuse = np.random.default_rng(5).uniform(0.1, 1.0, (v.shape[0], 3)).astype(np.float32)
+        cfile = tmp_path / "colours.bin"
+        diffuse.tofile(cfile)
+        cmd.append(str(cfile))
+    print("running", " ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    n = w * h
+    raw = np.fromfile(out, np.float32)
+    assert raw.size == 4 * n
+    rgb, rgba = raw[:3 * n].reshape(n, 3), raw[3 * n:].view(np.uint32)
+    fr = orc.grid_frame(v, lam)
+    _, offs, refs = orc.grid_build(v, fr)
+    cam = grt.camera_lookat(EYE, AT, (0, 1, 0), 60.0, w, h)
+    light = (1e10, 1e10, 1e10)
+    _, o_rgb, o_img, o, _ = _phong_frame_oracle(fr, offs, refs, v, nrm, cam, light, {}, diffuse)
+    assert_same(rgb, o_rgb, "rgb")
+    assert_same(rgba, o_img, "RGBA8 frame")
+    assert np.count_nonzero(o["id"] >= 0) > n // 20

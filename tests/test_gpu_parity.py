@@ -371,3 +371,70 @@ def test_transform_parity(gpu_ctx):
         cam = grt.camera_lookat((0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0, 1, 0), 60.0, 96, 54)
         rays = orc.primary_rays(grt.camera_dict(cam))
         check_closest(sc, fr, ooffs, orefs, ov, rays)
+
+
+# ---------------------------------------------------------------------------
+# Phong frame: primary + shadow shaded, one mirror bounce + its shadow added
+# ---------------------------------------------------------------------------
+def _phong_frame_gpu(sc, cam, light, params, diffuse):
+    import torch
+    n = cam.width * cam.height
+    rb, hb = grt.RayBatch.empty(n), grt.HitBatch.empty(n)
+    sb, fb, fhb = grt.RayBatch.empty(n), grt.RayBatch.empty(n), grt.HitBatch.empty(n)
+    s2 = grt.RayBatch.empty(n)
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    occ2 = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    rgb = torch.full((n, 3), -1.0, dtype=torch.float32, device="cuda:0")
+    rgba = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    kd = torch.from_numpy(diffuse).cuda() if diffuse is not None else None
+    sc.ctx.gen_primary(cam, rb)
+    sc.trace_closest(rb, hb, image_width=cam.width)
+    sc.gen_shadow(rb, hb, light, 1e-3, sb)
+    sc.trace_any(sb, occ)
+    sc.shade_phong(rb, hb, rgb, occ, 0, params, kd)
+    first = rgb.cpu().numpy().copy()
+    sc.gen_reflection(rb, hb, 1e-3, fb)
+    sc.trace_closest(fb, fhb)
+    sc.gen_shadow(fb, fhb, light, 1e-3, s2)
+    sc.trace_any(s2, occ2)
+    sc.shade_phong(fb, fhb, rgb, occ2, 1, params, kd, rgba)
+    return first, rgb.cpu().numpy(), rgba.cpu().numpy().view(np.uint32)
+
+
+def _phong_frame_oracle(fr, offs, refs, v, nrm, cam, light, params, diffuse):
+    rays = orc.primary_rays(grt.camera_dict(cam))
+    o = orc.trace_closest(fr, offs, refs, v, rays)
+    so = orc.gen_shadow(rays, o, nrm, light, 1e-3)
+    occ = orc.trace_any(fr, offs, refs, v, so)["occluded"]
+    first, _ = orc.shade_phong(rays, o, nrm, occ, params, 0, None, diffuse)
+    fo = orc.gen_reflection(rays, o, nrm, 1e-3)
+    fho = orc.trace_closest(fr, offs, refs, v, fo)
+    s2 = orc.gen_shadow(fo, fho, nrm, light, 1e-3)
+    occ2 = orc.trace_any(fr, offs, refs, v, s2)["occluded"]
+    rgb, img = orc.shade_phong(fo, fho, nrm, occ2, params, 1, first, diffuse)
+    return first, rgb, img, o, occ
+
+
+@pytest.mark.parametrize("colours", [False, True])
+def test_phong_frame_c1(gpu_ctx, c1_scene, colours):
+    v, nrm = c1_scene
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, nrm, 5.0)
+    light = (3.0, 4.0, -2.0)
+    params = {"light": light}
+    diffuse = None
+    if colours:
+        diffuse = rng(77).uniform(0.05, 1.0, size=(v.shape[0], 3)).astype(np.float32)
+        params.update({"spec_exp": 9, "ambient": (0.05, 0.02, 0.0), "shadow_coeff": 0.3})
+    cam = c1_camera(192, 160)
+    g_first, g_rgb, g_img = _phong_frame_gpu(sc, cam, light, grt.phong(params), diffuse)
+    o_first, o_rgb, o_img, o, occ = _phong_frame_oracle(fr, offs, refs, v, nrm, cam, light,
+                                                        params, diffuse)
+    assert_same(g_first, o_first, "primary Phong rgb")
+    assert_same(g_rgb, o_rgb, "rgb with the reflection added")
+    assert_same(g_img, o_img, "RGBA8 frame")
+    # the frame exercises every branch: misses, lit and shadowed hits, and
+    # reflections that add colour
+    hit = o["id"] >= 0
+    assert np.any(~hit) and np.any(hit & (occ == 1)) and np.any(hit & (occ == 0))
+    assert np.any(o_rgb[hit] != o_first[hit])
+    assert np.all(o_rgb[~hit] == 1.0)

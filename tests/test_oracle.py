@@ -426,3 +426,93 @@ def test_transform_is_rigid():
                            np.linalg.norm(tv[t].reshape(3, 3)[1] - tv[t].reshape(3, 3)[0]),
                            rtol=1e-4)
     assert np.array_equal(tv[-2:], v[-2:])          # ground plane untouched
+
+
+# ---------------------------------------------------------------------------
+# Phong shading (shading_FRAG.glsl:116-173): known answers on hand-built hits
+# ---------------------------------------------------------------------------
+def test_pow_u32_is_the_integer_power():
+    assert orc.pow_u32(2.0, 10) == 1024.0
+    assert orc.pow_u32(0.75, 1) == np.float32(0.75)
+    assert orc.pow_u32(0.3, 0) == 1.0
+    r = rng(11)
+    for _ in range(2000):
+        x = float(np.float32(r.uniform(0.5, 1.0)))   # x^e stays a normal float
+        e = int(r.integers(1, 100))
+        want = x ** e
+        # each of the <= 2 log2(e) roundings adds at most 2^-24 relative error,
+        # squared terms double theirs: 2 e ulps bound the whole ladder
+        assert abs(orc.pow_u32(x, e) - want) <= 2.0 * e * 2.0 ** -24 * want, (x, e)
+
+
+def _flat_scene():
+    """One triangle in the plane y = 0, vertex normals (0, 1, 0)."""
+    nrm = np.tile(np.float32([0, 1, 0]), 3).reshape(1, 9)
+    return nrm
+
+
+def _shade_one(o, d, light, kd=(0.2, 0.4, 0.6), occluded=0, spec_exp=50, t=1.0, depth=0,
+               rgb=None, hit=True):
+    rays = {k: np.float32([x]) for k, x in zip(("ox", "oy", "oz", "dx", "dy", "dz"), (*o, *d))}
+    hits = {"id": np.int32([0 if hit else -1]), "t": np.float32([t]),
+            "u": np.float32([0.25]), "v": np.float32([0.25])}
+    params = {"light": light, "spec_exp": spec_exp}
+    rgb, img = orc.shade_phong(rays, hits, _flat_scene(), np.uint8([occluded]), params, depth,
+                               rgb, np.float32([kd]))
+    return rgb[0], img[0]
+
+
+F = np.float32
+
+
+def test_phong_head_on_light_gives_diffuse_plus_full_specular():
+    # view straight down, light straight up: N.L = 1 and the mirror vector is L
+    c, img = _shade_one((0.25, 1, 0.25), (0, -1, 0), (0.25, 11, 0.25))
+    assert list(c) == [F(0.2) + F(0.5), F(0.4) + F(0.5), F(0.6) + F(0.5)]
+    # RGBA8 truncates c * 255 and clamps: 0.7 -> 178, 0.9 -> 229, 1.1 -> 255
+    assert img == 178 | (229 << 8) | (255 << 16) | (255 << 24)
+    # without the specular term only the diffuse colour is left
+    c, _ = _shade_one((0.25, 1, 0.25), (0, -1, 0), (0.25, 11, 0.25), spec_exp=0)
+    assert list(c) == [F(0.2), F(0.4), F(0.6)]
+
+
+def test_phong_shadow_dims_diffuse_and_drops_specular():
+    c, _ = _shade_one((0.25, 1, 0.25), (0, -1, 0), (0.25, 11, 0.25), occluded=1)
+    assert list(c) == [F(0.2) * F(0.2), F(0.4) * F(0.2), F(0.6) * F(0.2)]
+    # a light below the surface shadows the point: N.L < 0, diffuse clamps to 0
+    c, _ = _shade_one((0.25, 1, 0.25), (0, -1, 0), (0.25, -10, 0.25))
+    assert list(c) == [0.0, 0.0, 0.0]
+
+
+def test_phong_specular_follows_the_mirror_direction():
+    # 45-degree view from -x: the mirror direction is (1, 1, 0)/sqrt 2; a light
+    # along it gives the full specular, N.L = 1/sqrt 2 scales the diffuse
+    c, _ = _shade_one((-0.75, 1, 0.25), (1, -1, 0), (10.25, 10, 0.25))
+    s = 1.0 / math.sqrt(2.0)
+    for k, kd in enumerate((0.2, 0.4, 0.6)):
+        assert abs(c[k] - (kd * s + 0.5)) < 1e-6
+    # the light on the other side (along the incoming ray): no specular
+    c, _ = _shade_one((-0.75, 1, 0.25), (1, -1, 0), (-9.75, 10, 0.25))
+    for k, kd in enumerate((0.2, 0.4, 0.6)):
+        assert abs(c[k] - kd * s) < 1e-6
+    # a light 45 degrees off the mirror direction: (cos 45)^50 = 2^-25
+    c, _ = _shade_one((0.25, 1, 0.25), (0, -1, 0), (10.25, 10, 0.25), kd=(0, 0, 0))
+    assert all(abs(x / (0.5 * 2.0 ** -25) - 1.0) < 1e-4 for x in c)
+
+
+def test_phong_normal_faces_the_ray():
+    # seen from below, the surface normal is turned to (0, -1, 0)
+    c, _ = _shade_one((0.25, -1, 0.25), (0, 1, 0), (0.25, -11, 0.25))
+    assert list(c) == [F(0.2) + F(0.5), F(0.4) + F(0.5), F(0.6) + F(0.5)]
+
+
+def test_phong_misses_and_reflection_accumulation():
+    c, img = _shade_one((0, 0, 0), (0, 0, 1), (0, 10, 0), hit=False)
+    assert list(c) == [1.0, 1.0, 1.0] and img == 0xffffffff
+    base = np.float32([[0.1, 0.2, 0.3]])
+    c, _ = _shade_one((0, 0, 0), (0, 0, 1), (0, 10, 0), hit=False, depth=1, rgb=base)
+    assert list(c) == list(base[0])
+    col, _ = _shade_one((0.25, 1, 0.25), (0, -1, 0), (0.25, 11, 0.25))
+    for depth, w in ((1, F(0.4)), (2, F(0.4) * F(0.4))):
+        c, _ = _shade_one((0.25, 1, 0.25), (0, -1, 0), (0.25, 11, 0.25), depth=depth, rgb=base)
+        assert list(c) == [base[0][k] + w * col[k] for k in range(3)]
