@@ -9,7 +9,7 @@
 //   shading normal      Sample.cpp:100-114, RayTracing_kernel.cu:92-102
 //   headlight shading   RayTracing_kernel.cu:104-120
 //   Phong shading       shading_FRAG.glsl:116-173, PhongMaterialColor.cpp:19-85,
-//                       reflection blend GpuRenderer.cpp:236-297
+//                       reflection blend GpuRenderer.cpp:233-297
 // Every kernel is one output ray (or pixel) per lane, structure-of-arrays in
 // and out, and follows the CPU oracle's operation order bit for bit.
 #include "grt_launch.h"

@@ -229,8 +229,9 @@ def _time(fn, reps, torch):
 
 def batch_config(reps: int = 5):
     """Batched (one launch per level, hipGraph) vs plain; the autotuner picks
-    the schedule (one ray per lane or persistent), lambda in {2,5,8}, block
-    size and refill threshold."""
+    the schedule (one ray per lane, pooled or persistent), lambda in {2,5,8},
+    block size and refill threshold.  The plain path gets its best of the
+    single-scene schedules, the split one (4 lanes per ray) included."""
     torch = _torch()
     ctx = Context(0)
     stream = stream_create()
@@ -277,10 +278,11 @@ def batch_config(reps: int = 5):
 
     t_graph = _time(replay, reps, torch)
     t_eager = _time(lambda: (bj.run_batched(), ctx.synchronize()), reps, torch)
-    # the plain path at the same lambda, with its better schedule
+    # the plain path at the same lambda, with its best schedule
     t_plain_by = {}
-    for mode in ("simple", "pooled", "persistent"):
-        ctx.set_trace_mode(mode, best["refill"])
+    for mode, param in (("simple", best["refill"]), ("pooled", best["refill"]),
+                        ("persistent", best["refill"]), ("split", 4)):
+        ctx.set_trace_mode(mode, param)
         t_plain_by[mode] = _time(lambda: (bj.run_plain(), ctx.synchronize()), reps, torch)
     t_plain = min(t_plain_by.values())
     ctx.set_trace_mode(best["mode"], best["refill"])

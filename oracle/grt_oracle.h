@@ -25,7 +25,7 @@
  *                      shaders/traversalIntersection_FRAG.glsl:285-296
  *   - Phong shading:   shaders/shading_FRAG.glsl:8-36,116-173,
  *                      src/rtplugins/PhongMaterialColor.cpp:19-85,
- *                      src/rtgl/GpuRenderer.cpp:236-297 (reflection blend)
+ *                      src/rtgl/GpuRenderer.cpp:233-297 (reflection blend)
  *
  * Floating-point evaluation order is fixed: every fused multiply-add is an
  * explicit fmaf() call, everything else is a single IEEE-754 binary32

@@ -516,3 +516,19 @@ def test_phong_misses_and_reflection_accumulation():
     for depth, w in ((1, F(0.4)), (2, F(0.4) * F(0.4))):
         c, _ = _shade_one((0.25, 1, 0.25), (0, -1, 0), (0.25, 11, 0.25), depth=depth, rgb=base)
         assert list(c) == [base[0][k] + w * col[k] for k in range(3)]
+
+
+def test_headlight_known_answers():
+    """RayTracing_kernel.cu:104-120: colour = (ambient + (diffuse - ambient) *
+    nDotD) * diffuse with ambient 0.1, diffuse (0, 0, 1), nDotD = -N.d/|d|,
+    packed r | g << 8 | b << 16 | 255 << 24 from c * 255 truncated."""
+    facing = np.tile(np.float32([0, 0, -1]), 3)
+    # head-on: nDotD = 1 -> blue 1.0
+    assert orc.headlight((0, 0, 2.0), facing, 0, 0.3, 0.3) == 0xFFFF0000
+    # 60 degrees off the normal: nDotD = 0.5 -> blue 0.55 -> 140
+    d = (0.0, math.sqrt(3.0) / 2.0, 0.5)
+    assert orc.headlight(d, facing, 0, 0.3, 0.3) == 0xFF000000 | (140 << 16)
+    # seen from behind: nDotD = -1 -> blue clamps to 0, red and green stay 0
+    assert orc.headlight((0, 0, -1.0), facing, 0, 0.3, 0.3) == 0xFF000000
+    # a miss is white
+    assert orc.headlight((0, 0, 1.0), facing, -1, 0.0, 0.0) == 0xFFFFFFFF
