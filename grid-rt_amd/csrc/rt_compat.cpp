@@ -444,6 +444,108 @@ void HipRenderer::readHits(std::vector<int32_t>& id, std::vector<float>& t,
 }
 
 // ---------------------------------------------------------------------------
+// PrimitiveBuilder
+// ---------------------------------------------------------------------------
+void PrimitiveBuilder::begin(int primitiveType, Geometry* geometry)   // PrimitiveBuilder.cpp:18-23
+{
+    if (!geometry)
+        throw Error(GRT_INVALID_PARAM, "PrimitiveBuilder::begin: NULL geometry");
+    if (primitiveType != RT_TRIANGLES && primitiveType != RT_TRIANGLE_STRIP &&
+        primitiveType != RT_TRIANGLE_FAN && primitiveType != RT_POLYGON)
+        throw Error(GRT_INVALID_PARAM, "PrimitiveBuilder::begin: unknown primitive type");
+    type_ = primitiveType;
+    geometry_ = geometry;
+    start_ = geometry->vertices.size();
+}
+
+void PrimitiveBuilder::setMatrix(const float xf[12])
+{
+    static const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    const float* a = xf ? xf : id;
+    double m[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            m[r][c] = a[4 * r + c];
+    double cof[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            cof[r][c] = m[r1][c1] * m[r2][c2] - m[r1][c2] * m[r2][c1];
+        }
+    const double det = m[0][0] * cof[0][0] + m[0][1] * cof[0][1] + m[0][2] * cof[0][2];
+    if (!(det != 0.0) || !std::isfinite(det))
+        throw Error(GRT_INVALID_PARAM, "PrimitiveBuilder::setMatrix: singular matrix");
+    std::memcpy(xf_, a, sizeof(xf_));
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            nxf_[3 * r + c] = (float)(cof[r][c] / det);   // inverse transpose
+}
+
+void PrimitiveBuilder::setNormal(float x, float y, float z)
+{
+    normal_.set(x, y, z);
+}
+
+void PrimitiveBuilder::addVertex(float x, float y, float z)   // PrimitiveBuilder.cpp:71-88
+{
+    if (!geometry_)
+        throw Error(GRT_INVALID_PARAM, "PrimitiveBuilder::addVertex: no primitive begun");
+    const float* a = xf_;
+    geometry_->vertices.emplace_back(((a[0] * x + a[1] * y) + a[2] * z) + a[3],
+                                     ((a[4] * x + a[5] * y) + a[6] * z) + a[7],
+                                     ((a[8] * x + a[9] * y) + a[10] * z) + a[11]);
+    const float* q = nxf_;
+    const vec3f& n = normal_;
+    float t[3] = {(q[0] * n.x + q[1] * n.y) + q[2] * n.z, (q[3] * n.x + q[4] * n.y) + q[5] * n.z,
+                  (q[6] * n.x + q[7] * n.y) + q[8] * n.z};
+    const float len2 = (t[0] * t[0] + t[1] * t[1]) + t[2] * t[2];
+    if (len2 > 0.0f) {
+        const float inv = 1.0f / std::sqrt(len2);
+        t[0] = t[0] * inv;
+        t[1] = t[1] * inv;
+        t[2] = t[2] * inv;
+    }
+    geometry_->normals.emplace_back(t[0], t[1], t[2]);
+}
+
+void PrimitiveBuilder::store(uint32_t a, uint32_t b, uint32_t c)   // storeValidTriangle
+{
+    const std::vector<vec3f>& v = geometry_->vertices;
+    const float tri[9] = {v[a].x, v[a].y, v[a].z, v[b].x, v[b].y, v[b].z, v[c].x, v[c].y, v[c].z};
+    if (!grt_triangle_valid(tri))
+        return;
+    TriDesc d;
+    d.v[0] = a;
+    d.v[1] = b;
+    d.v[2] = c;
+    geometry_->triDesc.push_back(d);
+}
+
+void PrimitiveBuilder::end()   // PrimitiveBuilder.cpp:96-219
+{
+    if (!geometry_)
+        throw Error(GRT_INVALID_PARAM, "PrimitiveBuilder::end: no primitive begun");
+    const size_t n = geometry_->vertices.size() - start_;
+    const uint32_t s = (uint32_t)start_;
+    if (type_ == RT_TRIANGLES) {
+        for (size_t i = 0; i + 2 < n; i += 3)
+            store(s + (uint32_t)i, s + (uint32_t)i + 1, s + (uint32_t)i + 2);
+    } else if (type_ == RT_TRIANGLE_STRIP) {
+        for (size_t i = 0; i + 2 < n; ++i) {
+            const uint32_t k = s + (uint32_t)i;
+            if (i & 1)
+                store(k + 2, k + 1, k);
+            else
+                store(k, k + 1, k + 2);
+        }
+    } else {
+        for (size_t i = 1; i + 1 < n; ++i)
+            store(s, s + (uint32_t)i, s + (uint32_t)i + 1);
+    }
+    geometry_ = nullptr;
+}
+
+// ---------------------------------------------------------------------------
 // HipGridRenderer
 // ---------------------------------------------------------------------------
 HipGridRenderer::HipGridRenderer(grt_context ctx, PinholeCamera* camera, Geometry* geometry,

@@ -110,6 +110,22 @@ _SIGS = {
     "grt_shade_headlight": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _U32,
                                            _P]),
     "grt_phong_defaults": (None, [ctypes.POINTER(Phong)]),
+    "grt_mesh_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "grt_mesh_destroy": (ctypes.c_int, [_P]),
+    "grt_mesh_clear": (ctypes.c_int, [_P]),
+    "grt_mesh_set_transform": (ctypes.c_int, [_P, _P]),
+    "grt_mesh_set_color": (ctypes.c_int, [_P, _F, _F, _F]),
+    "grt_mesh_begin": (ctypes.c_int, [_P, ctypes.c_int]),
+    "grt_mesh_normal": (ctypes.c_int, [_P, _F, _F, _F]),
+    "grt_mesh_vertex": (ctypes.c_int, [_P, _F, _F, _F]),
+    "grt_mesh_end": (ctypes.c_int, [_P]),
+    "grt_mesh_load_obj": (ctypes.c_int, [_P, ctypes.c_char_p, _U32]),
+    "grt_mesh_add_sphere": (ctypes.c_int, [_P, _U32, _U32]),
+    "grt_mesh_add_cube": (ctypes.c_int, [_P]),
+    "grt_mesh_info": (ctypes.c_int, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "grt_mesh_copy": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "grt_scene_upload_mesh": (ctypes.c_int, [_P, _P]),
+    "grt_triangle_valid": (ctypes.c_int, [_P]),
     "grt_shade_phong": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _P, _U32,
                                        _P, ctypes.POINTER(Phong), _U32, _P, _P]),
     "grt_debug_scan": (ctypes.c_int, [_P, _P, _P, _U32]),
@@ -381,6 +397,86 @@ class Context:
             self.h = None
 
 
+TRIANGLES, TRIANGLE_STRIP, TRIANGLE_FAN, POLYGON = 0x1001, 0x1002, 0x1003, 0x1004
+MESH_DROP_DEGENERATE = 1
+
+
+class Mesh:
+    """Host-side triangle soup (grt_mesh_*): immediate-mode primitives with the
+    reference's assembly and degenerate filter, OBJ/MTL loading, procedural
+    sphere and cube.  Needs no GPU."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().grt_mesh_create(ctypes.byref(h)))
+        self.h = h
+
+    def set_transform(self, xf=None):
+        a = None if xf is None else np.ascontiguousarray(xf, np.float32).reshape(12)
+        check(lib().grt_mesh_set_transform(self.h, _ptr(a) if a is not None else None))
+        self._xf = a   # keep alive
+
+    def set_color(self, r, g, b):
+        check(lib().grt_mesh_set_color(self.h, r, g, b))
+
+    def begin(self, primitive):
+        check(lib().grt_mesh_begin(self.h, primitive))
+
+    def normal(self, x, y, z):
+        check(lib().grt_mesh_normal(self.h, x, y, z))
+
+    def vertex(self, x, y, z):
+        check(lib().grt_mesh_vertex(self.h, x, y, z))
+
+    def end(self):
+        check(lib().grt_mesh_end(self.h))
+
+    def primitive(self, primitive, vertices, normals=None):
+        """One primitive from arrays of vertices (k x 3) and optional normals."""
+        self.begin(primitive)
+        for k, p in enumerate(np.asarray(vertices, np.float32).reshape(-1, 3)):
+            if normals is not None:
+                self.normal(*map(float, np.asarray(normals, np.float32).reshape(-1, 3)[k]))
+            self.vertex(*map(float, p))
+        self.end()
+
+    def load_obj(self, path, drop_degenerate: bool = False):
+        check(lib().grt_mesh_load_obj(self.h, str(path).encode(),
+                                      MESH_DROP_DEGENERATE if drop_degenerate else 0))
+
+    def add_sphere(self, slices: int, stacks: int):
+        check(lib().grt_mesh_add_sphere(self.h, slices, stacks))
+
+    def add_cube(self):
+        check(lib().grt_mesh_add_cube(self.h))
+
+    def info(self):
+        n, d = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().grt_mesh_info(self.h, ctypes.byref(n), ctypes.byref(d)))
+        return n.value, d.value
+
+    def arrays(self):
+        """(vertices [n, 9], normals [n, 9], diffuse [n, 3], source index [n])."""
+        n, _ = self.info()
+        v = np.empty((n, 9), np.float32)
+        nr = np.empty((n, 9), np.float32)
+        kd = np.empty((n, 3), np.float32)
+        src = np.empty(n, np.uint32)
+        check(lib().grt_mesh_copy(self.h, _ptr(v), _ptr(nr), _ptr(kd), _ptr(src)))
+        return v, nr, kd, src
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().grt_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Scene:
     def __init__(self, ctx: Context):
         self.ctx = ctx
@@ -396,6 +492,10 @@ class Scene:
         if normals is not None:
             normals = np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
             check(lib().grt_scene_upload_normals(self.h, _ptr(normals), normals.shape[0]))
+
+    def upload_mesh(self, mesh: "Mesh"):
+        check(lib().grt_scene_upload_mesh(self.h, mesh.h))
+        self.n_tris = mesh.info()[0]
 
     def transform_clusters(self, xf: np.ndarray, tris_per_cluster: int, n_cluster_tris: int):
         xf = np.ascontiguousarray(xf, np.float32).reshape(-1, 12)

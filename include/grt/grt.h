@@ -132,6 +132,57 @@ int grt_scene_download_geometry(grt_scene scene, float* verts_host, float* norma
 int grt_scene_transform_clusters(grt_scene scene, const float* xf_host, uint32_t n_clusters,
                                  uint32_t tris_per_cluster, uint32_t n_cluster_tris);
 
+/* ---- scene ingestion (host) -------------------------------------------- */
+/* A host-side triangle soup built the way the reference's rt::Context fed
+ * its PrimitiveBuilder (Context.cpp:285-343, PrimitiveBuilder.cpp:71-219):
+ * primitives of vertices with a current normal and colour, through a current
+ * affine transform.  Assembly: GRT_TRIANGLES takes triples; GRT_TRIANGLE_STRIP
+ * takes (i, i+1, i+2) for even i and (i+2, i+1, i) for odd i;
+ * GRT_TRIANGLE_FAN and GRT_POLYGON take (0, i, i+1).  A triangle is kept when
+ * the nine constants of its Wald accelerator (TriAccel::buildFrom/valid,
+ * Triangle.cpp:11-63) are finite, so zero-area and non-finite triangles are
+ * dropped and the kept ones get consecutive ids.  The values are those of the
+ * reference's RTenum (include/rt/common.h:29-32). */
+typedef struct grt_mesh_t* grt_mesh;
+#define GRT_TRIANGLES      0x1001
+#define GRT_TRIANGLE_STRIP 0x1002
+#define GRT_TRIANGLE_FAN   0x1003
+#define GRT_POLYGON        0x1004
+#define GRT_MESH_DROP_DEGENERATE 1u
+int grt_mesh_create(grt_mesh* out);
+int grt_mesh_destroy(grt_mesh mesh);
+int grt_mesh_clear(grt_mesh mesh);
+/* Row-major 3x4 affine matrix applied to the following vertices; normals take
+ * the inverse transpose of its 3x3 part and are renormalised (NULL: identity). */
+int grt_mesh_set_transform(grt_mesh mesh, const float xf[12]);
+/* Diffuse colour of the following triangles (the material's). */
+int grt_mesh_set_color(grt_mesh mesh, float r, float g, float b);
+int grt_mesh_begin(grt_mesh mesh, int primitive);
+int grt_mesh_normal(grt_mesh mesh, float x, float y, float z);
+int grt_mesh_vertex(grt_mesh mesh, float x, float y, float z);
+int grt_mesh_end(grt_mesh mesh);
+/* Wavefront OBJ (GpuObjLoader.cpp:138-151): v, vn, f with 1-based or negative
+ * indices; a face of k corners becomes the fan (c0, ci, ci+1); a corner
+ * without a normal gets (0,0,0); mtllib/usemtl give the triangles the
+ * material's Kd (white without one).  The transform is not applied.  With
+ * GRT_MESH_DROP_DEGENERATE the faces go through the degenerate filter too. */
+int grt_mesh_load_obj(grt_mesh mesh, const char* path, uint32_t flags);
+/* rtdb::loadSphere (rtdb.cpp:142-192) and the unit cube of rtdb::loadCube,
+ * through the current transform. */
+int grt_mesh_add_sphere(grt_mesh mesh, uint32_t slices, uint32_t stacks);
+int grt_mesh_add_cube(grt_mesh mesh);
+/* The degenerate filter on one triangle (9 floats): 1 kept, 0 dropped. */
+int grt_triangle_valid(const float v9[9]);
+/* Kept triangles, and triangles assembled but dropped as degenerate. */
+int grt_mesh_info(grt_mesh mesh, uint32_t* n_tris, uint32_t* n_dropped);
+/* Copy out 9 floats of vertices, 9 of normals and 3 of diffuse colour per
+ * kept triangle, and each one's index among all assembled triangles (any
+ * pointer may be NULL). */
+int grt_mesh_copy(grt_mesh mesh, float* verts9, float* normals9, float* diffuse3,
+                  uint32_t* source_index);
+/* grt_scene_upload_triangles + grt_scene_upload_normals of the mesh. */
+int grt_scene_upload_mesh(grt_scene scene, grt_mesh mesh);
+
 /* ---- grid ------------------------------------------------------------- */
 /* Device build of the uniform grid (UniformGridAccStructBuilder::buildGeometry,
  * UniformGridAccStructBuilder.cpp:58-149; GpuGrid::rebuildGrid call site

@@ -335,6 +335,40 @@ private:
     uint32_t* frame_ = nullptr;
 };
 
+// rt::PrimitiveBuilder (PrimitiveBuilder.cpp:5-219), as rt::Context drove it
+// (beginPrimitive / setNormal / addVertex / endPrimitive, Context.cpp:285-343):
+// addVertex appends the vertex through the current matrix, and the current
+// normal through its inverse transpose, normalised, to the geometry; end()
+// assembles the primitive's triangles into triDesc (triangles; strips with
+// every other triangle reversed; fans and polygons around the first vertex)
+// and keeps those that pass the degenerate filter (grt_triangle_valid), so
+// the kept triangles get consecutive ids.
+enum : int {
+    RT_TRIANGLES = GRT_TRIANGLES,
+    RT_TRIANGLE_STRIP = GRT_TRIANGLE_STRIP,
+    RT_TRIANGLE_FAN = GRT_TRIANGLE_FAN,
+    RT_POLYGON = GRT_POLYGON
+};
+
+class PrimitiveBuilder {
+public:
+    void begin(int primitiveType, Geometry* geometry);
+    // Row-major 3x4 affine matrix (Transform::setMatrix); NULL: identity.
+    void setMatrix(const float xf[12]);
+    void setNormal(float x, float y, float z);
+    void addVertex(float x, float y, float z);
+    void end();
+
+private:
+    void store(uint32_t a, uint32_t b, uint32_t c);
+    int type_ = RT_TRIANGLES;
+    Geometry* geometry_ = nullptr;
+    size_t start_ = 0;
+    float xf_[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    float nxf_[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    vec3f normal_{0.0f, 0.0f, 1.0f};
+};
+
 // rtgl::GpuRenderer (GpuRenderer.h:18-71, GpuRenderer.cpp:43-297,663-703):
 // render() rebuilds the grid when needed (every frame with animation
 // enabled, as the reference did after each keyframe upload), traces one

@@ -438,3 +438,36 @@ def test_phong_frame_c1(gpu_ctx, c1_scene, colours):
     assert np.any(~hit) and np.any(hit & (occ == 1)) and np.any(hit & (occ == 0))
     assert np.any(o_rgb[hit] != o_first[hit])
     assert np.all(o_rgb[~hit] == 1.0)
+
+
+def test_mesh_scene_parity(gpu_ctx, tmp_path):
+    """A scene assembled on the host (procedural spheres and cubes through
+    transforms, plus an OBJ quad with a degenerate face) goes through
+    grt_scene_upload_mesh, the device grid build and the traversal, against
+    the oracle on the mesh's own arrays."""
+    m = grt.Mesh()
+    for k in range(5):
+        s = 0.05 + 0.02 * k
+        m.set_transform([s, 0, 0, 0.2 * k, 0, s, 0, 0.5, 0, 0, s, 0.3 * (k % 2)])
+        m.add_sphere(24, 20)
+        m.set_transform([s, 0, 0, 0.2 * k, 0, 2 * s, 0, 0.2, 0, 0, s, 0.6])
+        m.add_cube()
+    (tmp_path / "q.obj").write_text("v -1 -0.2 -1\nv 2 -0.2 -1\nv 2 -0.2 2\nv -1 -0.2 2\n"
+                                    "vn 0 1 0\nf 1//1 2//1 3//1 4//1\nf 1 1 2\n")
+    m.load_obj(tmp_path / "q.obj", drop_degenerate=True)
+    v, nrm, _, _ = m.arrays()
+    sc = grt.Scene(gpu_ctx)
+    sc.upload_mesh(m)
+    sc.build_grid(5.0)
+    fr = orc.grid_frame(v, 5.0)
+    _, offs, refs = orc.grid_build(v, fr)
+    assert_frame(sc.grid_info(), fr)
+    _, go, gr = sc.download_grid()
+    assert_same(go, offs, "mesh grid offsets")
+    assert_same(gr, refs, "mesh grid refs")
+    cam = grt.camera_lookat((0.4, 0.8, -2.0), (0.4, 0.4, 0.3), (0, 1, 0), 50.0, 160, 120)
+    rays = orc.primary_rays(grt.camera_dict(cam))
+    g, o = check_closest(sc, fr, offs, refs, v, rays, image_width=cam.width)
+    assert np.count_nonzero(o["id"] >= 0) > rays["ox"].shape[0] // 4
+    sh = orc.gen_shadow(rays, o, nrm, (5.0, 9.0, -4.0), 1e-3)
+    check_any(sc, fr, offs, refs, v, sh)
