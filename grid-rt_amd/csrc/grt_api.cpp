@@ -259,6 +259,7 @@ int grt_shutdown(grt_context ctx)
     (void)hipStreamSynchronize(ctx->stream);
     for (grt_scene_t* sc : std::vector<grt_scene_t*>(ctx->scenes.begin(), ctx->scenes.end()))
         grt_scene_destroy(sc);
+    ctx->instance_scratch.release(ctx);
     if (ctx->own_stream)
         (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -912,6 +913,79 @@ int grt_shade_headlight(grt_scene sc, const grt_rays* rays, const grt_hits* hits
     return hip_check(launch_shade_headlight(n, to_soa(rays), to_hits_in(hits, sc->n_tris),
                                             sc->normals.as<float>(), rgba_dev, sc->ctx->stream),
                      "shade_headlight");
+}
+
+int grt_affine_inverse(const float xf[12], float inv[12])
+{
+    if (!xf || !inv)
+        return set_error(GRT_INVALID_PARAM, "grt_affine_inverse: NULL argument");
+    double m[3][3], c[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k)
+            m[r][k] = xf[4 * r + k];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) {
+            const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+            c[r][k] = m[r1][k1] * m[r2][k2] - m[r1][k2] * m[r2][k1];
+        }
+    const double det = m[0][0] * c[0][0] + m[0][1] * c[0][1] + m[0][2] * c[0][2];
+    if (!(det != 0.0) || !std::isfinite(det))
+        return set_error(GRT_INVALID_PARAM, "grt_affine_inverse: singular matrix");
+    // inverse = transpose(cofactors) / det; translation = -inverse * t
+    double a[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k)
+            a[r][k] = c[k][r] / det;
+    for (int r = 0; r < 3; ++r) {
+        for (int k = 0; k < 3; ++k)
+            inv[4 * r + k] = (float)a[r][k];
+        inv[4 * r + 3] = (float)(-(a[r][0] * xf[3] + a[r][1] * xf[7] + a[r][2] * xf[11]));
+    }
+    return GRT_OK;
+}
+
+int grt_trace_instances(grt_context ctx, const grt_instance* instances, uint32_t n_instances,
+                        const grt_rays* rays, const grt_hits* hits, int32_t* instance_dev,
+                        uint32_t n, uint32_t image_width)
+{
+    if (!ctx || (!instances && n_instances) || !rays_ok(rays) || !hits_ok(hits) || !instance_dev)
+        return set_error(GRT_INVALID_PARAM, "grt_trace_instances: bad argument");
+    std::vector<Affine> inv(n_instances);
+    for (uint32_t k = 0; k < n_instances; ++k) {
+        grt_scene sc = instances[k].scene;
+        int rc = trace_ready(sc, "grt_trace_instances");
+        if (rc != GRT_OK)
+            return rc;
+        if (sc->ctx != ctx)
+            return set_error(GRT_INVALID_PARAM, "grt_trace_instances: instance %u of another context", k);
+        if ((rc = grt_affine_inverse(instances[k].xf, inv[k].m)) != GRT_OK)
+            return rc;
+    }
+    hipStream_t s = ctx->stream;
+    const HitSoA h{hits->id, hits->t, hits->u, hits->v};
+    int rc = hip_check(launch_instance_init(n, h, instance_dev, s), "instance init");
+    if (rc != GRT_OK || n == 0 || n_instances == 0)
+        return rc;
+    if ((rc = ctx->instance_scratch.reserve(ctx, (size_t)n * 48, false)) != GRT_OK)
+        return rc;
+    float* f = ctx->instance_scratch.as<float>();
+    const size_t N = n;
+    const grt_rays local = {f, f + N, f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N, f + 6 * N, f + 7 * N};
+    const grt_hits cand = {reinterpret_cast<int32_t*>(f + 8 * N), f + 9 * N, f + 10 * N, f + 11 * N};
+    for (uint32_t k = 0; k < n_instances; ++k) {
+        grt_scene sc = instances[k].scene;
+        if ((rc = hip_check(launch_instance_rays(n, to_soa(rays), hits->t, inv[k], to_soa_out(&local),
+                                                 s),
+                            "instance rays")) != GRT_OK)
+            return rc;
+        if ((rc = grt_trace_closest(sc, &local, &cand, n, image_width, nullptr)) != GRT_OK)
+            return rc;
+        if ((rc = hip_check(launch_instance_merge(n, to_hits_in(&cand, sc->n_tris), h, instance_dev,
+                                                  (int32_t)k, s),
+                            "instance merge")) != GRT_OK)
+            return rc;
+    }
+    return GRT_OK;
 }
 
 void grt_phong_defaults(grt_phong* p)

@@ -55,6 +55,7 @@ struct grt_context_t {
     uint32_t split_lanes = 4;   // GRT_TRACE_SPLIT: lanes per ray
     uint32_t coop = 16;   // cooperative cell tests in drained persistent waves (0: off)
     std::set<grt_scene_t*> scenes;
+    grt::DevMem instance_scratch;   // object-space rays and candidate hits of instance tracing
 };
 
 struct grt_scene_t {

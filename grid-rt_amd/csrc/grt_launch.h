@@ -149,6 +149,18 @@ hipError_t launch_gen_ao(uint32_t n, uint32_t spp, uint64_t seed, const RaySoA& 
                          float ao_tmax, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_count_active(const float* tmin, const float* tmax, uint32_t n,
                                unsigned long long* count, hipStream_t s);
+// ---- instance-level tracing (grt_instance.hip) --------------------------------
+// Row-major 3x4 affine matrix.
+struct Affine {
+    float m[12];
+};
+hipError_t launch_instance_init(uint32_t n, const HitSoA& hits, int32_t* instance, hipStream_t s);
+// Object-space rays of one instance, intervals cut below best_t.
+hipError_t launch_instance_rays(uint32_t n, const RaySoA& in, const float* best_t,
+                                const Affine& inv, const RaySoAOut& out, hipStream_t s);
+hipError_t launch_instance_merge(uint32_t n, const HitSoAIn& cand, const HitSoA& hits,
+                                 int32_t* instance, int32_t k, hipStream_t s);
+
 // Phong parameters (grt_phong, shading_FRAG.glsl:8-36).
 struct DevPhong {
     float light[3];

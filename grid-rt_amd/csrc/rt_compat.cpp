@@ -270,8 +270,29 @@ bool HipGridAccStruct::traceAnyGeometry(const Instance&, Ray& ray, Hit& hit)
     return found;
 }
 
-// IAccStruct.cpp:26-64: rays start at tnear = 2e-4 with tfar = +inf and every
-// instance's geometry is traced (instances carry no transform here).
+// The ray in an instance's object space (Transform::inverseTransform,
+// Transform.h:31-40): the origin through the inverse matrix, the direction
+// through its 3x3 part, not renormalised, so t is shared with world space.
+static Ray object_ray(const Instance& inst, const Ray& world)
+{
+    float a[12];
+    check(grt_affine_inverse(inst.transform, a));
+    const vec3f& o = world.orig;
+    const vec3f& d = world.dir;
+    Ray r = world;
+    r.orig.set(((a[0] * o.x + a[1] * o.y) + a[2] * o.z) + a[3],
+               ((a[4] * o.x + a[5] * o.y) + a[6] * o.z) + a[7],
+               ((a[8] * o.x + a[9] * o.y) + a[10] * o.z) + a[11]);
+    r.dir.set((a[0] * d.x + a[1] * d.y) + a[2] * d.z, (a[4] * d.x + a[5] * d.y) + a[6] * d.z,
+              (a[8] * d.x + a[9] * d.y) + a[10] * d.z);
+    r.update();
+    return r;
+}
+
+// IAccStruct.cpp:26-73: rays start at tnear = 2e-4 with tfar = +inf; each
+// instance's geometry is traced with the ray in its object space and keeps
+// the earlier hit unless it finds a strictly nearer one.  Batches of rays go
+// through grt_trace_instances instead.
 void HipGridAccStruct::traceNearestInstance(const std::vector<Instance>& instances,
                                             Sample& sample)
 {
@@ -283,7 +304,8 @@ void HipGridAccStruct::traceNearestInstance(const std::vector<Instance>& instanc
     for (const Instance& inst : instances) {
         if (!inst.geometry || !inst.geometry->accStruct)
             throw Error(GRT_NOT_READY, "traceNearestInstance: instance without a built geometry");
-        inst.geometry->accStruct->traceNearestGeometry(inst, sample.ray, sample.hit);
+        Ray local = object_ray(inst, sample.ray);
+        inst.geometry->accStruct->traceNearestGeometry(inst, local, sample.hit);
     }
 }
 
@@ -292,7 +314,8 @@ bool HipGridAccStruct::traceAnyInstance(const std::vector<Instance>& instances, 
     for (const Instance& inst : instances) {
         if (!inst.geometry || !inst.geometry->accStruct)
             throw Error(GRT_NOT_READY, "traceAnyInstance: instance without a built geometry");
-        if (inst.geometry->accStruct->traceAnyGeometry(inst, sample.ray, sample.hit))
+        Ray local = object_ray(inst, sample.ray);
+        if (inst.geometry->accStruct->traceAnyGeometry(inst, local, sample.hit))
             return true;
     }
     return false;

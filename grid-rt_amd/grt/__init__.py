@@ -56,6 +56,19 @@ class GridInfo(ctypes.Structure):
                 ("n", ctypes.c_int32 * 3), ("m", ctypes.c_uint32), ("r", ctypes.c_uint32)]
 
 
+class Instance(ctypes.Structure):
+    """grt_instance: a scene and its row-major 3x4 object-to-world matrix."""
+    _fields_ = [("scene", ctypes.c_void_p), ("xf", ctypes.c_float * 12)]
+
+
+def affine_inverse(xf) -> np.ndarray:
+    """The binary32 inverse grt_trace_instances takes rays through."""
+    a = np.ascontiguousarray(xf, np.float32).reshape(12)
+    out = np.empty(12, np.float32)
+    check(lib().grt_affine_inverse(_ptr(a), _ptr(out)))
+    return out
+
+
 class Phong(ctypes.Structure):
     """grt_phong: Phong shading parameters (shading_FRAG.glsl:8-36)."""
     _fields_ = [("light", ctypes.c_float * 3), ("light_color", ctypes.c_float * 3),
@@ -126,6 +139,9 @@ _SIGS = {
     "grt_mesh_copy": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "grt_scene_upload_mesh": (ctypes.c_int, [_P, _P]),
     "grt_triangle_valid": (ctypes.c_int, [_P]),
+    "grt_affine_inverse": (ctypes.c_int, [_P, _P]),
+    "grt_trace_instances": (ctypes.c_int, [_P, _P, _U32, ctypes.POINTER(Rays), ctypes.POINTER(Hits),
+                                           _P, _U32, _U32]),
     "grt_shade_phong": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _P, _U32,
                                        _P, ctypes.POINTER(Phong), _U32, _P, _P]),
     "grt_debug_scan": (ctypes.c_int, [_P, _P, _P, _U32]),
@@ -384,6 +400,18 @@ class Context:
     def set_coop(self, min_refs: int = 16):
         """Cooperative cell tests in drained persistent waves (0: off)."""
         check(lib().grt_set_coop(self.h, min_refs))
+
+    def trace_instances(self, instances, rays: RayBatch, hits: HitBatch, instance_out,
+                        image_width: int = 0):
+        """Closest hit over [(Scene, 3x4 object-to-world matrix), ...];
+        instance_out (int32 per ray) receives the instance index or -1."""
+        arr = (Instance * max(len(instances), 1))()
+        for k, (sc, xf) in enumerate(instances):
+            arr[k].scene = sc.h
+            arr[k].xf[:] = [float(x) for x in np.asarray(xf, np.float32).reshape(12)]
+        check(lib().grt_trace_instances(self.h, arr, len(instances), ctypes.byref(rays.c()),
+                                        ctypes.byref(hits.c()), _ptr(instance_out), rays.n,
+                                        image_width))
 
     def gen_primary(self, cam: Camera, out: RayBatch):
         check(lib().grt_gen_primary(self.h, ctypes.byref(cam), ctypes.byref(out.c())))

@@ -219,6 +219,29 @@ int grt_trace_closest(grt_scene scene, const grt_rays* rays, const grt_hits* hit
 int grt_trace_any(grt_scene scene, const grt_rays* rays, uint8_t* occluded_dev, uint32_t n,
                   uint32_t image_width, uint32_t* counters_dev);
 
+/* ---- instances --------------------------------------------------------- */
+/* A scene placed in the world by a row-major 3x4 object-to-world matrix
+ * (rt::Instance with its Transform, include/rt/Instance.h, Transform.h). */
+typedef struct {
+    grt_scene scene;
+    float xf[12];
+} grt_instance;
+/* Inverse of a row-major 3x4 affine matrix (binary64 cofactors, rounded to
+ * binary32), the matrix the rays are taken through.  GRT_INVALID_PARAM when
+ * the 3x3 part is singular. */
+int grt_affine_inverse(const float xf[12], float inv[12]);
+/* Closest hit over instances (IAccStruct::traceNearestInstance,
+ * IAccStruct.cpp:26-73): each world ray goes through every instance's
+ * inverse matrix (origin affine, direction linear and not renormalised, so t
+ * is the same in both spaces) and is traced in the instance's grid with its
+ * interval ending just below the best hit so far; the nearest hit wins, an
+ * earlier instance on ties.  hits receive the instance scene's triangle id;
+ * instance_dev (int32 per ray) the instance index, -1 for a miss.  All scenes
+ * belong to ctx.  Uses 48 bytes of context scratch per ray. */
+int grt_trace_instances(grt_context ctx, const grt_instance* instances, uint32_t n_instances,
+                        const grt_rays* rays, const grt_hits* hits, int32_t* instance_dev,
+                        uint32_t n, uint32_t image_width);
+
 /* ---- ray generation --------------------------------------------------- */
 /* PinholeCamera::setLookAt/setPerspective/setViewport/newFrame
  * (PinholeCamera.cpp:23-55,91-118), host only. */

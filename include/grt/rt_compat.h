@@ -160,10 +160,12 @@ public:
     std::vector<vec3f> normals;
 };
 
-// include/rt/Instance.h:13-20 without the transform.
+// include/rt/Instance.h:13-20: a geometry placed by a row-major 3x4
+// object-to-world matrix (identity by default).
 struct Instance {
     Geometry* geometry = nullptr;
     Aabb bbox;
+    float transform[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
 };
 
 // include/rt/IPlugin.h:9-18.

@@ -6,6 +6,8 @@ counters (cells visited, triangle tests) are compared bit for bit because both
 sides evaluate the same operation sequence (explicit FMAs, IEEE division and
 sqrt, no contraction).  All calls go through the C ABI (libgrt.so).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -471,3 +473,70 @@ def test_mesh_scene_parity(gpu_ctx, tmp_path):
     assert np.count_nonzero(o["id"] >= 0) > rays["ox"].shape[0] // 4
     sh = orc.gen_shadow(rays, o, nrm, (5.0, 9.0, -4.0), 1e-3)
     check_any(sc, fr, offs, refs, v, sh)
+
+
+# ---------------------------------------------------------------------------
+# Instances: rays through each instance's inverse matrix, nearest hit kept
+# ---------------------------------------------------------------------------
+def _oracle_instances(items, rays):
+    """The oracle restatement of IAccStruct::traceNearestInstance on the
+    oracle's own traversal: items are (frame, offsets, refs, verts, inverse)."""
+    n = rays["ox"].shape[0]
+    best = {"id": np.full(n, -1, np.int32), "t": np.full(n, np.inf, np.float32),
+            "u": np.zeros(n, np.float32), "v": np.zeros(n, np.float32)}
+    inst = np.full(n, -1, np.int32)
+    o = [rays[k] for k in ("ox", "oy", "oz")]
+    d = [rays[k] for k in ("dx", "dy", "dz")]
+    for k, (fr, offs, refs, v, a) in enumerate(items):
+        a = np.asarray(a, np.float32)
+        loc = {}
+        for r, name in enumerate("xyz"):
+            loc["o" + name] = ((a[4 * r] * o[0] + a[4 * r + 1] * o[1]) + a[4 * r + 2] * o[2]) + a[4 * r + 3]
+            loc["d" + name] = (a[4 * r] * d[0] + a[4 * r + 1] * d[1]) + a[4 * r + 2] * d[2]
+        loc["tmin"] = rays["tmin"].copy()
+        tmax = rays["tmax"].copy()
+        fin = best["t"] < np.inf
+        tmax[fin] = np.minimum(np.nextafter(best["t"][fin], np.float32(-np.inf)), tmax[fin])
+        loc["tmax"] = tmax
+        h = orc.trace_closest(fr, offs, refs, v, loc)
+        m = h["id"] >= 0
+        for key in best:
+            best[key][m] = h[key][m]
+        inst[m] = k
+    return best, inst
+
+
+def test_instances_parity(gpu_ctx, c1_scene):
+    import torch
+    v, nrm = c1_scene
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, nrm, 5.0)
+    cube = grt.Mesh()
+    cube.add_cube()
+    cv, cn, _, _ = cube.arrays()
+    csc, cfr, coffs, crefs = check_grid(gpu_ctx, cv, cn, 5.0)
+    c, s = math.cos(0.5), math.sin(0.5)
+    placed = [
+        (sc, fr, offs, refs, v, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]),
+        (sc, fr, offs, refs, v, [0.8 * c, 0, 0.8 * s, 0.3, 0, 0.8, 0, 0.05, -0.8 * s, 0, 0.8 * c, 0.5]),
+        (csc, cfr, coffs, crefs, cv, [2.0, 0, 0, 0.5, 0, 0.1, 0, -0.1, 0, 0, 2.0, 0.5]),
+        (sc, fr, offs, refs, v, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]),   # ties: instance 0 wins
+    ]
+    cam = grt.camera_lookat((0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0, 1, 0), 60.0, 128, 96)
+    rays = orc.primary_rays(grt.camera_dict(cam))
+    items = [(f, o, r, vv, grt.affine_inverse(xf)) for (_, f, o, r, vv, xf) in placed]
+    want, want_inst = _oracle_instances(items, rays)
+    assert set(np.unique(want_inst)) >= {-1, 0, 1, 2} and not np.any(want_inst == 3)
+    rb = grt.RayBatch.from_numpy(rays)
+    for mode, param in (("simple", 16), ("pooled", 16), ("split", 4)):
+        gpu_ctx.set_trace_mode(mode, param)
+        hb = grt.HitBatch.empty(rb.n)
+        inst = torch.full((rb.n,), 77, dtype=torch.int32, device="cuda:0")
+        gpu_ctx.trace_instances([(p[0], p[5]) for p in placed], rb, hb, inst, image_width=cam.width)
+        g = hb.numpy()
+        for k in ("id", "t", "u", "v"):
+            assert_same(g[k], want[k], f"instances hit.{k} [{mode}]")
+        assert_same(inst.cpu().numpy(), want_inst, f"instance index [{mode}]")
+    gpu_ctx.set_trace_mode("simple", 16)
+    with pytest.raises(grt.GrtError):
+        gpu_ctx.trace_instances([(sc, [0] * 12)], rb, grt.HitBatch.empty(rb.n),
+                                torch.empty(rb.n, dtype=torch.int32, device="cuda:0"))
