@@ -61,6 +61,8 @@ def parse():
                         "shadow): s = one ray per lane, w = one ray per lane with pooled "
                         "tests, p = persistent refill, 2/4/8 = lanes per ray; default: tuned")
     p.add_argument("--refill", type=int, default=16)
+    p.add_argument("--ray-lists", action="store_true",
+                   help="compact the secondary stages' active rays into lists and trace those")
     p.add_argument("--coop", type=int, default=16,
                    help="cooperative cell tests in drained persistent waves: minimum "
                         "references left (0 = off)")
@@ -121,6 +123,10 @@ class Frame:
         self.rhits = grt.HitBatch.empty(self.n)
         self.shadow2 = grt.RayBatch.empty(self.n)
         self.occ2 = torch.empty(self.n, dtype=torch.uint8, device="cuda:0")
+        # ray lists of the three secondary stages (active ray indices + count)
+        self.lists = [torch.empty(self.n, dtype=torch.int32, device="cuda:0") for _ in range(3)]
+        self.counts = [torch.zeros(1, dtype=torch.int32, device="cuda:0") for _ in range(3)]
+        self.use_lists = False
         self.rgb = torch.empty((self.n, 3), dtype=torch.float32, device="cuda:0")
         self.rgba = torch.empty(self.n, dtype=torch.int32, device="cuda:0")
         self.phong = grt.phong({"light": LIGHT})
@@ -178,20 +184,36 @@ class Frame:
         mark(3)
         sc.trace_closest(self.prim, self.hits, image_width=W)
         mark(4)
+        L, C = self.lists, self.counts
         sc.gen_shadow(self.prim, self.hits, LIGHT, OFFSET, self.shadow)
+        if self.use_lists:
+            self.ctx.compact_rays(self.shadow, W, L[0], C[0], occluded=self.occ)
         self.mode(1)
         mark(5)
-        sc.trace_any(self.shadow, self.occ, image_width=W)
+        if self.use_lists:
+            sc.trace_any_list(self.shadow, self.occ, L[0], C[0])
+        else:
+            sc.trace_any(self.shadow, self.occ, image_width=W)
         mark(6)
         sc.gen_reflection(self.prim, self.hits, OFFSET, self.refl)
+        if self.use_lists:
+            self.ctx.compact_rays(self.refl, W, L[1], C[1], miss_hits=self.rhits)
         self.mode(2)
         mark(7)
-        sc.trace_closest(self.refl, self.rhits, image_width=W)
+        if self.use_lists:
+            sc.trace_closest_list(self.refl, self.rhits, L[1], C[1])
+        else:
+            sc.trace_closest(self.refl, self.rhits, image_width=W)
         mark(8)
         sc.gen_shadow(self.refl, self.rhits, LIGHT, OFFSET, self.shadow2)
+        if self.use_lists:
+            self.ctx.compact_rays(self.shadow2, W, L[2], C[2], occluded=self.occ2)
         self.mode(3)
         mark(9)
-        sc.trace_any(self.shadow2, self.occ2, image_width=W)
+        if self.use_lists:
+            sc.trace_any_list(self.shadow2, self.occ2, L[2], C[2])
+        else:
+            sc.trace_any(self.shadow2, self.occ2, image_width=W)
         mark(10)
         sc.shade_phong(self.prim, self.hits, self.rgb, self.occ, 0, self.phong)
         sc.shade_phong(self.refl, self.rhits, self.rgb, self.occ2, 1, self.phong, rgba=self.rgba)
@@ -353,6 +375,7 @@ def main():
         print(json.dumps({"mode": args.mode, **res}))
         return
     frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill)
+    frame.use_lists = args.ray_lists
     if args.block:
         frame.ctx.set_block_size(args.block, args.block)
     frame.ctx.set_coop(args.coop)
@@ -385,6 +408,7 @@ def main():
                 **trace_ms, "shade": d[10]}
     if args.profile:
         print(json.dumps({"mode": "profile", "lambda": lam, "schedule": sched, "coop": args.coop,
+                          "lists": frame.use_lists,
                           "ms_per_step": round(ms_per_step, 3),
                           "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}}))
         return
@@ -451,7 +475,8 @@ def main():
             "roofline_by_stage": per_stage_roof,
             "stage_counters": counters,
             "traversal": {"schedule_per_stage": sched, "refill": args.refill,
-                          "block": args.block or 128, "lambda": lam, "coop": args.coop},
+                          "block": args.block or 128, "lambda": lam, "coop": args.coop,
+                          "secondary_ray_lists": frame.use_lists},
             "autotune_frame_ms_by_lambda": tune_table,
             "parallelism": "single GPU",
         },

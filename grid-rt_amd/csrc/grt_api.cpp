@@ -260,6 +260,7 @@ int grt_shutdown(grt_context ctx)
     for (grt_scene_t* sc : std::vector<grt_scene_t*>(ctx->scenes.begin(), ctx->scenes.end()))
         grt_scene_destroy(sc);
     ctx->instance_scratch.release(ctx);
+    ctx->compact_scratch.release(ctx);
     if (ctx->own_stream)
         (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -791,15 +792,10 @@ static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint
     return a;
 }
 
-int grt_trace_closest(grt_scene sc, const grt_rays* rays, const grt_hits* hits, uint32_t n,
-                      uint32_t image_width, uint32_t* counters_dev)
+// One closest-hit launch of the context's schedule.
+static int dispatch_closest(grt_scene sc, const TraceArgs& a, const grt_rays* rays,
+                            const grt_hits* hits)
 {
-    int rc = trace_ready(sc, "grt_trace_closest");
-    if (rc != GRT_OK)
-        return rc;
-    if (!rays_ok(rays) || !hits_ok(hits))
-        return set_error(GRT_INVALID_PARAM, "grt_trace_closest: NULL ray or hit buffer");
-    const TraceArgs a = trace_args(sc, n, image_width, counters_dev);
     const HitSoA h{hits->id, hits->t, hits->u, hits->v};
     grt_context_t* ctx = sc->ctx;
     if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
@@ -815,15 +811,10 @@ int grt_trace_closest(grt_scene sc, const grt_rays* rays, const grt_hits* hits, 
                      "trace_closest");
 }
 
-int grt_trace_any(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev, uint32_t n,
-                  uint32_t image_width, uint32_t* counters_dev)
+// One any-hit launch of the context's schedule.
+static int dispatch_any(grt_scene sc, const TraceArgs& a, const grt_rays* rays,
+                        uint8_t* occluded_dev)
 {
-    int rc = trace_ready(sc, "grt_trace_any");
-    if (rc != GRT_OK)
-        return rc;
-    if (!rays_ok(rays) || !occluded_dev)
-        return set_error(GRT_INVALID_PARAM, "grt_trace_any: NULL ray or output buffer");
-    const TraceArgs a = trace_args(sc, n, image_width, counters_dev);
     grt_context_t* ctx = sc->ctx;
     if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
         return hip_check(launch_trace_persistent(a, to_soa(rays), HitSoA{}, occluded_dev, true,
@@ -836,6 +827,78 @@ int grt_trace_any(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev, uin
                          "trace_any");
     return hip_check(launch_trace_any(a, to_soa(rays), occluded_dev, ctx->block_any, ctx->stream),
                      "trace_any");
+}
+
+int grt_trace_closest(grt_scene sc, const grt_rays* rays, const grt_hits* hits, uint32_t n,
+                      uint32_t image_width, uint32_t* counters_dev)
+{
+    int rc = trace_ready(sc, "grt_trace_closest");
+    if (rc != GRT_OK)
+        return rc;
+    if (!rays_ok(rays) || !hits_ok(hits))
+        return set_error(GRT_INVALID_PARAM, "grt_trace_closest: NULL ray or hit buffer");
+    return dispatch_closest(sc, trace_args(sc, n, image_width, counters_dev), rays, hits);
+}
+
+int grt_trace_any(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev, uint32_t n,
+                  uint32_t image_width, uint32_t* counters_dev)
+{
+    int rc = trace_ready(sc, "grt_trace_any");
+    if (rc != GRT_OK)
+        return rc;
+    if (!rays_ok(rays) || !occluded_dev)
+        return set_error(GRT_INVALID_PARAM, "grt_trace_any: NULL ray or output buffer");
+    return dispatch_any(sc, trace_args(sc, n, image_width, counters_dev), rays, occluded_dev);
+}
+
+int grt_compact_rays(grt_context ctx, const grt_rays* rays, uint32_t n, uint32_t image_width,
+                     uint32_t* list_dev, uint32_t* count_dev, uint8_t* occluded_dev,
+                     const grt_hits* miss_hits)
+{
+    if (!ctx || !rays_ok(rays) || !list_dev || !count_dev || (miss_hits && !hits_ok(miss_hits)))
+        return set_error(GRT_INVALID_PARAM, "grt_compact_rays: bad argument");
+    const uint64_t slots = slot_count(n, image_width);
+    if (slots > 0xffffffffull - kScanTile)
+        return set_error(GRT_INVALID_PARAM, "grt_compact_rays: too many rays");
+    int rc = ctx->compact_scratch.reserve(ctx, (slots / kScanTile + 2) * 4, false);
+    if (rc != GRT_OK)
+        return rc;
+    const HitSoA mh = miss_hits ? HitSoA{miss_hits->id, miss_hits->t, miss_hits->u, miss_hits->v}
+                                : HitSoA{nullptr, nullptr, nullptr, nullptr};
+    return hip_check(launch_compact_rays(n, image_width, rays->tmin, rays->tmax, list_dev,
+                                         count_dev, occluded_dev, mh,
+                                         ctx->compact_scratch.as<uint32_t>(), ctx->stream),
+                     "compact_rays");
+}
+
+int grt_trace_closest_list(grt_scene sc, const grt_rays* rays, const grt_hits* hits,
+                           const uint32_t* list_dev, const uint32_t* count_dev, uint32_t n,
+                           uint32_t* counters_dev)
+{
+    int rc = trace_ready(sc, "grt_trace_closest_list");
+    if (rc != GRT_OK)
+        return rc;
+    if (!rays_ok(rays) || !hits_ok(hits) || !list_dev || !count_dev)
+        return set_error(GRT_INVALID_PARAM, "grt_trace_closest_list: bad argument");
+    TraceArgs a = trace_args(sc, n, 0, counters_dev);
+    a.list = list_dev;
+    a.list_count = count_dev;
+    return dispatch_closest(sc, a, rays, hits);
+}
+
+int grt_trace_any_list(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev,
+                       const uint32_t* list_dev, const uint32_t* count_dev, uint32_t n,
+                       uint32_t* counters_dev)
+{
+    int rc = trace_ready(sc, "grt_trace_any_list");
+    if (rc != GRT_OK)
+        return rc;
+    if (!rays_ok(rays) || !occluded_dev || !list_dev || !count_dev)
+        return set_error(GRT_INVALID_PARAM, "grt_trace_any_list: bad argument");
+    TraceArgs a = trace_args(sc, n, 0, counters_dev);
+    a.list = list_dev;
+    a.list_count = count_dev;
+    return dispatch_any(sc, a, rays, occluded_dev);
 }
 
 int grt_camera_lookat(const float eye[3], const float center[3], const float up[3],

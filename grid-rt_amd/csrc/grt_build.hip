@@ -612,6 +612,95 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Ray-list compaction (the scan applied to ray batches)
+// ---------------------------------------------------------------------------
+// A block takes kScanTile consecutive slots in 16 rounds of 256, so that the
+// lanes of a wave read consecutive slots; the list keeps slot order.
+
+__device__ __forceinline__ bool compact_active(uint32_t slot, uint32_t n, uint32_t img_w,
+                                               const float* __restrict__ tmin,
+                                               const float* __restrict__ tmax, uint32_t& i,
+                                               bool& valid)
+{
+    i = slot_ray(slot, img_w, n, valid);
+    return valid && tmin[i] <= tmax[i];
+}
+
+__global__ __launch_bounds__(256) void k_compact_count(uint32_t n, uint32_t img_w,
+                                                       uint32_t n_slots,
+                                                       const float* __restrict__ tmin,
+                                                       const float* __restrict__ tmax,
+                                                       uint32_t* __restrict__ bsum)
+{
+    __shared__ uint32_t lds[8];
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t c = 0;
+    for (uint32_t r = 0; r < kScanTile / 256; ++r) {
+        const uint32_t slot = base + r * 256 + threadIdx.x;
+        uint32_t i;
+        bool valid;
+        c += (slot < n_slots && compact_active(slot, n, img_w, tmin, tmax, i, valid)) ? 1u : 0u;
+    }
+    uint32_t total;
+    block_excl_scan<256>(c, lds, total);
+    if (threadIdx.x == 0)
+        bsum[blockIdx.x] = total;
+}
+
+// Exclusive scan of the nb block counts in place; the total to *count.
+__global__ __launch_bounds__(1024) void k_compact_offsets(uint32_t* __restrict__ bsum, uint32_t nb,
+                                                          uint32_t* __restrict__ count)
+{
+    __shared__ uint32_t lds[17];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t v = b < nb ? bsum[b] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<1024>(v, lds, tot);
+        if (b < nb)
+            bsum[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0)
+        count[0] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(uint32_t n, uint32_t img_w, uint32_t n_slots,
+                                                       const float* __restrict__ tmin,
+                                                       const float* __restrict__ tmax,
+                                                       const uint32_t* __restrict__ bsum,
+                                                       uint32_t* __restrict__ list,
+                                                       uint8_t* __restrict__ occluded,
+                                                       HitSoA miss)
+{
+    __shared__ uint32_t lds[8];
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t pos = bsum[blockIdx.x];
+    for (uint32_t r = 0; r < kScanTile / 256; ++r) {
+        const uint32_t slot = base + r * 256 + threadIdx.x;
+        uint32_t i = 0;
+        bool valid = false;
+        const bool act = slot < n_slots && compact_active(slot, n, img_w, tmin, tmax, i, valid);
+        uint32_t total;
+        const uint32_t ex = block_excl_scan<256>(act ? 1u : 0u, lds, total);
+        if (act) {
+            list[pos + ex] = i;
+        } else if (valid) {
+            if (occluded)
+                occluded[i] = 0;
+            if (miss.id) {
+                miss.id[i] = -1;
+                miss.t[i] = INFINITY;
+                miss.u[i] = 0.0f;
+                miss.v[i] = 0.0f;
+            }
+        }
+        pos += total;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // 5. Per-cell sort
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
@@ -1127,6 +1216,24 @@ hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32
                        out, 0xffffffffu);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n, (const DevFrame*)nullptr,
                        (const uint32_t*)block_sums, out, SortLists{});
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_rays(uint32_t n, uint32_t img_w, const float* tmin, const float* tmax,
+                               uint32_t* list, uint32_t* count, uint8_t* occluded,
+                               const HitSoA& miss_hits, uint32_t* block_sums, hipStream_t s)
+{
+    const uint64_t slots = slot_count(n, img_w);
+    const uint32_t nb = (uint32_t)((slots + kScanTile - 1) / kScanTile);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(256), 0, s, n, img_w, (uint32_t)slots,
+                           tmin, tmax, block_sums);
+    }
+    hipLaunchKernelGGL(k_compact_offsets, dim3(1), dim3(1024), 0, s, block_sums, nb, count);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_compact_write, dim3(nb), dim3(256), 0, s, n, img_w, (uint32_t)slots,
+                           tmin, tmax, (const uint32_t*)block_sums, list, occluded, miss_hits);
+    }
     return hipGetLastError();
 }
 

@@ -124,4 +124,31 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Slot -> ray index.  With img_w != 0 each group of 64 consecutive slots is an
+// 8x8 pixel tile of a row-major image of that width.
+__device__ __forceinline__ uint32_t slot_ray(uint32_t slot, uint32_t img_w, uint32_t n,
+                                             bool& valid)
+{
+    if (img_w == 0) {
+        valid = slot < n;
+        return slot;
+    }
+    const uint32_t wave = slot >> 6, lane = slot & 63u;
+    const uint32_t tiles_x = (img_w + 7u) >> 3;
+    const uint32_t tx = wave % tiles_x, ty = wave / tiles_x;
+    const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
+    const uint32_t i = py * img_w + px;
+    valid = (px < img_w) && (i < n);
+    return i;
+}
+
+// Slots covering n rays: n, or whole 8x8 tiles of an image of width img_w.
+inline uint64_t slot_count(uint32_t n, uint32_t img_w)
+{
+    if (img_w == 0)
+        return n;
+    const uint64_t h = (n + (uint64_t)img_w - 1) / img_w;
+    return (uint64_t)((img_w + 7u) >> 3) * ((h + 7u) >> 3) * 64u;
+}
+
 }  // namespace grt

@@ -56,6 +56,7 @@ struct grt_context_t {
     uint32_t coop = 16;   // cooperative cell tests in drained persistent waves (0: off)
     std::set<grt_scene_t*> scenes;
     grt::DevMem instance_scratch;   // object-space rays and candidate hits of instance tracing
+    grt::DevMem compact_scratch;    // block sums of ray-list compaction
 };
 
 struct grt_scene_t {

@@ -93,6 +93,8 @@ struct TraceArgs {
                             // references left (0: off)
     uint32_t pool;          // one-ray-per-lane kernels: pool the wave's tests (1) or
                             // test per lane (0)
+    const uint32_t* list = nullptr;        // or the rays to trace: slot s traces list[s] ...
+    const uint32_t* list_count = nullptr;  // ... for s < *list_count (a device value) and s < n
 };
 hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
                                 uint32_t block, hipStream_t s);
@@ -144,6 +146,14 @@ hipError_t launch_gen_shadow(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
 hipError_t launch_gen_reflection(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
                                  const float* normals, float offset, const RaySoAOut& out,
                                  hipStream_t s);
+// Active rays (tmin <= tmax) in slot order (8x8 tiles of an image of width
+// img_w, or ray order when 0) -> list[0..*count); the next stage's result of
+// every inactive ray written as a miss (occluded = 0 and / or hit id -1,
+// t = +inf, u = v = 0, for the outputs that are set).  block_sums holds
+// slots / kScanTile + 1 entries.
+hipError_t launch_compact_rays(uint32_t n, uint32_t img_w, const float* tmin, const float* tmax,
+                               uint32_t* list, uint32_t* count, uint8_t* occluded,
+                               const HitSoA& miss_hits, uint32_t* block_sums, hipStream_t s);
 hipError_t launch_gen_ao(uint32_t n, uint32_t spp, uint64_t seed, const RaySoA& in,
                          const HitSoAIn& hits, const float* normals, float offset,
                          float ao_tmax, const RaySoAOut& out, hipStream_t s);

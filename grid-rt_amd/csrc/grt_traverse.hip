@@ -32,30 +32,16 @@ namespace grt {
 // Ray indexing
 // ---------------------------------------------------------------------------
 
-// Slot -> ray index.  With img_w != 0 each group of 64 consecutive slots is an
-// 8x8 pixel tile of a row-major image of that width.
-__device__ __forceinline__ uint32_t slot_ray(uint32_t slot, uint32_t img_w, uint32_t n,
-                                             bool& valid)
+// Slot -> ray: through the ray list when the launch has one (slots beyond
+// the list's device-side count are idle), else the image-tile mapping.
+__device__ __forceinline__ uint32_t trace_ray(const TraceArgs& a, uint32_t slot, bool& valid)
 {
-    if (img_w == 0) {
-        valid = slot < n;
-        return slot;
+    if (a.list) {
+        const uint32_t c = *a.list_count;
+        valid = slot < c && slot < a.n;
+        return valid ? a.list[slot] : 0u;
     }
-    const uint32_t wave = slot >> 6, lane = slot & 63u;
-    const uint32_t tiles_x = (img_w + 7u) >> 3;
-    const uint32_t tx = wave % tiles_x, ty = wave / tiles_x;
-    const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
-    const uint32_t i = py * img_w + px;
-    valid = (px < img_w) && (i < n);
-    return i;
-}
-
-static uint64_t slot_count(uint32_t n, uint32_t img_w)
-{
-    if (img_w == 0)
-        return n;
-    const uint64_t h = (n + (uint64_t)img_w - 1) / img_w;
-    return (uint64_t)((img_w + 7u) >> 3) * ((h + 7u) >> 3) * 64u;
+    return slot_ray(slot, a.image_width, a.n, valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -806,7 +792,7 @@ __global__ __launch_bounds__(256) void k_trace_closest(TraceArgs a, RaySoA rays,
 {
     __shared__ WavePool pools[POOL ? 4 : 1];
     bool valid;
-    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
+    const uint32_t i = trace_ray(a, blockIdx.x * blockDim.x + threadIdx.x, valid);
     trace_closest_lane<COUNT, POOL>(a.fr, a.offsets, a.trir, rays, hits, a.counters, a.coop, i,
                                     valid, POOL ? &pools[threadIdx.x >> 6] : nullptr);
 }
@@ -817,7 +803,7 @@ __global__ __launch_bounds__(256) void k_trace_any(TraceArgs a, RaySoA rays,
 {
     __shared__ WavePool pools[POOL ? 4 : 1];
     bool valid;
-    const uint32_t i = slot_ray(blockIdx.x * blockDim.x + threadIdx.x, a.image_width, a.n, valid);
+    const uint32_t i = trace_ray(a, blockIdx.x * blockDim.x + threadIdx.x, valid);
     trace_any_lane<COUNT, POOL>(a.fr, a.offsets, a.trir, rays, occluded, a.counters, a.coop, i,
                                 valid, POOL ? &pools[threadIdx.x >> 6] : nullptr);
 }
@@ -1010,7 +996,7 @@ __global__ __launch_bounds__(256) void k_trace_split(TraceArgs a, RaySoA rays, H
     const uint32_t shift = (uint32_t)__builtin_ctz(lanes);
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid;
-    const uint32_t i = slot_ray(t >> shift, a.image_width, a.n, valid);
+    const uint32_t i = trace_ray(a, t >> shift, valid);
     trace_split_lane<ANY, COUNT>(a.fr, a.offsets, a.trir, rays, hits, occluded, a.counters,
                                  a.coop, lanes, i, valid);
 }
@@ -1037,6 +1023,8 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a, RaySoA ra
     const bool ok = grid_ok(fr);
     const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t slot_limit =
+        a.list ? (*a.list_count < n_slots ? *a.list_count : n_slots) : n_slots;
 
     int32_t ray = -1;         // ray index traced by this lane, -1 = idle
     bool exhausted = false;   // no slots left for this lane
@@ -1063,11 +1051,11 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceArgs a, RaySoA ra
             base = __builtin_amdgcn_readlane(base, leader);
             if (need) {
                 const uint32_t slot = base + lanes_below(need_mask);
-                if (slot >= n_slots) {
+                if (slot >= slot_limit) {
                     exhausted = true;
                 } else {
                     bool valid;
-                    const uint32_t ri = slot_ray(slot, a.image_width, a.n, valid);
+                    const uint32_t ri = trace_ray(a, slot, valid);
                     if (valid) {
                         o0 = rays.ox[ri];
                         o1 = rays.oy[ri];

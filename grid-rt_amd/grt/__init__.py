@@ -140,6 +140,11 @@ _SIGS = {
     "grt_scene_upload_mesh": (ctypes.c_int, [_P, _P]),
     "grt_triangle_valid": (ctypes.c_int, [_P]),
     "grt_affine_inverse": (ctypes.c_int, [_P, _P]),
+    "grt_compact_rays": (ctypes.c_int, [_P, ctypes.POINTER(Rays), _U32, _U32, _P, _P, _P,
+                                        ctypes.POINTER(Hits)]),
+    "grt_trace_closest_list": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _P,
+                                              _P, _U32, _P]),
+    "grt_trace_any_list": (ctypes.c_int, [_P, ctypes.POINTER(Rays), _P, _P, _P, _U32, _P]),
     "grt_trace_instances": (ctypes.c_int, [_P, _P, _U32, ctypes.POINTER(Rays), ctypes.POINTER(Hits),
                                            _P, _U32, _U32]),
     "grt_shade_phong": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _P, _U32,
@@ -413,6 +418,16 @@ class Context:
                                         ctypes.byref(hits.c()), _ptr(instance_out), rays.n,
                                         image_width))
 
+    def compact_rays(self, rays: RayBatch, image_width: int, ray_list, count, occluded=None,
+                     miss_hits: "HitBatch | None" = None):
+        """List the active rays of a batch (grt_compact_rays): ray_list a device
+        int32 tensor of rays.n entries, count a one-element device tensor."""
+        check(lib().grt_compact_rays(self.h, ctypes.byref(rays.c()), rays.n, image_width,
+                                     _ptr(ray_list), _ptr(count),
+                                     _ptr(occluded) if occluded is not None else None,
+                                     ctypes.byref(miss_hits.c()) if miss_hits is not None
+                                     else None))
+
     def gen_primary(self, cam: Camera, out: RayBatch):
         check(lib().grt_gen_primary(self.h, ctypes.byref(cam), ctypes.byref(out.c())))
 
@@ -599,6 +614,16 @@ class Scene:
     def gen_reflection(self, rays: RayBatch, hits: HitBatch, offset: float, out: RayBatch):
         check(lib().grt_gen_reflection(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()),
                                        rays.n, offset, ctypes.byref(out.c())))
+
+    # Ray lists (grt_compact_rays / grt_trace_*_list): `ray_list` is a device
+    # int32/uint32 tensor of n entries, `count` a one-element device tensor.
+    def trace_closest_list(self, rays: RayBatch, hits: HitBatch, ray_list, count, counters=None):
+        check(lib().grt_trace_closest_list(self.h, ctypes.byref(rays.c()), ctypes.byref(hits.c()),
+                                           _ptr(ray_list), _ptr(count), rays.n, _ptr(counters)))
+
+    def trace_any_list(self, rays: RayBatch, occluded, ray_list, count, counters=None):
+        check(lib().grt_trace_any_list(self.h, ctypes.byref(rays.c()), _ptr(occluded),
+                                       _ptr(ray_list), _ptr(count), rays.n, _ptr(counters)))
 
     def gen_ao(self, rays: RayBatch, hits: HitBatch, spp: int, seed: int, offset: float,
                ao_tmax: float, out: RayBatch):

@@ -242,6 +242,28 @@ int grt_trace_instances(grt_context ctx, const grt_instance* instances, uint32_t
                         const grt_rays* rays, const grt_hits* hits, int32_t* instance_dev,
                         uint32_t n, uint32_t image_width);
 
+/* ---- ray lists ---------------------------------------------------------- */
+/* A secondary stage leaves many slots without a ray (pixels that missed, points
+ * facing away from the light).  grt_compact_rays lists the active rays of a
+ * batch (tmin <= tmax) in slot order -- 8x8 tiles of an image of width
+ * image_width, or ray order when 0 -- into list_dev (n entries) and their
+ * number into *count_dev on the device, and writes the next stage's result of
+ * every inactive ray as a miss: occluded_dev[i] = 0 when occluded_dev is set,
+ * hit id -1, t +inf, u = v = 0 when miss_hits is set.  The _list traces then
+ * trace the listed rays only, reading the count on the device (no host
+ * synchronisation), so waves are full of active rays in image order.  Results
+ * are those of the full traces; counters_dev entries of unlisted rays are not
+ * written.  Three kernels, no memset: safe to capture in a graph. */
+int grt_compact_rays(grt_context ctx, const grt_rays* rays, uint32_t n, uint32_t image_width,
+                     uint32_t* list_dev, uint32_t* count_dev, uint8_t* occluded_dev,
+                     const grt_hits* miss_hits);
+int grt_trace_closest_list(grt_scene scene, const grt_rays* rays, const grt_hits* hits,
+                           const uint32_t* list_dev, const uint32_t* count_dev, uint32_t n,
+                           uint32_t* counters_dev);
+int grt_trace_any_list(grt_scene scene, const grt_rays* rays, uint8_t* occluded_dev,
+                       const uint32_t* list_dev, const uint32_t* count_dev, uint32_t n,
+                       uint32_t* counters_dev);
+
 /* ---- ray generation --------------------------------------------------- */
 /* PinholeCamera::setLookAt/setPerspective/setViewport/newFrame
  * (PinholeCamera.cpp:23-55,91-118), host only. */

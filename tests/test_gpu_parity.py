@@ -540,3 +540,74 @@ def test_instances_parity(gpu_ctx, c1_scene):
     with pytest.raises(grt.GrtError):
         gpu_ctx.trace_instances([(sc, [0] * 12)], rb, grt.HitBatch.empty(rb.n),
                                 torch.empty(rb.n, dtype=torch.int32, device="cuda:0"))
+
+
+# ---------------------------------------------------------------------------
+# Ray lists: the listed (active) rays alone give the full stage's results
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("scene", ["c1", "stadium"])
+def test_ray_lists_match_full_stages(gpu_ctx, c1_scene, scene):
+    import torch
+    if scene == "c1":
+        v, nrm = c1_scene
+        cam = c1_camera(200, 136)
+    else:
+        v, nrm, _ = grt.gen_stadium(SEED_C2, 16, 2048, 16 * 2048 - 2)
+        cam = grt.camera_lookat((0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0, 1, 0), 60.0, 200, 136)
+    sc = grt.Scene(gpu_ctx)
+    sc.upload(v, nrm)
+    sc.build_grid(5.0)
+    n, W = cam.width * cam.height, cam.width
+    light = (3.0, 4.0, -2.0)
+    prim, hits = grt.RayBatch.empty(n), grt.HitBatch.empty(n)
+    gpu_ctx.gen_primary(cam, prim)
+    sc.trace_closest(prim, hits, image_width=W)
+    # full stages
+    sh, refl = grt.RayBatch.empty(n), grt.RayBatch.empty(n)
+    occ, rhits = torch.empty(n, dtype=torch.uint8, device="cuda:0"), grt.HitBatch.empty(n)
+    sc.gen_shadow(prim, hits, light, 1e-3, sh)
+    sc.gen_reflection(prim, hits, 1e-3, refl)
+    cnt_full = torch.zeros(2 * n, dtype=torch.int32, device="cuda:0")
+    sc.trace_any(sh, occ, image_width=W, counters=cnt_full)
+    rcnt_full = torch.zeros(2 * n, dtype=torch.int32, device="cuda:0")
+    sc.trace_closest(refl, rhits, image_width=W, counters=rcnt_full)
+    want_occ, want_r = occ.cpu().numpy(), rhits.numpy()
+    sh_active = (sh.f["tmin"] <= sh.f["tmax"]).cpu().numpy()
+    rf_active = (refl.f["tmin"] <= refl.f["tmax"]).cpu().numpy()
+    assert 0 < sh_active.sum() < n and 0 < rf_active.sum() < n
+    for mode, param in (("simple", 16), ("pooled", 16), ("persistent", 16), ("split", 4)):
+        gpu_ctx.set_trace_mode(mode, param)
+        lst = torch.full((n,), -1, dtype=torch.int32, device="cuda:0")
+        count = torch.full((1,), 12345, dtype=torch.int32, device="cuda:0")
+        sh2 = grt.RayBatch.empty(n)
+        occ2 = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
+        sc.gen_shadow(prim, hits, light, 1e-3, sh2)
+        gpu_ctx.compact_rays(sh2, W, lst, count, occluded=occ2)
+        c = int(count.item())
+        assert c == int(sh_active.sum())
+        # the list is the active rays in 8x8-tile order
+        slots = np.arange(((W + 7) // 8) * ((cam.height + 7) // 8) * 64)
+        tile, lane = slots // 64, slots % 64
+        px = (tile % ((W + 7) // 8)) * 8 + lane % 8
+        py = (tile // ((W + 7) // 8)) * 8 + lane // 8
+        keep = (px < W) & (py < cam.height)
+        order = (py * W + px)[keep]
+        assert np.array_equal(lst[:c].cpu().numpy(), order[sh_active[order]])
+        for k in sh.f:
+            assert_same(sh2.f[k].cpu().numpy(), sh.f[k].cpu().numpy(), f"shadow list ray {k}")
+        cnt = torch.zeros(2 * n, dtype=torch.int32, device="cuda:0")
+        sc.trace_any_list(sh2, occ2, lst, count, counters=cnt)
+        assert_same(occ2.cpu().numpy(), want_occ, f"listed shadow rays [{mode}]")
+        assert_same(cnt.cpu().numpy(), cnt_full.cpu().numpy(), f"listed shadow counters [{mode}]")
+        rf2, rh2 = grt.RayBatch.empty(n), grt.HitBatch.empty(n)
+        rh2.f["id"].fill_(-9)
+        sc.gen_reflection(prim, hits, 1e-3, rf2)
+        gpu_ctx.compact_rays(rf2, W, lst, count, miss_hits=rh2)
+        assert int(count.item()) == int(rf_active.sum())
+        cnt = torch.zeros(2 * n, dtype=torch.int32, device="cuda:0")
+        sc.trace_closest_list(rf2, rh2, lst, count, counters=cnt)
+        g = rh2.numpy()
+        for k in ("id", "t", "u", "v"):
+            assert_same(g[k], want_r[k], f"listed bounce hit.{k} [{mode}]")
+        assert_same(cnt.cpu().numpy(), rcnt_full.cpu().numpy(), f"listed bounce counters [{mode}]")
+    gpu_ctx.set_trace_mode("simple", 16)
