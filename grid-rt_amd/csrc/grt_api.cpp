@@ -160,12 +160,15 @@ static SortLists sort_lists(grt_scene_t* sc)
                      sc->sort_classes.as<uint32_t>()};
 }
 
-static int enqueue_scan(grt_scene_t* sc, uint32_t refs_cap, bool with_lists)
+// prefixes_ready: the tile sums were already scanned by an earlier scan of
+// the same counts (the synchronous build scans twice).
+static int enqueue_scan(grt_scene_t* sc, uint32_t refs_cap, bool with_lists, bool prefixes_ready)
 {
     return hip_check(launch_grid_scan(sc->frame.as<DevFrame>(), sc->counts.as<uint32_t>(),
                                       sc->offsets.as<uint32_t>(), sc->block_sums.as<uint32_t>(),
                                       sc->cells_cap, refs_cap,
-                                      with_lists ? sort_lists(sc) : SortLists{}, sc->ctx->stream),
+                                      with_lists ? sort_lists(sc) : SortLists{}, prefixes_ready,
+                                      sc->ctx->stream),
                      "grid scan");
 }
 
@@ -177,10 +180,11 @@ static int enqueue_build_passes(grt_scene_t* sc, uint32_t refs_cap, bool with_li
     int rc;
     if ((rc = hip_check(launch_grid_count(fr, verts, sc->n_tris, sc->counts.as<uint32_t>(),
                                           sc->big_list.as<uint32_t>(),
-                                          sc->small_counters.as<uint32_t>(), s),
+                                          sc->small_counters.as<uint32_t>(),
+                                          sc->block_sums.as<uint32_t>(), sc->cells_cap, s),
                         "grid count")) != GRT_OK)
         return rc;
-    return enqueue_scan(sc, refs_cap, with_lists);
+    return enqueue_scan(sc, refs_cap, with_lists, false);
 }
 
 static int enqueue_scatter_sort(grt_scene_t* sc)
@@ -577,7 +581,7 @@ int grt_grid_build(grt_scene sc, float lambda)
         return set_error(GRT_CAPACITY, "grid build status 0x%x", host.status);
     if ((rc = grid_reserve_refs(sc, host.r > 0 ? host.r : 1)) != GRT_OK)
         return rc;
-    if ((rc = enqueue_scan(sc, sc->refs_cap, true)) != GRT_OK)
+    if ((rc = enqueue_scan(sc, sc->refs_cap, true, true)) != GRT_OK)
         return rc;
     if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
         return rc;

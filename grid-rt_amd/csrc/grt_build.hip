@@ -228,15 +228,41 @@ __device__ __forceinline__ CellBox tri_cells(const DevFrame* fr, const float* v)
     return b;
 }
 
+// The count pass also sums the counts of every scan tile (kScanTile cells),
+// so the scan needs no reduction pass over the count array.
+constexpr uint32_t kScanTileShift = 12;
+static_assert((1u << kScanTileShift) == kScanTile, "scan tile");
+
+// Adds 1 to the tile sum of every active lane's cell: lanes in the tile of
+// the first active lane add together, the others one by one.  Called by the
+// active lanes of a wave together.
+__device__ __forceinline__ void tile_add_wave(uint32_t* tile_sums, bool active, uint32_t c)
+{
+    const uint64_t am = __ballot(active);
+    if (am == 0)
+        return;
+    const uint32_t f = (uint32_t)__builtin_ctzll(am);
+    const uint32_t tile = c >> kScanTileShift;
+    const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)tile, (int)f);
+    const bool same = active && tile == t0;
+    const uint64_t sm = __ballot(same);
+    if ((threadIdx.x & 63u) == f)
+        atomicAdd(&tile_sums[t0], (uint32_t)__popcll(sm));
+    if (active && !same)
+        atomicAdd(&tile_sums[tile], 1u);
+}
+
 template <bool SCATTER>
 __device__ __forceinline__ void cell_op(uint32_t c, uint32_t t, uint32_t* counts,
-                                        const uint32_t* offsets, uint32_t* refs)
+                                        const uint32_t* offsets, uint32_t* refs,
+                                        uint32_t* tile_sums)
 {
     if (SCATTER) {
         const uint32_t slot = atomicSub(&counts[c], 1u) - 1u;
         refs[offsets[c] + slot] = t;
     } else {
         atomicAdd(&counts[c], 1u);
+        tile_add_wave(tile_sums, true, c);
     }
 }
 
@@ -287,6 +313,8 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t* keys, uint32_t c)
 // adds the block's total; the scatter reserves a range of the cell's slots
 // (counting down, which returns counts[] to zero) and hands positions out of
 // it with LDS atomics.  The order within a cell is fixed by the sort pass.
+constexpr uint32_t kTileSlots = 32;   // per-block tile sums of the count pass
+
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ fr,
                                                    const float* __restrict__ verts, uint32_t n_tris,
@@ -294,10 +322,13 @@ __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ 
                                                    const uint32_t* __restrict__ offsets,
                                                    uint32_t* __restrict__ refs,
                                                    uint32_t* __restrict__ big_list,
-                                                   uint32_t* __restrict__ big_count)
+                                                   uint32_t* __restrict__ big_count,
+                                                   uint32_t* __restrict__ tile_sums)
 {
     __shared__ uint32_t keys[kHashSlots];
     __shared__ uint32_t slot_n[kHashSlots];   // count, then the next absolute position
+    __shared__ uint32_t tile_key[kTileSlots];
+    __shared__ uint32_t tile_n[kTileSlots];
     const uint32_t bad = SCATTER ? (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)
                                  : (kStatusFrameInvalid | kStatusCellsOver);
     if (fr->status & bad)
@@ -305,6 +336,10 @@ __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ 
     for (uint32_t i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
         keys[i] = kHashEmpty;
         slot_n[i] = 0;
+    }
+    if (!SCATTER && threadIdx.x < kTileSlots) {
+        tile_key[threadIdx.x] = kHashEmpty;
+        tile_n[threadIdx.x] = 0;
     }
     __syncthreads();
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -342,6 +377,7 @@ __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ 
                         refs[offsets[c] + slot] = t;
                     } else {
                         atomicAdd(&counts[c], 1u);
+                        atomicAdd(&tile_sums[c >> kScanTileShift], 1u);
                     }
                 }
             }
@@ -352,13 +388,32 @@ __global__ __launch_bounds__(256) void k_grid_pass(const DevFrame* __restrict__ 
         if (c == kHashEmpty)
             continue;
         const uint32_t n = slot_n[i];
-        if (SCATTER)
+        if (SCATTER) {
             slot_n[i] = offsets[c] + atomicSub(&counts[c], n) - n;
-        else
+        } else {
             atomicAdd(&counts[c], n);
+            // the block's tile sums in LDS, a global atomic when they are full
+            const uint32_t tile = c >> kScanTileShift;
+            uint32_t h = tile & (kTileSlots - 1);
+            bool done = false;
+            for (uint32_t p = 0; p < kTileSlots && !done; ++p) {
+                const uint32_t prev = atomicCAS(&tile_key[h], kHashEmpty, tile);
+                if (prev == kHashEmpty || prev == tile) {
+                    atomicAdd(&tile_n[h], n);
+                    done = true;
+                }
+                h = (h + 1) & (kTileSlots - 1);
+            }
+            if (!done)
+                atomicAdd(&tile_sums[tile], n);
+        }
     }
-    if (!SCATTER)
+    if (!SCATTER) {
+        __syncthreads();
+        if (threadIdx.x < kTileSlots && tile_key[threadIdx.x] != kHashEmpty)
+            atomicAdd(&tile_sums[tile_key[threadIdx.x]], tile_n[threadIdx.x]);
         return;
+    }
     __syncthreads();
     if (small) {
         for (int32_t z = 0; z < b.w[2]; ++z)
@@ -382,7 +437,8 @@ __global__ __launch_bounds__(256) void k_grid_pass_huge(const DevFrame* __restri
                                                         const uint32_t* __restrict__ offsets,
                                                         uint32_t* __restrict__ refs,
                                                         const uint32_t* __restrict__ big_list,
-                                                        const uint32_t* __restrict__ big_count)
+                                                        const uint32_t* __restrict__ big_count,
+                                                        uint32_t* __restrict__ tile_sums)
 {
     const uint32_t bad = SCATTER ? (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)
                                  : (kStatusFrameInvalid | kStatusCellsOver);
@@ -403,7 +459,7 @@ __global__ __launch_bounds__(256) void k_grid_pass_huge(const DevFrame* __restri
             const uint32_t z = yz / wy;
             const uint32_t c = ((uint32_t)b.lo[0] + x) +
                                nx * (((uint32_t)b.lo[1] + y) + ny * ((uint32_t)b.lo[2] + z));
-            cell_op<SCATTER>(c, t, counts, offsets, refs);
+            cell_op<SCATTER>(c, t, counts, offsets, refs, tile_sums);
         }
     }
 }
@@ -424,7 +480,8 @@ __global__ __launch_bounds__(256) void k_grid_pass_big(const DevFrame* __restric
                                                        const uint32_t* __restrict__ offsets,
                                                        uint32_t* __restrict__ refs,
                                                        const uint32_t* __restrict__ big_list,
-                                                       const uint32_t* __restrict__ big_count)
+                                                       const uint32_t* __restrict__ big_count,
+                                                       uint32_t* __restrict__ tile_sums)
 {
     const uint32_t bad = SCATTER ? (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)
                                  : (kStatusFrameInvalid | kStatusCellsOver);
@@ -444,7 +501,7 @@ __global__ __launch_bounds__(256) void k_grid_pass_big(const DevFrame* __restric
             const uint32_t z = yz / wy;
             const uint32_t c = ((uint32_t)b.lo[0] + x) +
                                nx * (((uint32_t)b.lo[1] + y) + ny * ((uint32_t)b.lo[2] + z));
-            cell_op<SCATTER>(c, t, counts, offsets, refs);
+            cell_op<SCATTER>(c, t, counts, offsets, refs, tile_sums);
         }
     }
 }
@@ -1173,33 +1230,37 @@ hipError_t launch_frame_from_box(const float bmin[3], const float bmax[3], uint3
 
 hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_tris,
                              uint32_t* counts, uint32_t* big_list, uint32_t* big_count,
-                             hipStream_t s)
+                             uint32_t* tile_sums, uint32_t cells_cap, hipStream_t s)
 {
     hipLaunchKernelGGL(k_zero_build_counters, dim3(1), dim3(64), 0, s, big_count);
+    const uint32_t nb = grid_for(cells_cap, kScanTile);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nb, 256, 1024)), dim3(256), 0, s, tile_sums, 0u,
+                       (uint64_t)nb);
     hipLaunchKernelGGL(k_grid_pass<false>, dim3(grid_for(n_tris, 256)), dim3(256), 0, s, fr, verts,
                        n_tris, counts, (const uint32_t*)nullptr, (uint32_t*)nullptr, big_list,
-                       big_count);
+                       big_count, tile_sums);
     hipLaunchKernelGGL(k_grid_pass_big<false>, dim3(2048), dim3(256), 0, s, fr, verts, counts,
                        (const uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)big_list,
-                       (const uint32_t*)big_count);
+                       (const uint32_t*)big_count, tile_sums);
     hipLaunchKernelGGL(k_grid_pass_huge<false>, dim3(2048), dim3(256), 0, s, fr, verts, n_tris,
                        counts, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                       (const uint32_t*)big_list, (const uint32_t*)big_count);
+                       (const uint32_t*)big_list, (const uint32_t*)big_count, tile_sums);
     return hipGetLastError();
 }
 
 hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
                             uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
-                            const SortLists& lists, hipStream_t s)
+                            const SortLists& lists, bool prefixes_ready, hipStream_t s)
 {
     const uint32_t nb = grid_for(cells_cap, kScanTile);
     if (lists.small)
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, lists.counts, 0u,
                            (uint64_t)(2 + kSortClasses));
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
-                       block_sums);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, 0u, fr, offsets,
-                       refs_cap);
+    // block_sums holds the count pass's tile sums: scan them into tile
+    // offsets (in place), unless an earlier scan of the same counts did.
+    if (!prefixes_ready)
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, 0u, fr, offsets,
+                           refs_cap);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
                        (const uint32_t*)block_sums, offsets, lists);
     return hipGetLastError();
@@ -1243,11 +1304,12 @@ hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t 
                                hipStream_t s)
 {
     hipLaunchKernelGGL(k_grid_pass<true>, dim3(grid_for(n_tris, 256)), dim3(256), 0, s, fr, verts,
-                       n_tris, counts, offsets, refs, (uint32_t*)nullptr, (uint32_t*)nullptr);
+                       n_tris, counts, offsets, refs, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                       (uint32_t*)nullptr);
     hipLaunchKernelGGL(k_grid_pass_big<true>, dim3(2048), dim3(256), 0, s, fr, verts, counts, offsets,
-                       refs, big_list, big_count);
+                       refs, big_list, big_count, (uint32_t*)nullptr);
     hipLaunchKernelGGL(k_grid_pass_huge<true>, dim3(2048), dim3(256), 0, s, fr, verts, n_tris,
-                       counts, offsets, refs, big_list, big_count);
+                       counts, offsets, refs, big_list, big_count, (uint32_t*)nullptr);
     return hipGetLastError();
 }
 

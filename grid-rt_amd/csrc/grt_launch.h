@@ -22,13 +22,14 @@ hipError_t launch_frame_from_verts(const float* verts, uint32_t n_tris, float la
 hipError_t launch_frame_from_box(const float bmin[3], const float bmax[3], uint32_t n_tris,
                                  float lambda, uint32_t cells_cap, DevFrame* fr, hipStream_t s);
 
-// Count pass: per-cell counts (atomics) and the lists of triangles spanning
+// Count pass: per-cell counts (atomics), per-scan-tile sums into tile_sums
+// (zeroed first; ceil(cells_cap/kScanTile) entries) and the lists of triangles spanning
 // more than 16 cells (medium: from the front of big_list, count in
 // big_count[0]; huge, over 65536 cells: from the back, count in big_count[3]).
 // big_list holds n_tris entries; counts must be zero on entry.
 hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_tris,
                              uint32_t* counts, uint32_t* big_list, uint32_t* big_count,
-                             hipStream_t s);
+                             uint32_t* tile_sums, uint32_t cells_cap, hipStream_t s);
 // Cells the per-cell sort has to visit, collected by the scan: `small` holds
 // the cells of 2..kSortSmallCells references, `large` the larger ones;
 // counts[0], counts[1] their lengths (zeroed by launch_grid_scan).
@@ -47,11 +48,12 @@ struct SortLists {
 
 // Exclusive scan counts[0..m) -> offsets[0..m], total also to fr->r; flags
 // kStatusRefsOver when the total exceeds refs_cap.  block_sums holds
-// ceil(cells_cap/kScanTile) entries.  With lists.small set, also fills the
-// sort lists.
+// ceil(cells_cap/kScanTile) entries: the per-tile sums the count pass left,
+// scanned in place into tile offsets first unless prefixes_ready (a second
+// scan of the same counts).  With lists.small set, also fills the sort lists.
 hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
                             uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
-                            const SortLists& lists, hipStream_t s);
+                            const SortLists& lists, bool prefixes_ready, hipStream_t s);
 // Scatter pass: refs[offsets[c] + k] = t; returns counts to zero.
 hipError_t launch_grid_scatter(const DevFrame* fr, const float* verts, uint32_t n_tris,
                                uint32_t* counts, const uint32_t* offsets, uint32_t* refs,
