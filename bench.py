@@ -61,8 +61,10 @@ def parse():
                         "shadow): s = one ray per lane, w = one ray per lane with pooled "
                         "tests, p = persistent refill, 2/4/8 = lanes per ray; default: tuned")
     p.add_argument("--refill", type=int, default=16)
-    p.add_argument("--ray-lists", action="store_true",
-                   help="compact the secondary stages' active rays into lists and trace those")
+    p.add_argument("--ray-lists", nargs="?", const="ccc", default="---",
+                   help="ray lists of the three secondary stages (shadow, bounce, bounce "
+                        "shadow), one letter each: - = none, c = compacted in slot order; "
+                        "the flag alone means ccc")
     p.add_argument("--coop", type=int, default=16,
                    help="cooperative cell tests in drained persistent waves: minimum "
                         "references left (0 = off)")
@@ -126,7 +128,7 @@ class Frame:
         # ray lists of the three secondary stages (active ray indices + count)
         self.lists = [torch.empty(self.n, dtype=torch.int32, device="cuda:0") for _ in range(3)]
         self.counts = [torch.zeros(1, dtype=torch.int32, device="cuda:0") for _ in range(3)]
-        self.use_lists = False
+        self.lists_kind = "---"
         self.rgb = torch.empty((self.n, 3), dtype=torch.float32, device="cuda:0")
         self.rgba = torch.empty(self.n, dtype=torch.int32, device="cuda:0")
         self.phong = grt.phong({"light": LIGHT})
@@ -184,33 +186,30 @@ class Frame:
         mark(3)
         sc.trace_closest(self.prim, self.hits, image_width=W)
         mark(4)
-        L, C = self.lists, self.counts
+        L, C, K = self.lists, self.counts, self.lists_kind
         sc.gen_shadow(self.prim, self.hits, LIGHT, OFFSET, self.shadow)
-        if self.use_lists:
-            self.ctx.compact_rays(self.shadow, W, L[0], C[0], occluded=self.occ)
+        self.make_list(0, self.shadow, occluded=self.occ)
         self.mode(1)
         mark(5)
-        if self.use_lists:
+        if K[0] != "-":
             sc.trace_any_list(self.shadow, self.occ, L[0], C[0])
         else:
             sc.trace_any(self.shadow, self.occ, image_width=W)
         mark(6)
         sc.gen_reflection(self.prim, self.hits, OFFSET, self.refl)
-        if self.use_lists:
-            self.ctx.compact_rays(self.refl, W, L[1], C[1], miss_hits=self.rhits)
+        self.make_list(1, self.refl, miss_hits=self.rhits)
         self.mode(2)
         mark(7)
-        if self.use_lists:
+        if K[1] != "-":
             sc.trace_closest_list(self.refl, self.rhits, L[1], C[1])
         else:
             sc.trace_closest(self.refl, self.rhits, image_width=W)
         mark(8)
         sc.gen_shadow(self.refl, self.rhits, LIGHT, OFFSET, self.shadow2)
-        if self.use_lists:
-            self.ctx.compact_rays(self.shadow2, W, L[2], C[2], occluded=self.occ2)
+        self.make_list(2, self.shadow2, occluded=self.occ2)
         self.mode(3)
         mark(9)
-        if self.use_lists:
+        if K[2] != "-":
             sc.trace_any_list(self.shadow2, self.occ2, L[2], C[2])
         else:
             sc.trace_any(self.shadow2, self.occ2, image_width=W)
@@ -218,6 +217,13 @@ class Frame:
         sc.shade_phong(self.prim, self.hits, self.rgb, self.occ, 0, self.phong)
         sc.shade_phong(self.refl, self.rhits, self.rgb, self.occ2, 1, self.phong, rgba=self.rgba)
         mark(11)
+
+    def make_list(self, k, rays, **out):
+        kind = self.lists_kind[k]
+        if kind == "c":
+            self.ctx.compact_rays(rays, WIDTH, self.lists[k], self.counts[k], **out)
+        elif kind != "-":
+            raise ValueError("ray list kind %r (expected - or c)" % kind)
 
     SCHEDULES = {"s": "simple", "w": "pooled", "p": "persistent", "2": "split", "4": "split",
                  "8": "split"}
@@ -375,7 +381,9 @@ def main():
         print(json.dumps({"mode": args.mode, **res}))
         return
     frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill)
-    frame.use_lists = args.ray_lists
+    if len(args.ray_lists) != 3:
+        raise SystemExit("--ray-lists takes three letters (shadow, bounce, bounce shadow)")
+    frame.lists_kind = args.ray_lists
     if args.block:
         frame.ctx.set_block_size(args.block, args.block)
     frame.ctx.set_coop(args.coop)
@@ -408,7 +416,7 @@ def main():
                 **trace_ms, "shade": d[10]}
     if args.profile:
         print(json.dumps({"mode": "profile", "lambda": lam, "schedule": sched, "coop": args.coop,
-                          "lists": frame.use_lists,
+                          "lists": frame.lists_kind,
                           "ms_per_step": round(ms_per_step, 3),
                           "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}}))
         return
@@ -476,7 +484,7 @@ def main():
             "stage_counters": counters,
             "traversal": {"schedule_per_stage": sched, "refill": args.refill,
                           "block": args.block or 128, "lambda": lam, "coop": args.coop,
-                          "secondary_ray_lists": frame.use_lists},
+                          "secondary_ray_lists": frame.lists_kind},
             "autotune_frame_ms_by_lambda": tune_table,
             "parallelism": "single GPU",
         },

@@ -6,8 +6,8 @@
 //         UniformGridAccStruct.cpp:44-63)
 //   2. k_grid_pass<false>       : per-cell reference counts
 //        (UniformGridAccStructBuilder.cpp:108-145; countCellVP.glsl:31-58)
-//   3. k_scan_*                 : exclusive scan of the counts (DPP wave scan,
-//        LDS block scan, reduce-then-scan across blocks)
+//   3. k_scan_*                 : exclusive scan of the counts (the count pass
+//        also sums each 4096-cell tile; DPP wave scan, LDS block scan)
 //   4. k_grid_pass<true>        : scatter triangle ids into their cells
 //        (gridVP_sem_GS.glsl:56-106)
 //   5. k_sort_small/large       : every cell's ids ascending, the order the
