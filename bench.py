@@ -61,10 +61,11 @@ def parse():
                         "shadow): s = one ray per lane, w = one ray per lane with pooled "
                         "tests, p = persistent refill, 2/4/8 = lanes per ray; default: tuned")
     p.add_argument("--refill", type=int, default=16)
-    p.add_argument("--ray-lists", nargs="?", const="ccc", default="---",
+    p.add_argument("--ray-lists", nargs="?", const="ccc", default=None,
                    help="ray lists of the three secondary stages (shadow, bounce, bounce "
                         "shadow), one letter each: - = none, c = compacted in slot order; "
-                        "the flag alone means ccc")
+                        "the flag alone means ccc; default: tuned with the schedule "
+                        "(--- when --sched is given)")
     p.add_argument("--coop", type=int, default=16,
                    help="cooperative cell tests in drained persistent waves: minimum "
                         "references left (0 = off)")
@@ -240,11 +241,13 @@ class Frame:
         return int((f["tmin"] <= f["tmax"]).sum().item())
 
 
-def autotune(frame, lams, sched):
-    """Pick lambda and the per-stage traversal schedule on the frame itself
-    (outside the timed region): for each lambda, time every stage under each
-    schedule and keep the fastest per stage."""
-    from grt.autotune import Autotuner, per_stage_schedule
+def autotune(frame, lams, sched, lists):
+    """Pick lambda, the per-stage traversal schedule and the secondary stages'
+    ray lists on the frame itself (outside the timed region): for each lambda,
+    time every stage under each schedule, with and without ray lists (each
+    stage's time includes its ray generation and list compaction), and keep the
+    fastest per stage."""
+    from grt.autotune import Autotuner
     tuner = Autotuner()
     table = {}
 
@@ -252,21 +255,30 @@ def autotune(frame, lams, sched):
         frame.set_lambda(c["lam"])
         if sched:
             frame.sched = sched
+            frame.lists_kind = lists or "---"
             ms = frame.time_stages()
-            table[c["lam"]] = {"sched": sched, "frame_ms": round(sum(ms), 3)}
+            table[c["lam"]] = {"sched": sched, "lists": frame.lists_kind,
+                               "frame_ms": round(sum(ms), 3)}
             return sum(ms)
         by = {}
         for ch in Frame.SCHEDULES:
-            frame.sched = ch * 4
-            by[ch] = frame.time_stages()
-        best = per_stage_schedule({ch: ms[2:] for ch, ms in by.items()})
+            for kind in (("-",) if lists else ("-", "c")):
+                frame.sched = ch * 4
+                frame.lists_kind = lists or kind * 3
+                by[(ch, kind)] = frame.time_stages()
+        # primary rays are never listed; each secondary stage takes its best pair
+        picks = [min((k for k in by if i > 0 or k[1] == "-"), key=lambda k: by[k][2 + i])
+                 for i in range(4)]
+        best = "".join(ch for ch, _ in picks)
+        best_lists = lists or "".join(kind for _, kind in picks[1:])
         total = sum(sum(ms[:2]) for ms in by.values()) / len(by) + \
-            sum(by[ch][2 + i] for i, ch in enumerate(best))
-        table[c["lam"]] = {"sched": best, "frame_ms": round(total, 3)}
+            sum(by[k][2 + i] for i, k in enumerate(picks))
+        table[c["lam"]] = {"sched": best, "lists": best_lists, "frame_ms": round(total, 3)}
         return total
 
     best = tuner.tune("stadium-1M", [{"lam": lam} for lam in lams], measure)
-    return best["lam"], table[best["lam"]]["sched"], table
+    entry = table[best["lam"]]
+    return best["lam"], entry["sched"], entry["lists"], table
 
 
 def stage_counters(frame, torch):
@@ -381,17 +393,19 @@ def main():
         print(json.dumps({"mode": args.mode, **res}))
         return
     frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill)
-    if len(args.ray_lists) != 3:
-        raise SystemExit("--ray-lists takes three letters (shadow, bounce, bounce shadow)")
-    frame.lists_kind = args.ray_lists
+    if args.ray_lists is not None and (len(args.ray_lists) != 3 or
+                                       set(args.ray_lists) - set("-c")):
+        raise SystemExit("--ray-lists takes three letters of - and c (shadow, bounce, "
+                         "bounce shadow)")
     if args.block:
         frame.ctx.set_block_size(args.block, args.block)
     frame.ctx.set_coop(args.coop)
     if args.copy_calibration:
         copy_calibration(torch)
     lams = [args.lam] if args.lam else list(LAMBDAS)
-    lam, sched, tune_table = autotune(frame, lams, args.sched)
+    lam, sched, lists, tune_table = autotune(frame, lams, args.sched, args.ray_lists)
     frame.sched = sched
+    frame.lists_kind = lists
     frame.set_lambda(lam)
     for w in range(args.warmup):
         frame.run(1000 + w)
