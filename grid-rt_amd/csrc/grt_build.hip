@@ -611,6 +611,24 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
     if (base >= n)
         return;
     const uint32_t i0 = base + threadIdx.x * 16u;
+    // A tile whose references sum to zero (bsum holds exclusive tile prefixes,
+    // out[n] the total) has only empty cells: every offset is the tile's
+    // prefix, and its counts need not be read.
+    const uint32_t pre = bsum[blockIdx.x];
+    const uint32_t next = base + kScanTile < n ? bsum[blockIdx.x + 1] : out[n];
+    if (next == pre) {
+        if (i0 + 16 <= n) {
+            uint4* q = reinterpret_cast<uint4*>(out + i0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                q[k] = make_uint4(pre, pre, pre, pre);
+        } else {
+            for (uint32_t k = 0; k < 16; ++k)
+                if (i0 + k < n)
+                    out[i0 + k] = pre;
+        }
+        return;
+    }
     uint32_t v[16];
     load16(in, i0, n, v);
     uint32_t s = 0;
