@@ -617,11 +617,12 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
     const uint32_t pre = bsum[blockIdx.x];
     const uint32_t next = base + kScanTile < n ? bsum[blockIdx.x + 1] : out[n];
     if (next == pre) {
-        if (i0 + 16 <= n) {
-            uint4* q = reinterpret_cast<uint4*>(out + i0);
+        if (base + kScanTile <= n) {
+            // Lane-contiguous 16-byte stores: each store of a wave covers 1 KB.
+            uint4* q = reinterpret_cast<uint4*>(out + base) + threadIdx.x;
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                q[k] = make_uint4(pre, pre, pre, pre);
+                q[256 * k] = make_uint4(pre, pre, pre, pre);
         } else {
             for (uint32_t k = 0; k < 16; ++k)
                 if (i0 + k < n)
@@ -1196,16 +1197,31 @@ __global__ __launch_bounds__(256) void k_gather_refs(const DevFrame* __restrict_
                                                      const float4* __restrict__ tri,
                                                      float4* __restrict__ trir)
 {
+    // Each wave takes 64 consecutive references at a time: lane l gathers the
+    // record of reference k0 + l into the wave's LDS slice, then the wave
+    // writes the 64 records (3 KB, contiguous in trir) with lane-contiguous
+    // 16-byte stores.
+    __shared__ float4 stage[4][192];
     if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
         return;
     const uint64_t r_total = fr->r;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < r_total;
-         k += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t t = refs[k];
-        const float4 a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
-        trir[3 * k] = a;
-        trir[3 * k + 1] = b;
-        trir[3 * k + 2] = c;
+    const uint32_t lane = threadIdx.x & 63u;
+    float4* st = stage[threadIdx.x >> 6];
+    const uint64_t stride = (uint64_t)gridDim.x * (blockDim.x >> 6) * 64u;
+    for (uint64_t k0 = (blockIdx.x * (uint64_t)(blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
+         k0 < r_total; k0 += stride) {
+        const uint64_t k = k0 + lane;
+        if (k < r_total) {
+            const uint64_t t = refs[k];
+            st[3 * lane] = tri[3 * t];
+            st[3 * lane + 1] = tri[3 * t + 1];
+            st[3 * lane + 2] = tri[3 * t + 2];
+        }
+        wave_lds_sync();
+        const uint32_t m = 3u * (uint32_t)(r_total - k0 < 64u ? r_total - k0 : 64u);
+        for (uint32_t j = lane; j < m; j += 64u)
+            trir[3 * k0 + j] = st[j];
+        wave_lds_sync();
     }
 }
 

@@ -142,6 +142,15 @@ __device__ __forceinline__ uint32_t slot_ray(uint32_t slot, uint32_t img_w, uint
     return i;
 }
 
+// Lanes of one wave exchange data through LDS: order the accesses around this
+// point (a wave's LDS operations execute in program order).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Slots covering n rays: n, or whole 8x8 tiles of an image of width img_w.
 inline uint64_t slot_count(uint32_t n, uint32_t img_w)
 {

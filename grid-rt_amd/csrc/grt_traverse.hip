@@ -414,15 +414,6 @@ struct WavePool {
     unsigned long long win[64];      // per owner: (t, reference) minimum / first occluder
 };
 
-// Lanes of one wave exchange data through LDS: order the accesses around this
-// point (a wave's LDS operations execute in program order).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ uint32_t ordered_bits(float t)
 {
     const uint32_t b = __float_as_uint(t == 0.0f ? 0.0f : t);
