@@ -630,8 +630,29 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
         }
         return;
     }
+    // Whole tiles move through LDS so that global loads and stores are
+    // lane-contiguous (each 16-byte access of a wave covers 1 KB); thread t
+    // then scans cells [16t, 16t + 16) of the tile from LDS.
+    __shared__ uint4 tile[kScanTile / 4];
+    const bool whole = base + kScanTile <= n;
     uint32_t v[16];
-    load16(in, i0, n, v);
+    if (whole) {
+        const uint4* src = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            tile[256 * k + threadIdx.x] = src[256 * k + threadIdx.x];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 x = tile[4 * threadIdx.x + k];
+            v[4 * k] = x.x;
+            v[4 * k + 1] = x.y;
+            v[4 * k + 2] = x.z;
+            v[4 * k + 3] = x.w;
+        }
+    } else {
+        load16(in, i0, n, v);
+    }
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k)
@@ -674,7 +695,16 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
         o[k] = p;
         p += v[k];
     }
-    if (i0 + 16 <= n) {
+    if (whole) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            tile[4 * threadIdx.x + k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+        __syncthreads();
+        uint4* dst = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            dst[256 * k + threadIdx.x] = tile[256 * k + threadIdx.x];
+    } else if (i0 + 16 <= n) {
         uint4* q = reinterpret_cast<uint4*>(out + i0);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
