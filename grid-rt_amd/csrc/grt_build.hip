@@ -21,6 +21,7 @@ namespace grt {
 
 constexpr uint32_t kSmallCells = 64;
 constexpr uint32_t kHugeCells = 65536;
+constexpr uint32_t kHugeMerge = 4;   // huge triangles applied together in one cell walk
 constexpr uint32_t kSortTile = 16384;   // LDS tile of the per-cell sort (64 KB)
 
 // ---------------------------------------------------------------------------
@@ -252,6 +253,25 @@ __device__ __forceinline__ void tile_add_wave(uint32_t* tile_sums, bool active, 
         atomicAdd(&tile_sums[tile], 1u);
 }
 
+// As tile_add_wave, adding m (per lane) instead of 1.
+__device__ __forceinline__ void tile_add_wave_n(uint32_t* tile_sums, bool active, uint32_t c,
+                                                uint32_t m)
+{
+    const uint64_t am = __ballot(active);
+    if (am == 0)
+        return;
+    const uint32_t f = (uint32_t)__builtin_ctzll(am);
+    const uint32_t tile = c >> kScanTileShift;
+    const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)tile, (int)f);
+    const bool same = active && tile == t0;
+    const uint32_t sum =
+        (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(same ? m : 0u), 63);
+    if ((threadIdx.x & 63u) == f)
+        atomicAdd(&tile_sums[t0], sum);
+    if (active && !same)
+        atomicAdd(&tile_sums[tile], m);
+}
+
 template <bool SCATTER>
 __device__ __forceinline__ void cell_op(uint32_t c, uint32_t t, uint32_t* counts,
                                         const uint32_t* offsets, uint32_t* refs,
@@ -446,6 +466,75 @@ __global__ __launch_bounds__(256) void k_grid_pass_huge(const DevFrame* __restri
         return;
     const uint32_t nh = big_count[3];
     const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    if (nh > 0 && nh <= kHugeMerge) {
+        // Few huge triangles (ground planes) whose boxes overlap: walk the
+        // union of their boxes once, one thread per cell and blocks over
+        // whole rows, and apply every huge triangle of the cell with plain
+        // loads and stores -- no other thread touches the cell in this kernel.
+        CellBox bx[kHugeMerge];
+        uint32_t tid[kHugeMerge];
+        int32_t ulo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, uhi[3] = {-1, -1, -1};
+        uint64_t vol = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kHugeMerge; ++i) {
+            tid[i] = 0;
+            bx[i] = CellBox{{0, 0, 0}, {0, 0, 0}};
+            if (i < nh) {
+                tid[i] = big_list[n_tris - 1u - i];
+                bx[i] = tri_cells(fr, verts + 9ull * tid[i]);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    ulo[d] = bx[i].lo[d] < ulo[d] ? bx[i].lo[d] : ulo[d];
+                    const int32_t h = bx[i].lo[d] + bx[i].w[d] - 1;
+                    uhi[d] = h > uhi[d] ? h : uhi[d];
+                }
+                vol += (uint64_t)bx[i].w[0] * (uint64_t)bx[i].w[1] * (uint64_t)bx[i].w[2];
+            }
+        }
+        const uint32_t wx = (uint32_t)(uhi[0] - ulo[0] + 1), wy = (uint32_t)(uhi[1] - ulo[1] + 1),
+                       wz = (uint32_t)(uhi[2] - ulo[2] + 1);
+        if ((uint64_t)wx * wy * wz <= 2 * vol) {
+            const uint32_t rows = wy * wz;
+            for (uint32_t row = blockIdx.x; row < rows; row += gridDim.x) {
+                const uint32_t y = (uint32_t)ulo[1] + row % wy, z = (uint32_t)ulo[2] + row / wy;
+                const uint32_t row_c = nx * (y + ny * z);
+                for (uint32_t x0 = 0; x0 < wx; x0 += blockDim.x) {
+                    const uint32_t x = (uint32_t)ulo[0] + x0 + threadIdx.x;
+                    const bool in_row = x0 + threadIdx.x < wx;
+                    bool hit[kHugeMerge];
+                    uint32_t m = 0;
+#pragma unroll
+                    for (uint32_t i = 0; i < kHugeMerge; ++i) {
+                        hit[i] = in_row && i < nh &&
+                                 (uint32_t)((int32_t)x - bx[i].lo[0]) < (uint32_t)bx[i].w[0] &&
+                                 (uint32_t)((int32_t)y - bx[i].lo[1]) < (uint32_t)bx[i].w[1] &&
+                                 (uint32_t)((int32_t)z - bx[i].lo[2]) < (uint32_t)bx[i].w[2];
+                        m += hit[i] ? 1u : 0u;
+                    }
+                    const uint32_t c = x + row_c;
+                    if (SCATTER) {
+                        if (m != 0) {
+                            const uint32_t left = counts[c], off = offsets[c];
+                            uint32_t q = 0;
+#pragma unroll
+                            for (uint32_t i = 0; i < kHugeMerge; ++i) {
+                                if (hit[i]) {
+                                    refs[off + left - 1u - q] = tid[i];
+                                    ++q;
+                                }
+                            }
+                            counts[c] = left - m;
+                        }
+                    } else {
+                        if (m != 0)
+                            counts[c] += m;
+                        tile_add_wave_n(tile_sums, m != 0, c, m);
+                    }
+                }
+            }
+            return;
+        }
+    }
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = 0; i < nh; ++i) {
         const uint32_t t = big_list[n_tris - 1u - i];
@@ -574,13 +663,29 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* __restrict__ bsu
         return;
     const uint32_t nb = (n + kScanTile - 1) / kScanTile;
     uint64_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += 1024) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nb ? bsum[i] : 0u;
+    // 4096 sums per round, four consecutive ones per thread.
+    for (uint32_t base = 0; base < nb; base += 4096) {
+        const uint32_t i = base + 4u * threadIdx.x;
+        uint32_t v[4] = {0u, 0u, 0u, 0u};
+        const bool vec = i + 4 <= nb;
+        if (vec) {
+            const uint4 x = *reinterpret_cast<const uint4*>(bsum + i);
+            v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w;
+        } else {
+            for (uint32_t k = 0; k < 4; ++k)
+                if (i + k < nb)
+                    v[k] = bsum[i + k];
+        }
         uint32_t tot;
-        const uint32_t ex = block_excl_scan<1024>(v, lds, tot);
-        if (i < nb)
-            bsum[i] = (uint32_t)carry + ex;
+        const uint32_t ex = (uint32_t)carry + block_excl_scan<1024>(v[0] + v[1] + v[2] + v[3], lds, tot);
+        const uint32_t o[4] = {ex, ex + v[0], ex + v[0] + v[1], ex + v[0] + v[1] + v[2]};
+        if (vec) {
+            *reinterpret_cast<uint4*>(bsum + i) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+            for (uint32_t k = 0; k < 4; ++k)
+                if (i + k < nb)
+                    bsum[i + k] = o[k];
+        }
         carry += tot;
     }
     if (threadIdx.x == 0) {

@@ -217,6 +217,35 @@ def test_grid_build_stadium_small(gpu_ctx):
     check_grid(gpu_ctx, v, n, 5.0)
 
 
+def _huge_scene(kind):
+    """A flat slab of small triangles plus triangles spanning more than 65,536
+    cells at lambda 64: a ground pair (boxes overlap: one merged cell walk),
+    planes at two heights (their union box is mostly empty) and five planes
+    (more than are merged); the last two take the per-triangle path."""
+    r = rng(11)
+    p = r.random((20000, 3), dtype=np.float32)
+    p[:, 1] *= 0.05
+    small = np.tile(p, 3) + (r.random((20000, 9), dtype=np.float32) - 0.5) * 0.01
+    def plane(y, flip=False):
+        a = [[0, y, 0, 1, y, 0, 1, y, 1], [0, y, 0, 1, y, 1, 0, y, 1]]
+        return np.array(a[1:] + a[:1] if flip else a, np.float32)
+    if kind == "ground_pair":
+        big = plane(0.0)
+    elif kind == "two_levels":
+        big = np.vstack([plane(0.0), plane(0.05)])
+    else:
+        big = np.vstack([plane(0.0), plane(0.0, True), plane(0.0)[:1]])
+    return np.vstack([small, big]).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["ground_pair", "two_levels", "five_planes"])
+def test_grid_build_huge_triangles(gpu_ctx, kind):
+    v = _huge_scene(kind)
+    fr = orc.grid_frame(v, 64.0)
+    assert fr["n"][0] * fr["n"][2] > 65536, fr["n"]
+    check_grid(gpu_ctx, v, np.zeros_like(v), 64.0)
+
+
 def test_grid_build_stadium_full(gpu_ctx):
     """Config 2 size: 2^20 triangles, 256 clusters + ground plane."""
     v, n, _ = grt.gen_stadium(SEED_C2, 256, 4096, (1 << 20) - 2)
