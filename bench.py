@@ -455,11 +455,14 @@ def main():
     dk = by_kernel[dom]
     launches = len(dk["stages"])
     achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
+    # DRAM bytes per launch of the dominant kernel at this lambda, from the
+    # committed FETCH_SIZE / WRITE_SIZE profile (null if it has no such line).
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "trace_stadium_traffic.md")
     if os.path.exists(tpath):
+        key = "hbm_bytes_per_launch[lambda=%g][%s]:" % (lam, dom)
         for line in open(tpath):
-            if line.startswith("hbm_bytes_per_launch[%s]:" % dom):
+            if line.startswith(key):
                 traffic = float(line.split(":", 1)[1])
     per_stage_roof = {}
     for name, _ in STAGES:
