@@ -441,26 +441,20 @@ def main():
     info = frame.scene.grid_info()
     counters = stage_counters(frame, torch)
 
-    # Roofline of the dominant traversal kernel: the kernel (by name) with the
-    # most time per frame; achieved = its algorithmic bytes per launch over its
-    # average launch duration, both averaged over its launches in a frame.
-    by_kernel = {}
-    for i, (name, closest) in enumerate(STAGES):
-        k = kernel_name(closest, sched[i])
-        e = by_kernel.setdefault(k, {"stages": [], "ms": 0.0, "bytes": 0})
-        e["stages"].append(name)
-        e["ms"] += trace_ms[name]
-        e["bytes"] += counters[name]["algorithmic_bytes"]
-    dom = max(by_kernel, key=lambda k: by_kernel[k]["ms"])
-    dk = by_kernel[dom]
-    launches = len(dk["stages"])
-    achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
-    # DRAM bytes per launch of the dominant kernel at this lambda, from the
-    # committed FETCH_SIZE / WRITE_SIZE profile (null if it has no such line).
+    # Roofline of the dominant traversal launch: the stage whose traversal
+    # launch takes the most time per frame (one launch per stage and frame);
+    # achieved = that stage's algorithmic bytes over its launch duration.
+    dom_stage = max(trace_ms, key=trace_ms.get)
+    si = [name for name, _ in STAGES].index(dom_stage)
+    dom = kernel_name(STAGES[si][1], sched[si])
+    dom_bytes = counters[dom_stage]["algorithmic_bytes"]
+    achieved = dom_bytes / (trace_ms[dom_stage] * 1e-3) / 1e9
+    # DRAM bytes of that launch at this lambda, from the committed
+    # FETCH_SIZE / WRITE_SIZE profile (null if it has no such line).
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "trace_stadium_traffic.md")
     if os.path.exists(tpath):
-        key = "hbm_bytes_per_launch[lambda=%g][%s]:" % (lam, dom)
+        key = "hbm_bytes_per_launch[lambda=%g][%s][%s]:" % (lam, dom_stage, dom)
         for line in open(tpath):
             if line.startswith(key):
                 traffic = float(line.split(":", 1)[1])
@@ -508,9 +502,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "grt::" + dom, "stages": dk["stages"],
-                     "avg_launch_ms": round(dk["ms"] / launches, 3),
-                     "algorithmic_bytes_per_launch": dk["bytes"] // launches},
+                     "kernel": "grt::" + dom, "stage": dom_stage,
+                     "avg_launch_ms": round(trace_ms[dom_stage], 3),
+                     "algorithmic_bytes_per_launch": dom_bytes},
     }
     if not args.no_serve:
         # p50/p99 request latency, continuous batching vs the plain path, on
