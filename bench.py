@@ -268,9 +268,14 @@ def autotune(frame, lams, sched, lists):
                 frame.sched = ch * 4
                 frame.lists_kind = lists or kind * 3
                 by[(ch, kind)] = frame.time_stages(frames=3)
-        # primary rays are never listed; each secondary stage takes its best pair
-        picks = [min((k for k in by if i > 0 or k[1] == "-"), key=lambda k: by[k][2 + i])
-                 for i in range(4)]
+        # primary rays are never listed; each secondary stage takes its best
+        # pair, a ray list only when it beats the same schedule without one by 2%
+        picks = []
+        for i in range(4):
+            k = min((k for k in by if i > 0 or k[1] == "-"), key=lambda k: by[k][2 + i])
+            if k[1] == "c" and by[(k[0], "-")][2 + i] <= 1.02 * by[k][2 + i]:
+                k = (k[0], "-")
+            picks.append(k)
         best = "".join(ch for ch, _ in picks)
         best_lists = lists or "".join(kind for _, kind in picks[1:])
         total = sum(sum(ms[:2]) for ms in by.values()) / len(by) + \
