@@ -55,6 +55,20 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
     return res;
 }
 
+// Streaming 16-byte store that is not kept in the caches (the grid's dense
+// arrays are written once per build and read later by traversal).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void stream_store(uint4* p, uint4 v)
+{
+    __builtin_nontemporal_store(u32x4{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4*>(p));
+}
+__device__ __forceinline__ void stream_store(float4* p, float4 v)
+{
+    stream_store(reinterpret_cast<uint4*>(p),
+                 make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
+                            __float_as_uint(v.w)));
+}
+
 // ---------------------------------------------------------------------------
 // 1. Frame
 // ---------------------------------------------------------------------------
@@ -727,7 +741,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
             uint4* q = reinterpret_cast<uint4*>(out + base) + threadIdx.x;
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                q[256 * k] = make_uint4(pre, pre, pre, pre);
+                stream_store(q + 256 * k, make_uint4(pre, pre, pre, pre));
         } else {
             for (uint32_t k = 0; k < 16; ++k)
                 if (i0 + k < n)
@@ -808,7 +822,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
         uint4* dst = reinterpret_cast<uint4*>(out + base);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            dst[256 * k + threadIdx.x] = tile[256 * k + threadIdx.x];
+            stream_store(dst + 256 * k + threadIdx.x, tile[256 * k + threadIdx.x]);
     } else if (i0 + 16 <= n) {
         uint4* q = reinterpret_cast<uint4*>(out + i0);
 #pragma unroll
@@ -1355,7 +1369,7 @@ __global__ __launch_bounds__(256) void k_gather_refs(const DevFrame* __restrict_
         wave_lds_sync();
         const uint32_t m = 3u * (uint32_t)(r_total - k0 < 64u ? r_total - k0 : 64u);
         for (uint32_t j = lane; j < m; j += 64u)
-            trir[3 * k0 + j] = st[j];
+            stream_store(trir + 3 * k0 + j, st[j]);
         wave_lds_sync();
     }
 }
