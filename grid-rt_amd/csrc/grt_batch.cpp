@@ -155,9 +155,8 @@ int grt_batch_trace(grt_batch b)
     grt_context_t* ctx = b->ctx;
     return hip_check(launch_trace_batch(b->tasks.as<BatchTask>(), b->map.as<uint32_t>(),
                                         b->total_slots, b->counter.as<uint32_t>(),
-                                        ctx->trace_mode,
-                                        ctx->block_closest, ctx->refill, ctx->coop,
-                                        ctx->stream),
+                                        ctx->trace_mode, ctx->block_closest, ctx->refill,
+                                        ctx->split_lanes, ctx->coop, ctx->stream),
                      "grt_batch_trace");
 }
 

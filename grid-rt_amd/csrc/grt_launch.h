@@ -136,11 +136,11 @@ hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* m
                            hipStream_t s);
 // Multi-scene traversal over all tasks (mixed closest / any hit) with the
 // schedule `mode` (GRT_TRACE_*): one ray per lane (each wave in one task), with
-// per-lane or pooled tests, or persistent waves refilled from the slot counter.
+// per-lane or pooled tests; `lanes` lanes per ray (2, 4, 8 or 16); or
+// persistent waves refilled from the slot counter.
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
                               uint32_t* slot_counter, int mode, uint32_t block,
-                              uint32_t refill, uint32_t coop,
-                              hipStream_t s);
+                              uint32_t refill, uint32_t lanes, uint32_t coop, hipStream_t s);
 hipError_t launch_gen_primary(const DevCamera& cam, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_gen_shadow(uint32_t n, const RaySoA& in, const HitSoAIn& hits,
                              const float* normals, float lx, float ly, float lz, float offset,

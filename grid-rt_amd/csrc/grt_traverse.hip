@@ -992,6 +992,32 @@ __global__ __launch_bounds__(256) void k_trace_split(TraceArgs a, RaySoA rays, H
                                  a.coop, lanes, i, valid);
 }
 
+// Batched multi-scene traversal with `lanes` lanes per ray (64 / lanes rays
+// per wave): ray slot s is traced by threads [s * lanes, (s + 1) * lanes).  A
+// wave covers 64 / lanes consecutive slots inside one 64-slot group, so one
+// task, and the task's closest/any choice is uniform over the wave.
+__global__ __launch_bounds__(256) void k_trace_batch_split(const BatchTask* __restrict__ tasks,
+                                                           const uint32_t* __restrict__ map,
+                                                           uint32_t total_slots, uint32_t coop,
+                                                           uint32_t lanes)
+{
+    const uint32_t shift = (uint32_t)__builtin_ctz(lanes);
+    const uint32_t slot = (blockIdx.x * blockDim.x + threadIdx.x) >> shift;
+    const uint32_t group = __builtin_amdgcn_readfirstlane(slot >> 6);
+    if ((group << 6) >= total_slots)
+        return;   // whole waves only: total_slots is a multiple of 64
+    const uint32_t task = __builtin_amdgcn_readfirstlane(map[group]);
+    const BatchTask& T = tasks[task];
+    bool valid;
+    const uint32_t i = slot_ray(slot - T.slot_begin, T.image_width, T.n, valid);
+    if (T.any)
+        trace_split_lane<true, false>(T.fr, T.offsets, T.trir, T.rays, T.hits, T.occluded, nullptr,
+                                      coop, lanes, i, valid);
+    else
+        trace_split_lane<false, false>(T.fr, T.offsets, T.trir, T.rays, T.hits, T.occluded,
+                                       nullptr, coop, lanes, i, valid);
+}
+
 // ---------------------------------------------------------------------------
 // Persistent waves with lane refill
 // ---------------------------------------------------------------------------
@@ -1489,10 +1515,20 @@ hipError_t launch_task_map(const BatchTask* tasks, uint32_t n_tasks, uint32_t* m
 
 hipError_t launch_trace_batch(const BatchTask* tasks, const uint32_t* map, uint32_t total_slots,
                               uint32_t* slot_counter, int mode, uint32_t block,
-                              uint32_t refill, uint32_t coop, hipStream_t s)
+                              uint32_t refill, uint32_t lanes, uint32_t coop, hipStream_t s)
 {
     if (total_slots == 0)
         return hipSuccess;
+    if (mode == GRT_TRACE_SPLIT) {
+        if (lanes < 2 || lanes > 16 || (lanes & (lanes - 1)) != 0)
+            return hipErrorInvalidValue;
+        const uint64_t threads = (uint64_t)total_slots * lanes;
+        if (threads > 0xffffffffull - block)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_trace_batch_split, dim3((uint32_t)((threads + block - 1) / block)),
+                           dim3(block), 0, s, tasks, map, total_slots, coop, lanes);
+        return hipGetLastError();
+    }
     if (mode != GRT_TRACE_PERSISTENT) {
         const dim3 g((total_slots + block - 1) / block);
         if (mode == GRT_TRACE_POOLED)

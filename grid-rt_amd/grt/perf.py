@@ -229,9 +229,10 @@ def _time(fn, reps, torch):
 
 def batch_config(reps: int = 5):
     """Batched (one launch per level, hipGraph) vs plain; the autotuner picks
-    the schedule (one ray per lane, pooled or persistent), lambda in {2,5,8},
-    block size and refill threshold.  The plain path gets its best of the
-    single-scene schedules, the split one (4 lanes per ray) included."""
+    the schedule (one ray per lane, pooled, persistent, or 2/4/8 lanes per ray,
+    i.e. 64/32/16/8 rays per wave), lambda in {2,5,8}, block size and refill
+    threshold.  The plain path gets its best of the single-scene schedules, the
+    split one (4 lanes per ray) included."""
     torch = _torch()
     ctx = Context(0)
     stream = stream_create()
@@ -244,6 +245,9 @@ def batch_config(reps: int = 5):
              for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256) for m in ("simple", "pooled")]
     cands += [{"lam": lam, "block": blk, "mode": "persistent", "refill": rf}
               for lam in (2.0, 5.0, 8.0) for blk in (64, 128, 256) for rf in (8, 16, 32)]
+    # several lanes per ray: 32, 16 or 8 rays per wave ("refill" carries the lanes)
+    cands += [{"lam": lam, "block": blk, "mode": "split", "refill": lanes}
+              for lam in (2.0, 5.0, 8.0) for blk in (128, 256) for lanes in (2, 4, 8)]
     cur = {"lam": 5.0}
 
     def measure(c):
@@ -290,6 +294,7 @@ def batch_config(reps: int = 5):
     out = {"jobs": len(jobs), "triangles": int(sum(j["tris"] for j in bj.jobs)),
            "primary_rays": int(sum(j["n"] for j in bj.jobs)), "rays_per_run": rays,
            "tuned": best,
+           "rays_per_wave": 64 // best["refill"] if best["mode"] == "split" else 64,
            "batched_graph_ms": round(t_graph * 1e3, 3), "batched_eager_ms": round(t_eager * 1e3, 3),
            "plain_ms": round(t_plain * 1e3, 3),
            "plain_ms_by_schedule": {k: round(v * 1e3, 3) for k, v in t_plain_by.items()},

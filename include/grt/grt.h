@@ -94,9 +94,9 @@ int grt_set_block_size(grt_context ctx, uint32_t closest, uint32_t any);
  * too few rays to fill the GPU.  For SIMPLE and POOLED `param` is kept as the
  * refill threshold (1..64) of later persistent launches.  The per-lane
  * schedules finish dense cells wave-cooperatively (grt_set_coop).  Results and
- * counters are identical; only the schedule differs.  The serving loop follows
- * the setting; batches (grt_batch_trace) follow it too, except that they run
- * GRT_TRACE_SPLIT as GRT_TRACE_SIMPLE. */
+ * counters are identical; only the schedule differs.  The serving loop and
+ * batches (grt_batch_trace) follow the setting; a batch under GRT_TRACE_SPLIT
+ * gives every ray of every item `param` lanes (64 / param rays per wave). */
 #define GRT_TRACE_SIMPLE     0
 #define GRT_TRACE_PERSISTENT 1
 #define GRT_TRACE_POOLED     2

@@ -69,11 +69,13 @@ def test_batch_items_match_oracle(gpu_ctx, three_scenes):
             items.append((sc, rb, hb, None, width))
             outs.append(hb)
             want.append(orc.trace_closest(fr, offs, refs, v, rays))
-    # both batched kernels (one ray per lane, persistent), cooperative pass on
-    # and off, two item orders
-    for mode, coop, rev in (("simple", 16, False), ("simple", 0, True), ("pooled", 16, True),
-                            ("persistent", 16, True), ("persistent", 1, False)):
-        gpu_ctx.set_trace_mode(mode, 16)
+    # every batched kernel (one ray per lane, persistent, several lanes per
+    # ray), cooperative pass on and off, two item orders
+    for mode, param, coop, rev in (("simple", 16, 16, False), ("simple", 16, 0, True),
+                                   ("pooled", 16, 16, True), ("persistent", 16, 16, True),
+                                   ("persistent", 1, 1, False), ("split", 2, 16, False),
+                                   ("split", 8, 0, True), ("split", 16, 16, False)):
+        gpu_ctx.set_trace_mode(mode, param)
         gpu_ctx.set_coop(coop)
         order = list(range(len(items)))
         if rev:
@@ -88,7 +90,7 @@ def test_batch_items_match_oracle(gpu_ctx, three_scenes):
         assert batch.slots() % 64 == 0
         batch.trace()
         torch.cuda.synchronize()
-        tag = f"[{mode} coop={coop}]"
+        tag = f"[{mode} param={param} coop={coop}]"
         for k, (spec, out, w) in enumerate(zip(specs, outs, want)):
             if spec[2]:
                 assert_same(out.cpu().numpy(), w, f"item {k} occluded {tag}")
@@ -180,9 +182,10 @@ def test_serve_result_same_alone_and_in_mix(gpu_ctx, three_scenes):
 
     mix, res_mix = make(reqs, True, [0.0] * len(reqs))
     assert np.all(res_mix[2] >= res_mix[1]) and np.all(res_mix[3] > 0)
-    # the persistent and pooled batched kernels serve the same mix identically
-    for mode in ("persistent", "pooled"):
-        gpu_ctx.set_trace_mode(mode, 16)
+    # the persistent, pooled and split batched kernels serve the same mix
+    # identically
+    for mode, param in (("persistent", 16), ("pooled", 16), ("split", 4)):
+        gpu_ctx.set_trace_mode(mode, param)
         try:
             mixp, _ = make(reqs, True, [0.0] * len(reqs))
         finally:
