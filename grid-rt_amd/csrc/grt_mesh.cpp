@@ -363,10 +363,34 @@ int grt_mesh_end(grt_mesh m)
     return GRT_OK;
 }
 
+namespace {
+
+int load_obj(grt_mesh m, const char* path, uint32_t flags);
+
+}  // namespace
+
 int grt_mesh_load_obj(grt_mesh m, const char* path, uint32_t flags)
 {
     if (!mesh_ok(m, "grt_mesh_load_obj") || !path)
         return set_error(GRT_INVALID_PARAM, "grt_mesh_load_obj: NULL argument");
+    // All or nothing: a file refused part way leaves the mesh as it was.
+    const size_t kept = m->source.size();
+    const uint32_t assembled = m->assembled;
+    const int rc = load_obj(m, path, flags);
+    if (rc != GRT_OK) {
+        m->verts.resize(9 * kept);
+        m->normals.resize(9 * kept);
+        m->diffuse.resize(3 * kept);
+        m->source.resize(kept);
+        m->assembled = assembled;
+    }
+    return rc;
+}
+
+namespace {
+
+int load_obj(grt_mesh m, const char* path, uint32_t flags)
+{
     std::ifstream in(path);
     if (!in)
         return set_error(GRT_INVALID_PARAM, "grt_mesh_load_obj: cannot open %s", path);
@@ -426,6 +450,8 @@ int grt_mesh_load_obj(grt_mesh m, const char* path, uint32_t flags)
     }
     return GRT_OK;
 }
+
+}  // namespace
 
 // rtdb::loadSphere (rtdb.cpp:142-192): stacks/2 strips of slices+1 vertex
 // pairs, latitude angles j * 2pi / (slices/2), longitudes i * pi / slices,

@@ -165,7 +165,8 @@ int grt_mesh_end(grt_mesh mesh);
  * indices; a face of k corners becomes the fan (c0, ci, ci+1); a corner
  * without a normal gets (0,0,0); mtllib/usemtl give the triangles the
  * material's Kd (white without one).  The transform is not applied.  With
- * GRT_MESH_DROP_DEGENERATE the faces go through the degenerate filter too. */
+ * GRT_MESH_DROP_DEGENERATE the faces go through the degenerate filter too.  A
+ * file refused part way (bad vertex or face corner) adds nothing. */
 int grt_mesh_load_obj(grt_mesh mesh, const char* path, uint32_t flags);
 /* rtdb::loadSphere (rtdb.cpp:142-192) and the unit cube of rtdb::loadCube,
  * through the current transform. */

@@ -158,6 +158,41 @@ def test_obj_faces_normals_and_materials(tmp_path):
         grt.Mesh().load_obj(tmp_path / "missing.obj")
 
 
+def test_obj_loader_survives_mutated_files(tmp_path):
+    """Random byte mutations of a valid OBJ either load (every vertex of every
+    triangle a finite coordinate triple taken from the file's `v` lines, or
+    dropped) or are refused with an error; the loader never crashes and a
+    refused load leaves the mesh as it was."""
+    base = ("mtllib m.mtl\nusemtl red\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvn 0 0 1\n"
+            "f 1//1 2//1 3//1 4//1\nf -1 -2 -3\n")
+    (tmp_path / "m.mtl").write_text("newmtl red\nKd 1 0 0\n")
+    r = np.random.default_rng(3)
+    alphabet = list("0123456789 -/\nfvn#e.")
+    loaded = refused = 0
+    for k in range(300):
+        s = list(base)
+        for _ in range(int(r.integers(1, 8))):
+            s[int(r.integers(len(s)))] = alphabet[int(r.integers(len(alphabet)))]
+        path = tmp_path / f"f{k}.obj"
+        path.write_text("".join(s))
+        m = grt.Mesh()
+        m.begin(grt.TRIANGLES)
+        for p in ((5, 5, 5), (6, 5, 5), (5, 6, 5)):
+            m.vertex(*p)
+        m.end()
+        try:
+            m.load_obj(path, drop_degenerate=bool(k & 1))
+        except grt.GrtError:
+            refused += 1
+            assert m.info() == (1, 0), "a refused load changed the mesh"
+            continue
+        loaded += 1
+        t, n, kd, src = _tris(m)
+        assert np.isfinite(t).all()
+        assert len(t) == m.info()[0] >= 1
+    assert loaded > 0 and refused > 0
+
+
 def test_procedural_sphere_and_cube():
     m = grt.Mesh()
     m.add_sphere(16, 16)
