@@ -839,6 +839,8 @@ int grt_trace_closest(grt_scene sc, const grt_rays* rays, const grt_hits* hits, 
     int rc = trace_ready(sc, "grt_trace_closest");
     if (rc != GRT_OK)
         return rc;
+    if (n == 0)
+        return GRT_OK;   // nothing to trace; the buffers may be NULL
     if (!rays_ok(rays) || !hits_ok(hits))
         return set_error(GRT_INVALID_PARAM, "grt_trace_closest: NULL ray or hit buffer");
     return dispatch_closest(sc, trace_args(sc, n, image_width, counters_dev), rays, hits);
@@ -850,6 +852,8 @@ int grt_trace_any(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev, uin
     int rc = trace_ready(sc, "grt_trace_any");
     if (rc != GRT_OK)
         return rc;
+    if (n == 0)
+        return GRT_OK;   // nothing to trace; the buffers may be NULL
     if (!rays_ok(rays) || !occluded_dev)
         return set_error(GRT_INVALID_PARAM, "grt_trace_any: NULL ray or output buffer");
     return dispatch_any(sc, trace_args(sc, n, image_width, counters_dev), rays, occluded_dev);
@@ -882,6 +886,8 @@ int grt_trace_closest_list(grt_scene sc, const grt_rays* rays, const grt_hits* h
     int rc = trace_ready(sc, "grt_trace_closest_list");
     if (rc != GRT_OK)
         return rc;
+    if (n == 0)
+        return GRT_OK;   // nothing to trace; the buffers may be NULL
     if (!rays_ok(rays) || !hits_ok(hits) || !list_dev || !count_dev)
         return set_error(GRT_INVALID_PARAM, "grt_trace_closest_list: bad argument");
     TraceArgs a = trace_args(sc, n, 0, counters_dev);
@@ -897,6 +903,8 @@ int grt_trace_any_list(grt_scene sc, const grt_rays* rays, uint8_t* occluded_dev
     int rc = trace_ready(sc, "grt_trace_any_list");
     if (rc != GRT_OK)
         return rc;
+    if (n == 0)
+        return GRT_OK;   // nothing to trace; the buffers may be NULL
     if (!rays_ok(rays) || !occluded_dev || !list_dev || !count_dev)
         return set_error(GRT_INVALID_PARAM, "grt_trace_any_list: bad argument");
     TraceArgs a = trace_args(sc, n, 0, counters_dev);

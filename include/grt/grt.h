@@ -211,7 +211,8 @@ int grt_grid_download(grt_scene scene, uint32_t* offsets_host, uint32_t* refs_ho
  * UniformGridAccStruct.cpp:30-42,113-196,362-378; kernel rayTrace,
  * RayTracing_kernel.cu:140-275).  image_width != 0 maps each wave to an 8x8
  * tile of a row-major image of that width.  counters_dev: NULL or 2 uint32 per
- * ray {cells visited, triangle tests}. */
+ * ray {cells visited, triangle tests}.  n = 0 does nothing (the buffers may be
+ * NULL); so do the other grt_trace_* calls. */
 int grt_trace_closest(grt_scene scene, const grt_rays* rays, const grt_hits* hits, uint32_t n,
                       uint32_t image_width, uint32_t* counters_dev);
 /* Any hit in (tmin, tmax] (findAnyHit, shadow_FRAG.glsl:147-245; the

@@ -298,6 +298,48 @@ def _random_rays(n, seed, inside=True):
             "tmax": np.where(np.arange(n) % 5 == 0, np.float32(0.3), np.float32(np.inf))}
 
 
+def _degenerate_rays(n, seed):
+    """Rays that break the usual assumptions, interleaved with ordinary ones
+    so that they share waves: zero, NaN, infinite and tiny directions, NaN
+    and far-away origins, empty and one-point intervals, negative tmin."""
+    rays = _random_rays(n, seed, inside=True)
+    f = np.float32
+    cases = [((0.5, 0.5, 0.5), (0, 0, 0), 0, np.inf),
+             ((0.5, 0.5, 0.5), (np.nan, 1, 0), 0, np.inf),
+             ((np.nan, 0.5, 0.5), (1, 0, 0), 0, np.inf),
+             ((0.2, 0.5, 0.5), (1, 0.1, 0), 0.5, 0.2),
+             ((0.2, 0.5, 0.5), (1, 0.1, 0), 0.3, 0.3),
+             ((1e30, 0.5, 0.5), (-1, 0, 0), 0, np.inf),
+             ((0.5, 0.5, 0.5), (np.inf, 0, 0), 0, np.inf),
+             ((0.5, 0.5, 0.5), (1, 1, 1), -1.0, np.inf),
+             ((0.5, 0.5, 0.5), (1e-30, 1e-30, 1e-30), 0, np.inf),
+             ((-0.5, 0.5, 0.5), (1, 0, 0), 0, np.nan)]
+    for k, (o, d, t0, t1) in enumerate(cases):
+        sl = slice(k, n, 7 * len(cases))
+        for a, key in enumerate(("ox", "oy", "oz")):
+            rays[key][sl] = f(o[a])
+        for a, key in enumerate(("dx", "dy", "dz")):
+            rays[key][sl] = f(d[a])
+        rays["tmin"][sl] = f(t0)
+        rays["tmax"][sl] = f(t1)
+    return rays
+
+
+def test_degenerate_rays(gpu_ctx, c1_scene):
+    """Every schedule returns the oracle's results and counters on degenerate
+    rays, and zero-length launches are accepted."""
+    import torch
+    v, n = c1_scene
+    sc, fr, offs, refs = check_grid(gpu_ctx, v, n, 5.0)
+    rays = _degenerate_rays(8192, 41)
+    check_closest(sc, fr, offs, refs, v, rays)
+    check_any(sc, fr, offs, refs, v, rays)
+    empty = grt.RayBatch.empty(0)
+    sc.trace_closest(empty, grt.HitBatch.empty(0))
+    sc.trace_any(empty, torch.empty(0, dtype=torch.uint8, device="cuda:0"))
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("inside", [True, False])
 def test_closest_random_rays(gpu_ctx, c1_scene, inside):
     v, n = c1_scene
