@@ -212,6 +212,46 @@ def test_trace_edge(gpu_ctx, name):
     check_any(sc, fr, offs, refs, v, rays)
 
 
+@pytest.mark.parametrize("over", ["cells", "refs"])
+def test_rebuild_over_capacity_is_safe(gpu_ctx, c1_scene, over):
+    """An asynchronous rebuild that outgrows the reserved grid (more cells, or
+    more references with the cells reserved) leaves a grid every trace treats
+    as empty: traces already queued return misses, the check reports
+    GRT_CAPACITY, later traces are refused, and a synchronous build recovers
+    with the oracle's grid and hits."""
+    import torch
+    v, n = c1_scene
+    sc = grt.Scene(gpu_ctx)
+    sc.upload(v, n)
+    sc.build_grid(2.0)
+    if over == "refs":
+        sc.reserve_grid(int(orc.grid_frame(v, 40.0)["m"]) + 4096, 1)
+    cam = c1_camera(96, 64)
+    rb, hb = grt.RayBatch.empty(cam.width * cam.height), grt.HitBatch.empty(cam.width * cam.height)
+    gpu_ctx.gen_primary(cam, rb)
+    sc.rebuild_grid_async(40.0)
+    hb.f["id"].fill_(7)
+    sc.trace_closest(rb, hb, image_width=cam.width)
+    torch.cuda.synchronize()
+    assert (hb.numpy()["id"] == -1).all()
+    with pytest.raises(grt.GrtError) as e:
+        sc.check_grid()
+    assert e.value.code == grt.GRT_CAPACITY
+    with pytest.raises(grt.GrtError):
+        sc.trace_closest(rb, hb, image_width=cam.width)
+    sc.build_grid(5.0)
+    fr = orc.grid_frame(v, 5.0)
+    _, ooffs, orefs = orc.grid_build(v, fr)
+    _, offs, refs = sc.download_grid()
+    assert_same(offs, ooffs, "offsets after recovery")
+    assert_same(refs, orefs, "refs after recovery")
+    sc.trace_closest(rb, hb, image_width=cam.width)
+    o = orc.trace_closest(fr, ooffs, orefs, v, rb.numpy())
+    g = hb.numpy()
+    for k in ("id", "t", "u", "v"):
+        assert_same(g[k], o[k], f"hit.{k} after recovery")
+
+
 def test_grid_build_stadium_small(gpu_ctx):
     v, n, _ = grt.gen_stadium(SEED_C2, 16, 2048, 16 * 2048 - 2)
     check_grid(gpu_ctx, v, n, 5.0)
