@@ -1,7 +1,7 @@
 /*
  * grt_oracle.c -- CPU oracle for the uniform-grid 3D-DDA + Moller-Trumbore path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); the product
+ * TEST INFRASTRUCTURE ONLY (parity reference and CPU baseline); the product
  * library never links it.  Every function cites the reference file:line it
  * restates.  Build: oracle/Makefile (gcc -O3 -ffp-contract=off -fopenmp).
  *

@@ -1,8 +1,9 @@
 /*
  * grt_oracle.h -- CPU oracle for the uniform-grid 3D-DDA + Moller-Trumbore path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
- * compared against, and the CPU baseline that the benchmark times beside it.
+ * TEST INFRASTRUCTURE ONLY.  This library is the reference the parity tests
+ * compare the HIP path against, and the CPU baseline that the benchmark times
+ * beside it.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
  * it.  The product library (libgrt.so) never links or calls it.
  *

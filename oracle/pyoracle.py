@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker / CPU baseline.  The product
+cpu_baseline leg of bench.py as parity reference / CPU baseline.  The product
 (libgrt.so and its Python bindings) never imports this module.
 """
 from __future__ import annotations
