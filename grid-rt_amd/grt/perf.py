@@ -49,7 +49,7 @@ def _summary(arrival, start, finish, rays, wall):
 
 
 def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120,
-                   loads=(0.5, 0.8, 0.95, 1.0, 1.25, 1.5, 2.0), n_open: int = 400,
+                   loads=(0.5, 0.8, 0.95, 1.0, 1.25, 1.5, 2.0, 2.25, 2.5), n_open: int = 400,
                    budget: int = 8 << 30,
                    max_active: int = 64, slo_ms: float = 50.0):
     """Serving (continuous batching) against the plain path on the mixed
