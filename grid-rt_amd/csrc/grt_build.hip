@@ -786,9 +786,15 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
         uint32_t ts, tl;
         uint32_t ps = block_excl_scan<256>(ns, lds, ts);
         uint32_t pl = block_excl_scan<256>(nl, lds2, tl);
-        if (threadIdx.x == 0) {
-            list_base[0] = ts ? atomicAdd(lists.counts, ts) : 0u;
-            list_base[1] = tl ? atomicAdd(lists.counts + 1, tl) : 0u;
+        if (threadIdx.x == 0 && (ts | tl)) {
+            // both list lengths in one 64-bit atomic: every block adds to
+            // the same counters, and such atomics serialise (each length
+            // stays below 2^32, so the low half never carries)
+            const unsigned long long old =
+                atomicAdd(reinterpret_cast<unsigned long long*>(lists.counts),
+                          (unsigned long long)ts | ((unsigned long long)tl << 32));
+            list_base[0] = (uint32_t)old;
+            list_base[1] = (uint32_t)(old >> 32);
         }
         __syncthreads();
         ps += list_base[0];
@@ -1145,22 +1151,26 @@ __device__ __forceinline__ void wave_sort(uint32_t* __restrict__ seg, uint32_t n
 
 constexpr uint32_t kSortWaveMax = 4096;
 
-// Split lists.large by size class (see SortLists), one atomic per wave and
-// class.
+// Split lists.large by size class (see SortLists).  The waves of a block
+// reserve their runs in an LDS count per class, and the block then takes one
+// global atomic per class it holds: many blocks adding to the same few
+// counters serialise, so the fewer such atomics the better.
 __global__ __launch_bounds__(256) void k_sort_classify(const DevFrame* __restrict__ fr,
                                                        const uint32_t* __restrict__ offsets,
                                                        SortLists lists)
 {
+    __shared__ uint32_t block_cnt[kSortClasses];
+    __shared__ uint32_t block_base[kSortClasses];
     if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
         return;
     uint32_t cnt = lists.counts[1];
     cnt = cnt < lists.large_cap ? cnt : lists.large_cap;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
-    // the loop runs over whole waves (li0 is the wave's first entry), so every
-    // lane takes part in the ballots
-    for (uint32_t li0 = blockIdx.x * blockDim.x + threadIdx.x - lane; li0 < cnt; li0 += stride) {
-        const uint32_t li = li0 + lane;
+    // the loop runs over whole blocks (li0 is the block's first entry), so
+    // every thread takes part in the ballots and barriers
+    for (uint32_t li0 = blockIdx.x * blockDim.x; li0 < cnt; li0 += stride) {
+        const uint32_t li = li0 + threadIdx.x;
         int cls = -1;
         uint32_t c = 0;
         if (li < cnt) {
@@ -1168,19 +1178,34 @@ __global__ __launch_bounds__(256) void k_sort_classify(const DevFrame* __restric
             const uint32_t n = offsets[c + 1] - offsets[c];
             cls = n <= 64u ? 0 : n <= 256u ? 1 : n <= 1024u ? 2 : n <= kSortWaveMax ? 3 : 4;
         }
+        if (threadIdx.x < (uint32_t)kSortClasses)
+            block_cnt[threadIdx.x] = 0u;
+        __syncthreads();
+        uint32_t at = 0;
+        uint64_t mine = 0;
 #pragma unroll
         for (int q = 0; q < kSortClasses; ++q) {
             const uint64_t m = __ballot(cls == q);
             if (m == 0)
                 continue;
             const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-            uint32_t at = 0;
+            uint32_t w = 0;
             if (lane == leader)
-                at = atomicAdd(lists.counts + 2 + q, (uint32_t)__popcll(m));
-            at = (uint32_t)__builtin_amdgcn_readlane((int)at, (int)leader);
-            if (cls == q)
-                lists.cls[(size_t)q * lists.large_cap + at + lanes_below(m)] = c;
+                w = atomicAdd(block_cnt + q, (uint32_t)__popcll(m));
+            w = (uint32_t)__builtin_amdgcn_readlane((int)w, (int)leader);
+            if (cls == q) {
+                at = w;
+                mine = m;
+            }
         }
+        __syncthreads();
+        if (threadIdx.x < (uint32_t)kSortClasses) {
+            const uint32_t k = block_cnt[threadIdx.x];
+            block_base[threadIdx.x] = k ? atomicAdd(lists.counts + 2 + threadIdx.x, k) : 0u;
+        }
+        __syncthreads();
+        if (cls >= 0)
+            lists.cls[(size_t)cls * lists.large_cap + block_base[cls] + at + lanes_below(mine)] = c;
     }
 }
 
@@ -1192,7 +1217,8 @@ __global__ __launch_bounds__(256) void k_sort_wave(const DevFrame* __restrict__ 
 {
     if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
         return;
-    const uint32_t cnt = lists.counts[2 + Q];
+    uint32_t cnt = lists.counts[2 + Q];
+    cnt = cnt < lists.large_cap ? cnt : lists.large_cap;
     const uint32_t* __restrict__ list = lists.cls + (size_t)Q * lists.large_cap;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
@@ -1256,7 +1282,8 @@ __global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__
     __shared__ uint32_t lds[kSortTile];
     if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
         return;
-    const uint32_t cnt = lists.counts[2 + 4];
+    uint32_t cnt = lists.counts[2 + 4];
+    cnt = cnt < lists.large_cap ? cnt : lists.large_cap;
     const uint32_t* __restrict__ list = lists.cls + (size_t)4 * lists.large_cap;
     for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const uint32_t c = list[li];

@@ -32,7 +32,8 @@ hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_
                              uint32_t* tile_sums, uint32_t cells_cap, hipStream_t s);
 // Cells the per-cell sort has to visit, collected by the scan: `small` holds
 // the cells of 2..kSortSmallCells references, `large` the larger ones;
-// counts[0], counts[1] their lengths (zeroed by launch_grid_scan).
+// counts[0], counts[1] their lengths (zeroed by launch_grid_scan; counts is
+// 8-byte aligned, the scan adds to both with one 64-bit atomic).
 // `cls` (large_cap entries per class) receives the large list split by size
 // for the sort kernels: (16,64], (64,256], (256,1024], (1024,4096], larger;
 // counts[2..6] their lengths.
