@@ -59,7 +59,8 @@ def parse():
     p.add_argument("--sched", default=None,
                    help="traversal schedule per stage (primary, shadow, bounce, bounce "
                         "shadow): s = one ray per lane, w = one ray per lane with pooled "
-                        "tests, p = persistent refill, 2/4/8 = lanes per ray; default: tuned")
+                        "tests, p = persistent refill, 2/4/8 = lanes per ray, x = 16 lanes "
+                        "per ray; default: tuned")
     p.add_argument("--refill", type=int, default=16)
     p.add_argument("--ray-lists", nargs="?", const="ccc", default=None,
                    help="ray lists of the three secondary stages (shadow, bounce, bounce "
@@ -97,7 +98,7 @@ def kernel_name(closest, sched_char):
     """The traversal kernel a stage launches under a schedule character."""
     if sched_char == "p":
         return "k_trace_persistent<%s, false>" % ("false" if closest else "true")
-    if sched_char in "248":
+    if sched_char in "248x":
         return "k_trace_split<%s, false>" % ("false" if closest else "true")
     pooled = "true" if sched_char == "w" else "false"
     return ("k_trace_closest<false, %s>" if closest else "k_trace_any<false, %s>") % pooled
@@ -229,14 +230,15 @@ class Frame:
             raise ValueError("ray list kind %r (expected - or c)" % kind)
 
     SCHEDULES = {"s": "simple", "w": "pooled", "p": "persistent", "2": "split", "4": "split",
-                 "8": "split"}
+                 "8": "split", "x": "split"}
+    SPLIT_LANES = {"2": 2, "4": 4, "8": 8, "x": 16}
 
     def mode(self, stage):
         """Traversal schedule of a stage: 's' one ray per lane, 'w' one ray per
         lane with the wave's tests pooled, 'p' persistent, '2'/'4'/'8' that
-        many lanes per ray."""
+        many lanes per ray, 'x' 16 lanes per ray."""
         ch = self.sched[stage]
-        self.ctx.set_trace_mode(self.SCHEDULES[ch], int(ch) if ch in "248" else self.refill)
+        self.ctx.set_trace_mode(self.SCHEDULES[ch], self.SPLIT_LANES.get(ch, self.refill))
 
     def active(self, rb):
         f = rb.f
