@@ -786,13 +786,14 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
         uint32_t ts, tl;
         uint32_t ps = block_excl_scan<256>(ns, lds, ts);
         uint32_t pl = block_excl_scan<256>(nl, lds2, tl);
-        if (threadIdx.x == 0 && (ts | tl)) {
+        if (threadIdx.x == 0) {
             // both list lengths in one 64-bit atomic: every block adds to
             // the same counters, and such atomics serialise (each length
             // stays below 2^32, so the low half never carries)
             const unsigned long long old =
-                atomicAdd(reinterpret_cast<unsigned long long*>(lists.counts),
-                          (unsigned long long)ts | ((unsigned long long)tl << 32));
+                (ts | tl) ? atomicAdd(reinterpret_cast<unsigned long long*>(lists.counts),
+                                      (unsigned long long)ts | ((unsigned long long)tl << 32))
+                          : 0ull;
             list_base[0] = (uint32_t)old;
             list_base[1] = (uint32_t)(old >> 32);
         }
