@@ -520,7 +520,23 @@ int grt_scene_transform_clusters(grt_scene sc, const float* xf_host, uint32_t n_
     const size_t xb = (size_t)n_clusters * 12 * sizeof(float);
     if ((rc = sc->xf.reserve(sc->ctx, xb, false)) != GRT_OK)
         return rc;
-    // Stage through pinned memory so the copy is truly asynchronous.
+    // Stage through pinned memory so the copy is truly asynchronous.  The
+    // previous transform's copy must have left the staging buffer: wait for
+    // that copy alone (an event recorded after it), not for the whole stream,
+    // so the host can enqueue this frame while the previous one still runs.
+    if (!sc->xf_copied) {
+        hipError_t e = hipEventCreateWithFlags(&sc->xf_copied, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            sc->xf_copied = nullptr;
+            return hip_check(e, "transform staging event");
+        }
+    }
+    if (sc->xf_pending) {
+        hipError_t e = hipEventSynchronize(sc->xf_copied);
+        if (e != hipSuccess)
+            return hip_check(e, "transform staging");
+        sc->xf_pending = false;
+    }
     if (sc->xf_host_bytes < xb) {
         if (sc->xf_host)
             (void)hipHostFree(sc->xf_host);
@@ -532,14 +548,13 @@ int grt_scene_transform_clusters(grt_scene sc, const float* xf_host, uint32_t n_
         }
         sc->xf_host_bytes = xb;
     }
-    // The previous transform's copy must have left the staging buffer.
-    hipError_t e = hipStreamSynchronize(s);
-    if (e != hipSuccess)
-        return hip_check(e, "transform staging");
     std::memcpy(sc->xf_host, xf_host, xb);
-    e = hipMemcpyAsync(sc->xf.ptr, sc->xf_host, xb, hipMemcpyHostToDevice, s);
+    hipError_t e = hipMemcpyAsync(sc->xf.ptr, sc->xf_host, xb, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipEventRecord(sc->xf_copied, s);
     if (e != hipSuccess)
         return hip_check(e, "uploading transforms");
+    sc->xf_pending = true;
     sc->grid_valid = false;
     return hip_check(launch_transform_pack(sc->n_tris, tris_per_cluster, n_cluster_tris,
                                            sc->xf.as<float>(), sc->rest_verts.as<float>(),
@@ -579,8 +594,13 @@ int grt_grid_build(grt_scene sc, float lambda)
         return rc;
     if (host.status != 0)
         return set_error(GRT_CAPACITY, "grid build status 0x%x", host.status);
-    if ((rc = grid_reserve_refs(sc, host.r > 0 ? host.r : 1)) != GRT_OK)
+    if ((rc = grid_reserve_refs(sc, host.r > 0 ? host.r : 1)) != GRT_OK) {
+        // The count pass ran and the scatter will not: return the counts to
+        // zero for the next build.
+        (void)launch_fill_u32(sc->counts.as<uint32_t>(), 0u, sc->cells_cap, s);
+        (void)hipStreamSynchronize(s);
         return rc;
+    }
     if ((rc = enqueue_scan(sc, sc->refs_cap, true, true)) != GRT_OK)
         return rc;
     if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
@@ -645,11 +665,8 @@ int grt_grid_check(grt_scene sc)
         return rc;
     sc->info_host = host;
     sc->info_stale = false;
-    if (host.status & kStatusRefsOver) {
-        // The scatter did not run, so the counts were not returned to zero.
-        (void)launch_fill_u32(sc->counts.as<uint32_t>(), 0u, sc->cells_cap, sc->ctx->stream);
-        (void)hipStreamSynchronize(sc->ctx->stream);
-    }
+    // A rebuild over the reference capacity zeroed the counts on the device
+    // (the scan's apply pass), so the next build needs no clean-up here.
     if (host.status != 0) {
         sc->grid_valid = false;
         return set_error(GRT_CAPACITY, "grid rebuild exceeded capacity (status 0x%x)",

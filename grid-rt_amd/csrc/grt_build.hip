@@ -715,10 +715,16 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* __restrict__ bsu
 // With lists.small set, also appends every cell with 2..kSortSmallCells
 // references to lists.small and every larger cell to lists.large (the cells
 // the sort pass has to visit), one atomic per block and list.
+// When the total exceeds the reference capacity (kStatusRefsOver) the scatter
+// will not run, so nothing would return the counts to zero: with `clear` set
+// (the grid's counts, the same array as `in`), each block then zeroes the
+// counts of its tile instead of writing offsets, and the next build starts
+// from clean counts whether or not the host checks the status first.
 __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n_host,
                                                     const DevFrame* fr,
                                                     const uint32_t* __restrict__ bsum,
-                                                    uint32_t* __restrict__ out, SortLists lists)
+                                                    uint32_t* __restrict__ out, SortLists lists,
+                                                    uint32_t* clear)
 {
     __shared__ uint32_t lds[8];
     __shared__ uint32_t lds2[8];
@@ -749,11 +755,25 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
         }
         return;
     }
+    const bool whole = base + kScanTile <= n;
+    if (clear && (fr->status & kStatusRefsOver)) {
+        // Tiles with a zero sum (above) hold zero counts already.
+        if (whole) {
+            uint4* q = reinterpret_cast<uint4*>(clear + base) + threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                q[256 * k] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            for (uint32_t k = 0; k < 16; ++k)
+                if (i0 + k < n)
+                    clear[i0 + k] = 0u;
+        }
+        return;
+    }
     // Whole tiles move through LDS so that global loads and stores are
     // lane-contiguous (each 16-byte access of a wave covers 1 KB); thread t
     // then scans cells [16t, 16t + 16) of the tile from LDS.
     __shared__ uint4 tile[kScanTile / 4];
-    const bool whole = base + kScanTile <= n;
     uint32_t v[16];
     if (whole) {
         const uint4* src = reinterpret_cast<const uint4*>(in + base);
@@ -1459,7 +1479,7 @@ hipError_t launch_grid_count(const DevFrame* fr, const float* verts, uint32_t n_
     return hipGetLastError();
 }
 
-hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
+hipError_t launch_grid_scan(DevFrame* fr, uint32_t* counts, uint32_t* offsets,
                             uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
                             const SortLists& lists, bool prefixes_ready, hipStream_t s)
 {
@@ -1472,8 +1492,8 @@ hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offs
     if (!prefixes_ready)
         hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, 0u, fr, offsets,
                            refs_cap);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, counts, 0u, (const DevFrame*)fr,
-                       (const uint32_t*)block_sums, offsets, lists);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, (const uint32_t*)counts, 0u,
+                       (const DevFrame*)fr, (const uint32_t*)block_sums, offsets, lists, counts);
     return hipGetLastError();
 }
 
@@ -1487,7 +1507,7 @@ hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, block_sums, n, (DevFrame*)nullptr,
                        out, 0xffffffffu);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n, (const DevFrame*)nullptr,
-                       (const uint32_t*)block_sums, out, SortLists{});
+                       (const uint32_t*)block_sums, out, SortLists{}, (uint32_t*)nullptr);
     return hipGetLastError();
 }
 

@@ -68,6 +68,8 @@ struct grt_scene_t {
     grt::DevMem verts, normals, rest_verts, rest_normals, tri, xf;
     void* xf_host = nullptr;
     size_t xf_host_bytes = 0;
+    hipEvent_t xf_copied = nullptr;   // recorded after each copy out of xf_host
+    bool xf_pending = false;
     // Grid.
     grt::DevMem frame, partial, counts, offsets, refs, block_sums, big_list, sort_list,
         sort_list_small, sort_classes, small_counters, trir;
@@ -86,6 +88,8 @@ struct grt_scene_t {
     }
     ~grt_scene_t()
     {
+        if (xf_copied)
+            (void)hipEventDestroy(xf_copied);
         if (xf_host)
             (void)hipHostFree(xf_host);
     }

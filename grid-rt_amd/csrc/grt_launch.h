@@ -48,11 +48,12 @@ struct SortLists {
 };
 
 // Exclusive scan counts[0..m) -> offsets[0..m], total also to fr->r; flags
-// kStatusRefsOver when the total exceeds refs_cap.  block_sums holds
+// kStatusRefsOver when the total exceeds refs_cap, and then zeroes the counts
+// instead of writing offsets (the scatter, which would, does not run).  block_sums holds
 // ceil(cells_cap/kScanTile) entries: the per-tile sums the count pass left,
 // scanned in place into tile offsets first unless prefixes_ready (a second
 // scan of the same counts).  With lists.small set, also fills the sort lists.
-hipError_t launch_grid_scan(DevFrame* fr, const uint32_t* counts, uint32_t* offsets,
+hipError_t launch_grid_scan(DevFrame* fr, uint32_t* counts, uint32_t* offsets,
                             uint32_t* block_sums, uint32_t cells_cap, uint32_t refs_cap,
                             const SortLists& lists, bool prefixes_ready, hipStream_t s);
 // Scatter pass: refs[offsets[c] + k] = t; returns counts to zero.
@@ -82,7 +83,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* block_sums,
                            DevFrame* tmp_frame, hipStream_t s);
 
-// ---- traversal and ray generation (grt_trace.hip) ----------------------------
+// ---- traversal and ray generation (grt_traverse.hip, grt_raygen.hip) ---------
 struct TraceArgs {
     const DevFrame* fr;
     const uint32_t* offsets;

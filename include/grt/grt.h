@@ -129,8 +129,9 @@ int grt_scene_download_geometry(grt_scene scene, float* verts_host, float* norma
  * with c = t / tris_per_cluster, M_c the row-major 3x4 matrix xf_host[12c..];
  * normals get the 3x3 part; other triangles keep the rest pose.  The rest pose
  * is the last uploaded geometry.  The matrices are staged in pinned memory:
- * the call first waits on the context's stream for the previous transform's
- * copy to leave that buffer. */
+ * the call first waits (on an event, not on the whole stream) for the previous
+ * transform's host-to-device copy to leave that buffer, so the host can
+ * enqueue a frame while the device still works on the previous one. */
 int grt_scene_transform_clusters(grt_scene scene, const float* xf_host, uint32_t n_clusters,
                                  uint32_t tris_per_cluster, uint32_t n_cluster_tris);
 

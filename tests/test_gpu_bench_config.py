@@ -1,0 +1,243 @@
+"""Parity at the benchmarked configuration (the headline frame).
+
+Workload: the full 2^20-triangle stadium -- 256 Gaussian clusters of 4096
+triangles plus a 2-triangle ground plane 100x the cluster extent -- after
+per-frame rigid cluster transforms, with its grid rebuilt on the device at the
+two resolutions the frame autotuner chooses between: lambda 256 (about
+2658 x 39 x 2658 = 2.8e8 cells, 1.5e7 references) and lambda 512 (about
+3348 x 48 x 3348 = 5.4e8 cells, 2.3e7 references).  Both grids hold more than
+2^24 cells and ground triangles that span millions of cells each.
+
+The four traversal stages of the 1920 x 1080 frame -- primary closest hit,
+shadow any hit, one mirror bounce, bounce shadow -- run under every schedule
+the autotuner may pick (one ray per lane with per-lane or pooled tests,
+persistent refill, 2, 4, 8 and 16 lanes per ray), over the whole image and
+over compacted ray lists, and the Phong frame is shaded from them.
+
+Bar: bit-exact against the oracle (its OpenMP path) -- transformed vertices,
+grid frame, offsets, references, generated rays, hit id / t / u / v,
+occlusion flags, per-ray cell and test counters, RGB and RGBA8 pixels.
+"""
+import numpy as np
+import pytest
+import torch
+
+import grt
+import pyoracle as orc
+from conftest import SEED_C2
+
+pytestmark = pytest.mark.gpu
+
+N_CLUSTERS, PER_CLUSTER, N_CLUSTER_TRIS = 256, 4096, (1 << 20) - 2
+WIDTH, HEIGHT = 1920, 1080
+EYE, CENTER, UP, FOVY = (0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0.0, 1.0, 0.0), 60.0
+LIGHT, OFFSET = (1000.0, 1000.0, 1000.0), 1e-4
+
+# (schedule, refill threshold or lanes per ray): every candidate of the frame
+# autotuner ('s', 'w', 'p', '2', '4', '8', 'x'), all with cooperative dense
+# cells at the bench's threshold of 16.
+SCHEDULES = [("simple", 16), ("pooled", 16), ("persistent", 16), ("split", 2), ("split", 4),
+             ("split", 8), ("split", 16)]
+
+
+def bits(a):
+    a = np.asarray(a)
+    return a.view(np.uint32) if a.dtype == np.float32 else a
+
+
+def assert_same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
+    diff = np.nonzero(bits(a) != bits(b))[0]
+    assert diff.size == 0, (f"{what}: {diff.size} of {a.size} differ, first at {diff[:5]}: "
+                            f"{a[diff[:5]]} vs {b[diff[:5]]}")
+
+
+@pytest.fixture(scope="module")
+def stadium():
+    return grt.gen_stadium(SEED_C2, N_CLUSTERS, PER_CLUSTER, N_CLUSTER_TRIS)
+
+
+def _oracle_pose(stadium, frame):
+    v, nrm, centers = stadium
+    xf = grt.gen_cluster_transforms(SEED_C2, frame, centers, 0.05)
+    ov, on = orc.transform(v, nrm, xf, PER_CLUSTER, N_CLUSTER_TRIS)
+    return xf, ov, on
+
+
+def _scene_like_bench(ctx, stadium, lam):
+    """The bench's set-up: frame 0 transformed and built synchronously, the
+    grid reserved with the bench's headroom (per-frame rebuilds never
+    reallocate)."""
+    v, nrm, _ = stadium
+    sc = grt.Scene(ctx)
+    sc.upload(v, nrm)
+    xf0, _, _ = _oracle_pose(stadium, 0)
+    sc.transform_clusters(xf0, PER_CLUSTER, N_CLUSTER_TRIS)
+    sc.build_grid(lam)
+    info = sc.grid_info()
+    sc.reserve_grid(int(info["m"] * 1.25) + 4096, int(info["r"] * 1.5) + 4096)
+    return sc
+
+
+def _check_grid(sc, ov, lam, tag):
+    info, offs, refs = sc.download_grid()
+    fr = orc.grid_frame(ov, lam)
+    _, ooffs, orefs = orc.grid_build(ov, fr)
+    for k in ("bmin", "bmax", "cell", "inv_cell"):
+        assert_same(np.asarray(info[k], np.float32), np.asarray(fr[k], np.float32), f"{k} {tag}")
+    assert list(info["n"]) == list(fr["n"]), tag
+    assert info["m"] == fr["m"] and info["m"] > (1 << 24), (info["m"], tag)
+    assert info["r"] == len(orefs), tag
+    assert_same(offs, ooffs, f"offsets {tag}")
+    assert_same(refs, orefs, f"refs {tag}")
+    return fr, ooffs, orefs
+
+
+@pytest.mark.parametrize("lam", [256.0, 512.0])
+def test_bench_grid_per_frame_rebuild(gpu_ctx, stadium, lam):
+    """Two frames of transform + asynchronous rebuild, as the timed loop runs
+    them: vertices, normals and the whole grid equal the oracle's."""
+    sc = _scene_like_bench(gpu_ctx, stadium, lam)
+    try:
+        for frame in (1, 2):
+            xf, ov, on = _oracle_pose(stadium, frame)
+            sc.transform_clusters(xf, PER_CLUSTER, N_CLUSTER_TRIS)
+            sc.rebuild_grid_async(lam)
+            sc.check_grid()
+            gv, gn = sc.download_geometry()
+            assert_same(gv.ravel(), ov.ravel(), f"vertices frame {frame}")
+            assert_same(gn.ravel(), on.ravel(), f"normals frame {frame}")
+            _check_grid(sc, ov, lam, f"[lambda {lam:g} frame {frame}]")
+    finally:
+        sc.close()
+
+
+def _trace_closest_all(ctx, sc, rays, o, tag, list_args=None):
+    n = rays.n
+    active = None
+    if list_args is not None:
+        f = rays.numpy()
+        active = f["tmin"] <= f["tmax"]
+    for mode, param in SCHEDULES:
+        ctx.set_trace_mode(mode, param)
+        hb = grt.HitBatch.empty(n)
+        cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
+        if list_args is None:
+            sc.trace_closest(rays, hb, image_width=WIDTH, counters=cnt)
+        else:
+            lst, count = list_args
+            ctx.compact_rays(rays, WIDTH, lst, count, miss_hits=hb)
+            sc.trace_closest_list(rays, hb, lst, count, counters=cnt)
+        g = hb.numpy()
+        gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+        t = f"{tag} [{mode} {param}{' listed' if list_args is not None else ''}]"
+        for k in ("id", "t", "u", "v"):
+            assert_same(g[k], o[k], f"hit.{k} {t}")
+        sel = slice(None) if active is None else active
+        assert_same(gc[sel, 0], o["cells"][sel], f"cells {t}")
+        assert_same(gc[sel, 1], o["tests"][sel], f"tests {t}")
+    ctx.set_trace_mode("simple", 16)
+    return g
+
+
+def _trace_any_all(ctx, sc, rays, o, tag, list_args=None):
+    n = rays.n
+    active = None
+    if list_args is not None:
+        f = rays.numpy()
+        active = f["tmin"] <= f["tmax"]
+    for mode, param in SCHEDULES:
+        ctx.set_trace_mode(mode, param)
+        occ = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
+        cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
+        if list_args is None:
+            sc.trace_any(rays, occ, image_width=WIDTH, counters=cnt)
+        else:
+            lst, count = list_args
+            ctx.compact_rays(rays, WIDTH, lst, count, occluded=occ)
+            sc.trace_any_list(rays, occ, lst, count, counters=cnt)
+        gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+        t = f"{tag} [{mode} {param}{' listed' if list_args is not None else ''}]"
+        assert_same(occ.cpu().numpy(), o["occluded"], f"occluded {t}")
+        sel = slice(None) if active is None else active
+        assert_same(gc[sel, 0], o["cells"][sel], f"cells {t}")
+        assert_same(gc[sel, 1], o["tests"][sel], f"tests {t}")
+    ctx.set_trace_mode("simple", 16)
+    return occ
+
+
+def _assert_rays(rb, o, tag):
+    g = rb.numpy()
+    for k in g:
+        assert_same(g[k], o[k], f"{tag}.{k}")
+
+
+@pytest.mark.parametrize("lam", [256.0, 512.0])
+def test_bench_frame_four_stages(gpu_ctx, stadium, lam):
+    """The whole 1920 x 1080 frame after one per-frame rebuild: every stage
+    under every schedule, unlisted and listed, equals the oracle."""
+    ctx = gpu_ctx
+    ctx.set_coop(16)
+    sc = _scene_like_bench(ctx, stadium, lam)
+    try:
+        xf, ov, on = _oracle_pose(stadium, 1)
+        sc.transform_clusters(xf, PER_CLUSTER, N_CLUSTER_TRIS)
+        sc.rebuild_grid_async(lam)
+        sc.check_grid()
+        fr, offs, refs = _check_grid(sc, ov, lam, f"[lambda {lam:g}]")
+        n = WIDTH * HEIGHT
+        lst = torch.empty(n, dtype=torch.int32, device="cuda:0")
+        count = torch.zeros(1, dtype=torch.int32, device="cuda:0")
+        tag = f"[lambda {lam:g}]"
+
+        cam = grt.camera_lookat(EYE, CENTER, UP, FOVY, WIDTH, HEIGHT)
+        prim = grt.RayBatch.empty(n)
+        ctx.gen_primary(cam, prim)
+        orays = orc.primary_rays(grt.camera_dict(cam))
+        _assert_rays(prim, orays, f"primary rays {tag}")
+        oh = orc.trace_closest(fr, offs, refs, ov, orays, counters=True)
+        assert (oh["id"] >= 0).sum() > n // 4
+        g = _trace_closest_all(ctx, sc, prim, oh, f"primary {tag}")
+        hits = grt.HitBatch.from_numpy(g)
+
+        shadow = grt.RayBatch.empty(n)
+        sc.gen_shadow(prim, hits, LIGHT, OFFSET, shadow)
+        osh = orc.gen_shadow(orays, oh, on, LIGHT, OFFSET)
+        _assert_rays(shadow, osh, f"shadow rays {tag}")
+        oocc = orc.trace_any(fr, offs, refs, ov, osh, counters=True)
+        assert oocc["occluded"].sum() > 1000
+        occ = _trace_any_all(ctx, sc, shadow, oocc, f"shadow {tag}")
+        _trace_any_all(ctx, sc, shadow, oocc, f"shadow {tag}", (lst, count))
+
+        refl = grt.RayBatch.empty(n)
+        sc.gen_reflection(prim, hits, OFFSET, refl)
+        orf = orc.gen_reflection(orays, oh, on, OFFSET)
+        _assert_rays(refl, orf, f"bounce rays {tag}")
+        orh = orc.trace_closest(fr, offs, refs, ov, orf, counters=True)
+        assert (orh["id"] >= 0).sum() > 10000
+        rg = _trace_closest_all(ctx, sc, refl, orh, f"bounce {tag}")
+        _trace_closest_all(ctx, sc, refl, orh, f"bounce {tag}", (lst, count))
+        rhits = grt.HitBatch.from_numpy(rg)
+
+        shadow2 = grt.RayBatch.empty(n)
+        sc.gen_shadow(refl, rhits, LIGHT, OFFSET, shadow2)
+        osh2 = orc.gen_shadow(orf, orh, on, LIGHT, OFFSET)
+        _assert_rays(shadow2, osh2, f"bounce shadow rays {tag}")
+        oocc2 = orc.trace_any(fr, offs, refs, ov, osh2, counters=True)
+        occ2 = _trace_any_all(ctx, sc, shadow2, oocc2, f"bounce shadow {tag}")
+        _trace_any_all(ctx, sc, shadow2, oocc2, f"bounce shadow {tag}", (lst, count))
+
+        # Phong frame of both levels (the bench's shading pass).
+        rgb = torch.empty((n, 3), dtype=torch.float32, device="cuda:0")
+        rgba = torch.empty(n, dtype=torch.int32, device="cuda:0")
+        params = {"light": LIGHT}
+        sc.shade_phong(prim, hits, rgb, occ, 0, params)
+        sc.shade_phong(refl, rhits, rgb, occ2, 1, params, rgba=rgba)
+        o_rgb, o_rgba = orc.shade_phong(orays, oh, on, oocc["occluded"], params, depth=0)
+        o_rgb, o_rgba = orc.shade_phong(orf, orh, on, oocc2["occluded"], params, depth=1,
+                                        rgb=o_rgb)
+        assert_same(rgb.cpu().numpy().ravel(), o_rgb.ravel(), f"Phong rgb {tag}")
+        assert_same(rgba.cpu().numpy().view(np.uint32), o_rgba, f"Phong rgba8 {tag}")
+    finally:
+        sc.close()

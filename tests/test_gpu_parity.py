@@ -252,6 +252,36 @@ def test_rebuild_over_capacity_is_safe(gpu_ctx, c1_scene, over):
         assert_same(g[k], o[k], f"hit.{k} after recovery")
 
 
+@pytest.mark.parametrize("then", ["build", "reserve_rebuild"])
+def test_rebuild_over_refs_then_build_without_check(gpu_ctx, c1_scene, then):
+    """A rebuild that outgrows the reference capacity leaves the per-cell
+    counts clean on the device: a synchronous build, or a reserve plus
+    asynchronous rebuild, issued without grt_grid_check in between still
+    returns the oracle's grid."""
+    v, n = c1_scene
+    sc = grt.Scene(gpu_ctx)
+    sc.upload(v, n)
+    sc.build_grid(2.0)
+    fr40 = orc.grid_frame(v, 40.0)
+    sc.reserve_grid(int(fr40["m"]) + 4096, 1)
+    sc.rebuild_grid_async(40.0)     # cells fit, references do not
+    if then == "build":
+        lam = 5.0
+        sc.build_grid(lam)
+    else:
+        lam = 40.0
+        _, o40, _ = orc.grid_build(v, fr40)
+        sc.reserve_grid(int(fr40["m"]) + 4096, int(o40[-1]) + 4096)
+        sc.rebuild_grid_async(lam)
+        sc.check_grid()
+    fr = orc.grid_frame(v, lam)
+    _, ooffs, orefs = orc.grid_build(v, fr)
+    info, offs, refs = sc.download_grid()
+    assert_frame(info, fr)
+    assert_same(offs, ooffs, f"offsets after the overflow ({then})")
+    assert_same(refs, orefs, f"refs after the overflow ({then})")
+
+
 def test_grid_build_stadium_small(gpu_ctx):
     v, n, _ = grt.gen_stadium(SEED_C2, 16, 2048, 16 * 2048 - 2)
     check_grid(gpu_ctx, v, n, 5.0)
