@@ -242,6 +242,9 @@ int grt_init(int device, grt_context* out)
                          count);
     if ((e = hipSetDevice(device)) != hipSuccess)
         return hip_check(e, "hipSetDevice");
+    int rc = noalloc_setup(device);
+    if (rc != GRT_OK)
+        return rc;
     grt_context_t* ctx = new (std::nothrow) grt_context_t();
     if (!ctx)
         return set_error(GRT_NO_MEMORY, "grt_init: out of host memory");

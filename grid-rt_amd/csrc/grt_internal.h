@@ -17,6 +17,9 @@ namespace grt {
 
 int set_error(int code, const char* fmt, ...);
 int hip_check(hipError_t e, const char* what);
+// Allocation-free compaction (grt_noalloc.hip): take the current device's
+// address of the fixed scratch array, once per device (called by grt_init).
+int noalloc_setup(int device);
 
 // One device allocation, counted against its context.
 struct DevMem {

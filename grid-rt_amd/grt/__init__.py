@@ -142,6 +142,8 @@ _SIGS = {
     "grt_affine_inverse": (ctypes.c_int, [_P, _P]),
     "grt_compact_rays": (ctypes.c_int, [_P, ctypes.POINTER(Rays), _U32, _U32, _P, _P, _P,
                                         ctypes.POINTER(Hits)]),
+    "grt_compact_rays_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Rays), _U32, _U32, _P, _P,
+                                                _P, ctypes.POINTER(Hits)]),
     "grt_trace_closest_list": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits), _P,
                                               _P, _U32, _P]),
     "grt_trace_any_list": (ctypes.c_int, [_P, ctypes.POINTER(Rays), _P, _P, _P, _U32, _P]),
@@ -427,6 +429,19 @@ class Context:
                                      _ptr(occluded) if occluded is not None else None,
                                      ctypes.byref(miss_hits.c()) if miss_hits is not None
                                      else None))
+
+    def compact_rays_noalloc(self, rays: RayBatch, image_width: int, ray_list, count,
+                             occluded=None, miss_hits: "HitBatch | None" = None,
+                             n: int | None = None):
+        """compact_rays without device allocation (grt_compact_rays_noalloc):
+        the tile sums live in a fixed per-GPU array, so at most 67,100,672 ray
+        slots per call."""
+        check(lib().grt_compact_rays_noalloc(self.h, ctypes.byref(rays.c()),
+                                             rays.n if n is None else n, image_width,
+                                             _ptr(ray_list), _ptr(count),
+                                             _ptr(occluded) if occluded is not None else None,
+                                             ctypes.byref(miss_hits.c()) if miss_hits is not None
+                                             else None))
 
     def gen_primary(self, cam: Camera, out: RayBatch):
         check(lib().grt_gen_primary(self.h, ctypes.byref(cam), ctypes.byref(out.c())))

@@ -262,6 +262,17 @@ int grt_trace_instances(grt_context ctx, const grt_instance* instances, uint32_t
 int grt_compact_rays(grt_context ctx, const grt_rays* rays, uint32_t n, uint32_t image_width,
                      uint32_t* list_dev, uint32_t* count_dev, uint8_t* occluded_dev,
                      const grt_hits* miss_hits);
+/* grt_compact_rays without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * tile sums live in a fixed 64 KiB device array per GPU (set up by grt_init),
+ * so a call takes at most 67,100,672 ray slots (n, or the 8x8-tile slots of
+ * an image of width image_width) and returns GRT_CAPACITY beyond that.
+ * Results equal grt_compact_rays'.  Calls that overlap on one GPU share the
+ * array and race: issue them on one stream, or serialise them. */
+int grt_compact_rays_noalloc(grt_context ctx, const grt_rays* rays, uint32_t n,
+                             uint32_t image_width, uint32_t* list_dev, uint32_t* count_dev,
+                             uint8_t* occluded_dev, const grt_hits* miss_hits);
 int grt_trace_closest_list(grt_scene scene, const grt_rays* rays, const grt_hits* hits,
                            const uint32_t* list_dev, const uint32_t* count_dev, uint32_t n,
                            uint32_t* counters_dev);
