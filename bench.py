@@ -334,15 +334,39 @@ def stage_counters(frame, torch):
     return out
 
 
+def host_description():
+    """CPU model, logical CPUs of the host (nproc) and CPUs this process may run
+    on (its affinity mask)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "allowed_cpus": allowed}
+
+
 def cpu_baseline(frame, torch, budget_s):
-    """The oracle (C, OpenMP over rows, schedule(dynamic)) on the frame's
-    primary + shadow rays: a row sample when the whole frame would take longer
-    than budget_s, else the whole frame repeated until about budget_s has been
-    spent; also its single-thread grid build of the frame."""
+    """The oracle (C, OpenMP over rows, schedule(dynamic, 16 rows), mirroring
+    MultiThreadRenderer.cpp:23-27) on the frame's primary + shadow rays with
+    every CPU this process may use (OMP_NUM_THREADS when the host sets it):
+    a row sample when the whole frame would take longer than budget_s, else
+    the whole frame repeated until about budget_s has been spent.  Also the
+    same trace on one thread (SingleThreadRenderer.cpp:8-33) on a row sample
+    of about budget_s / 3, and the single-thread grid build of the frame
+    (UniformGridAccStructBuilder::buildGeometry)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or orc.max_threads()
-    threads = max(1, min(threads, 16))
+    host = host_description()
+    env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, env_threads or host["allowed_cpus"])
     v, nrm = frame.scene.download_geometry()
     t0 = time.perf_counter()
     fr = orc.grid_frame(v, frame.lam)
@@ -350,41 +374,53 @@ def cpu_baseline(frame, torch, budget_s):
     build_s = time.perf_counter() - t0
     rays = orc.primary_rays(frame.grt.camera_dict(frame.cam))
 
-    def run_rows(stride, chunk_rows=16):
+    def run_rows(stride, n_threads, chunk_rows=16):
         rows = np.arange(0, HEIGHT, stride)
         idx = (rows[:, None] * WIDTH + np.arange(WIDTH)[None, :]).ravel()
         sub = {k: a[idx] for k, a in rays.items()} if stride > 1 else rays
         t = time.perf_counter()
-        # MultiThreadRenderer.cpp:23-27: OpenMP schedule(dynamic, 16) over rows.
-        h = orc.trace_closest(fr, offs, refs, v, sub, threads=threads, chunk=chunk_rows * WIDTH)
+        h = orc.trace_closest(fr, offs, refs, v, sub, threads=n_threads,
+                              chunk=chunk_rows * WIDTH)
         sh = orc.gen_shadow(sub, h, nrm, LIGHT, OFFSET)
-        orc.trace_any(fr, offs, refs, v, sh, threads=threads, chunk=chunk_rows * WIDTH)
+        orc.trace_any(fr, offs, refs, v, sh, threads=n_threads, chunk=chunk_rows * WIDTH)
         dt = time.perf_counter() - t
         return len(idx) + int((sh["tmin"] <= sh["tmax"]).sum()), dt, len(rows)
 
     # Probe on every 16th row in one-row chunks (enough chunks for every
     # thread) to size the sample.
-    run_rows(64, 1)
-    n_probe, dt_probe, _ = run_rows(16, 1)
+    run_rows(64, threads, 1)
+    n_probe, dt_probe, _ = run_rows(16, threads, 1)
     full_s = dt_probe * 16
     if full_s > budget_s:
         stride = int(np.ceil(full_s / budget_s))
-        n_rays, spent, n_rows = run_rows(stride)
+        n_rays, spent, n_rows = run_rows(stride, threads)
         what = f"{n_rows} image rows (every {stride}th row)"
     else:
         reps = max(1, int(round(budget_s / max(full_s, 1e-3))))
         n_rays, spent = 0, 0.0
         for _ in range(reps):
-            n, dt, n_rows = run_rows(1)
+            n, dt, n_rows = run_rows(1, threads)
             n_rays += n
             spent += dt
         what = f"the whole frame ({HEIGHT} rows) {reps} times"
+    # One thread: a row sample sized from the multi-threaded rate.
+    one_budget = budget_s / 3.0
+    est_one_s = (spent / max(n_rays, 1)) * threads * (2 * WIDTH * HEIGHT)
+    stride1 = max(1, int(np.ceil(est_one_s / one_budget)))
+    n1, dt1, rows1 = run_rows(stride1, 1)
     return {"value": round(n_rays / spent / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
+            "single_thread": {"value": round(n1 / dt1 / 1e6, 3), "unit": "Mrays/s",
+                              "sample": f"{rows1} image rows (every {stride1}th row), "
+                                        f"{n1} rays in {dt1:.1f} s"},
+            "host": {**host, "omp_num_threads_env": env_threads or None},
+            "grid_build_single_thread_ms": round(build_s * 1e3, 1),
             "sample": (f"CPU oracle on {what}: primary closest-hit + shadow any-hit, "
-                       f"{n_rays} rays in {spent:.1f} s, OpenMP schedule(dynamic) over 16-row "
-                       f"chunks, same grid (lambda {frame.lam:g}); single-thread grid build of "
-                       f"the frame {build_s * 1e3:.0f} ms")}
+                       f"{n_rays} rays in {spent:.1f} s on {threads} threads (every CPU "
+                       f"the process may use, or OMP_NUM_THREADS when set), OpenMP "
+                       f"schedule(dynamic) over 16-row chunks, same grid (lambda "
+                       f"{frame.lam:g}); single-thread grid build of the frame "
+                       f"{build_s * 1e3:.0f} ms")}
 
 
 def main():
@@ -471,20 +507,26 @@ def main():
     achieved = dom_bytes / (trace_ms[dom_stage] * 1e-3) / 1e9
     # DRAM bytes of that launch at this lambda, from the committed
     # FETCH_SIZE / WRITE_SIZE profile (null if it has no such line).
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "trace_stadium_traffic.md")
-    if os.path.exists(tpath):
-        key = "hbm_bytes_per_launch[lambda=%g][%s][%s]:" % (lam, dom_stage, dom)
-        for line in open(tpath):
-            if line.startswith(key):
-                traffic = float(line.split(":", 1)[1])
+    traffic_of = {name: profile_traffic(lam, sched, frame.lists_kind, name,
+                                        kernel_name(closest, sched[i]))
+                  for i, (name, closest) in enumerate(STAGES)}
+    traffic = traffic_of[dom_stage]
     per_stage_roof = {}
     for name, _ in STAGES:
         gbs = counters[name]["algorithmic_bytes"] / (trace_ms[name] * 1e-3) / 1e9
         per_stage_roof[name] = {"gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if traffic_of[name] is not None:
+            mgbs = traffic_of[name] / (trace_ms[name] * 1e-3) / 1e9
+            per_stage_roof[name].update({"measured_gbs": round(mgbs, 1),
+                                         "measured_frac": round(mgbs / HBM_PEAK_GBS, 4)})
     ps_bytes = counters["primary"]["algorithmic_bytes"] + counters["shadow"]["algorithmic_bytes"]
     ps_ms = trace_ms["primary"] + trace_ms["shadow"]
     ps_gbs = ps_bytes / (ps_ms * 1e-3) / 1e9
+    roof_ps = {"gbs": round(ps_gbs, 1), "frac": round(ps_gbs / HBM_PEAK_GBS, 4)}
+    if traffic_of["primary"] is not None and traffic_of["shadow"] is not None:
+        mgbs = (traffic_of["primary"] + traffic_of["shadow"]) / (ps_ms * 1e-3) / 1e9
+        roof_ps.update({"measured_gbs": round(mgbs, 1),
+                        "measured_frac": round(mgbs / HBM_PEAK_GBS, 4)})
 
     out = {
         "metric": "Mrays/s (primary+shadow) at 1 GPU; p50/p99 request latency, serve vs plain",
@@ -507,10 +549,16 @@ def main():
             "rays_per_frame": rays,
             "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"]},
             "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
-            "mrays_primary_shadow_only": round((rays["primary"] + rays["shadow"]) /
-                                               (ps_ms * 1e-3) / 1e6, 2),
-            "roofline_primary_shadow": {"gbs": round(ps_gbs, 1),
-                                        "frac": round(ps_gbs / HBM_PEAK_GBS, 4)},
+            "value_definition": "every ray the frame traces -- closest-hit rays (primary, "
+                                "mirror bounce) and shadow rays of both levels, inactive "
+                                "slots excluded -- per second of whole-frame time "
+                                "(transform, grid rebuild, ray generation, traversal, "
+                                "Phong shading)",
+            "mrays_primary_shadow_stages": round((rays["primary"] + rays["shadow"]) /
+                                                 (ps_ms * 1e-3) / 1e6, 2),
+            "mrays_primary_shadow_per_frame_time": round((rays["primary"] + rays["shadow"]) /
+                                                         (ms_per_step * 1e-3) / 1e6, 2),
+            "roofline_primary_shadow": roof_ps,
             "roofline_by_stage": per_stage_roof,
             "stage_counters": counters,
             "traversal": {"schedule_per_stage": sched, "refill": args.refill,
@@ -522,6 +570,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
+                     "measured_frac": (None if traffic is None else
+                                       round(traffic / (trace_ms[dom_stage] * 1e-3) / 1e9 /
+                                             HBM_PEAK_GBS, 4)),
                      "kernel": "grt::" + dom, "stage": dom_stage,
                      "avg_launch_ms": round(trace_ms[dom_stage], 3),
                      "algorithmic_bytes_per_launch": dom_bytes},
@@ -538,6 +589,21 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(frame, torch, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def profile_traffic(lam, sched, lists, stage, kernel):
+    """HBM bytes per launch of a stage's traversal kernel at this lambda,
+    schedule and ray-list choice, from the committed FETCH_SIZE / WRITE_SIZE
+    profile (None when the profile has no such line)."""
+    tpath = os.path.join(ROOT, "profiles", "trace_stadium_traffic.md")
+    if not os.path.exists(tpath):
+        return None
+    key = "hbm_bytes_per_launch[lambda=%g][sched=%s][lists=%s][%s][%s]:" % (
+        lam, sched, lists, stage, kernel)
+    for line in open(tpath):
+        if line.startswith(key):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def copy_calibration(torch, n=1 << 28, reps=3):
