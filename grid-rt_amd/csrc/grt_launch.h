@@ -163,6 +163,25 @@ hipError_t launch_gen_ao(uint32_t n, uint32_t spp, uint64_t seed, const RaySoA& 
                          float ao_tmax, const RaySoAOut& out, hipStream_t s);
 hipError_t launch_count_active(const float* tmin, const float* tmax, uint32_t n,
                                unsigned long long* count, hipStream_t s);
+// ---- kd-tree comparison accelerator (grt_kdtree_build.cpp, grt_kdtree.hip) ----------
+// Deepest level of a kd-tree; the device traversal keeps one stack entry per
+// level.
+constexpr uint32_t kKdMaxDepth = 48;
+struct KdArgs {
+    float bmin[3], bmax[3];   // the tree's box (tight box of the vertices)
+    const uint2* nodes;       // {split bits | first element, (left << 2) | axis | leaf count}
+    const float4* records;    // 3 float4 per leaf element
+    uint32_t n;
+    uint32_t image_width;     // 0: lane i traces ray i; else 8x8 pixel tiles per wave
+    uint32_t* counters;       // NULL or 2 per ray {leaves visited, triangle tests}
+};
+// rec[k] = tri[elements[k]] (3 float4 each), k < n.
+hipError_t launch_kd_gather(const uint32_t* elements, uint32_t n, const float4* tri, float4* rec,
+                            hipStream_t s);
+// Closest hit into `hits`, or any hit into `occluded` when it is set.
+hipError_t launch_kd_trace(const KdArgs& a, const RaySoA& rays, const HitSoA& hits,
+                           uint8_t* occluded, uint32_t block, hipStream_t s);
+
 // ---- instance-level tracing (grt_instance.hip) --------------------------------
 // Row-major 3x4 affine matrix.
 struct Affine {

@@ -56,6 +56,13 @@ class GridInfo(ctypes.Structure):
                 ("n", ctypes.c_int32 * 3), ("m", ctypes.c_uint32), ("r", ctypes.c_uint32)]
 
 
+class KdTreeInfo(ctypes.Structure):
+    _fields_ = [("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3),
+                ("nodes", ctypes.c_uint32), ("leaves", ctypes.c_uint32),
+                ("empty_leaves", ctypes.c_uint32), ("elements", ctypes.c_uint32),
+                ("depth", ctypes.c_uint32), ("build_ms", ctypes.c_double)]
+
+
 class Instance(ctypes.Structure):
     """grt_instance: a scene and its row-major 3x4 object-to-world matrix."""
     _fields_ = [("scene", ctypes.c_void_p), ("xf", ctypes.c_float * 12)]
@@ -84,6 +91,14 @@ _U32 = ctypes.c_uint32
 _F = ctypes.c_float
 _SIGS = {
     "grt_last_error": (ctypes.c_char_p, []),
+    "grt_kdtree_build": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "grt_kdtree_build_host": (ctypes.c_int, [_P, _U32, ctypes.POINTER(_P)]),
+    "grt_kdtree_destroy": (ctypes.c_int, [_P]),
+    "grt_kdtree_info_get": (ctypes.c_int, [_P, ctypes.POINTER(KdTreeInfo)]),
+    "grt_kdtree_download": (ctypes.c_int, [_P, _P, _P]),
+    "grt_kdtree_trace_closest": (ctypes.c_int, [_P, ctypes.POINTER(Rays), ctypes.POINTER(Hits),
+                                                _U32, _U32, _P]),
+    "grt_kdtree_trace_any": (ctypes.c_int, [_P, ctypes.POINTER(Rays), _P, _U32, _U32, _P]),
     "grt_init": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
     "grt_shutdown": (ctypes.c_int, [_P]),
     "grt_set_stream": (ctypes.c_int, [_P, _P]),
@@ -666,6 +681,66 @@ class Scene:
         if getattr(self, "h", None):
             lib().grt_scene_destroy(self.h)
             self.h = None
+
+
+class KdTree:
+    """The kd-tree comparison accelerator (grt_kdtree_*): a surface-area-
+    heuristic kd-tree over a scene's current triangles, built on the host and
+    traced on the device."""
+
+    def __init__(self, scene: Scene):
+        self.scene = scene
+        h = _P()
+        check(lib().grt_kdtree_build(scene.h, ctypes.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_host(cls, verts: np.ndarray) -> "KdTree":
+        """The same tree built from host vertices without a device (no traces)."""
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+        self = cls.__new__(cls)
+        self.scene = None
+        h = _P()
+        check(lib().grt_kdtree_build_host(_ptr(verts), verts.shape[0], ctypes.byref(h)))
+        self.h = h
+        return self
+
+    def info(self) -> dict:
+        i = KdTreeInfo()
+        check(lib().grt_kdtree_info_get(self.h, ctypes.byref(i)))
+        return {"bmin": np.array(i.bmin[:], np.float32), "bmax": np.array(i.bmax[:], np.float32),
+                "nodes": int(i.nodes), "leaves": int(i.leaves),
+                "empty_leaves": int(i.empty_leaves), "elements": int(i.elements),
+                "depth": int(i.depth), "build_ms": float(i.build_ms)}
+
+    def download(self):
+        """(nodes [n, 2] uint32, elements [m] uint32) as built."""
+        inf = self.info()
+        nodes = np.empty((inf["nodes"], 2), np.uint32)
+        elements = np.empty(max(inf["elements"], 1), np.uint32)
+        check(lib().grt_kdtree_download(self.h, _ptr(nodes), _ptr(elements)))
+        return nodes, elements[:inf["elements"]]
+
+    def trace_closest(self, rays: RayBatch, hits: HitBatch, image_width: int = 0,
+                      counters=None):
+        check(lib().grt_kdtree_trace_closest(self.h, ctypes.byref(rays.c()),
+                                             ctypes.byref(hits.c()), rays.n, image_width,
+                                             _ptr(counters)))
+
+    def trace_any(self, rays: RayBatch, occluded, image_width: int = 0, counters=None):
+        check(lib().grt_kdtree_trace_any(self.h, ctypes.byref(rays.c()), _ptr(occluded), rays.n,
+                                         image_width, _ptr(counters)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().grt_kdtree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -247,6 +247,46 @@ int grt_trace_instances(grt_context ctx, const grt_instance* instances, uint32_t
                         const grt_rays* rays, const grt_hits* hits, int32_t* instance_dev,
                         uint32_t n, uint32_t image_width);
 
+/* ---- kd-tree comparison accelerator ------------------------------------- */
+/* The reference's alternative accelerator (KdTreeAccStructBuilder::
+ * buildGeometry + TriangleTreeBuilder::buildTree, KdTreeAccStructBuilder.cpp:
+ * 21-28, TriangleTreeBuilder.cpp:13-358; KdTreeAccStruct::traceNearestGeometry,
+ * KdTreeAccStruct.cpp:122-240): a surface-area-heuristic kd-tree over a
+ * scene's current triangles, built on the host (perfect splits, traversal cost
+ * 1, intersection cost 1.4, empty-space bias; depth at most 48) and traced on
+ * the device front to back.  Hits use the grid traversal's window with the
+ * leaf's exit distance in place of the cell's, so a closest hit is the same
+ * triangle, t, u and v as the grid's except on exact ties.  The tree keeps its
+ * own copy of the triangle records: rebuild it after the scene changes. */
+typedef struct grt_kdtree_t* grt_kdtree;
+typedef struct {
+    float bmin[3]; float bmax[3];   /* the tree's box: tight box of all vertices */
+    uint32_t nodes;                 /* inner nodes + leaves */
+    uint32_t leaves;
+    uint32_t empty_leaves;
+    uint32_t elements;              /* triangle references in leaves */
+    uint32_t depth;                 /* deepest leaf level (root = 0) */
+    double build_ms;                /* host build + upload */
+} grt_kdtree_info;
+/* Build from the scene's current vertices (synchronises the stream). */
+int grt_kdtree_build(grt_scene scene, grt_kdtree* out);
+/* The same tree from host vertices (9 floats per triangle), without a device:
+ * info and download work, the trace calls return GRT_NOT_READY. */
+int grt_kdtree_build_host(const float* verts_host, uint32_t n_tris, grt_kdtree* out);
+int grt_kdtree_destroy(grt_kdtree tree);
+int grt_kdtree_info_get(grt_kdtree tree, grt_kdtree_info* out);
+/* The tree as built: 2 uint32 per node -- inner {split bits, (left << 2) |
+ * axis}, the right child at left + 1, nodes in breadth-first order; leaf
+ * {first element, 0x80000000 | count} -- and the triangle id of every leaf
+ * element (either pointer may be NULL). */
+int grt_kdtree_download(grt_kdtree tree, uint32_t* nodes_host, uint32_t* elements_host);
+/* Closest hit / any hit, as grt_trace_closest / grt_trace_any; counters_dev:
+ * NULL or 2 uint32 per ray {leaves visited, triangle tests}. */
+int grt_kdtree_trace_closest(grt_kdtree tree, const grt_rays* rays, const grt_hits* hits,
+                             uint32_t n, uint32_t image_width, uint32_t* counters_dev);
+int grt_kdtree_trace_any(grt_kdtree tree, const grt_rays* rays, uint8_t* occluded_dev,
+                         uint32_t n, uint32_t image_width, uint32_t* counters_dev);
+
 /* ---- ray lists ---------------------------------------------------------- */
 /* A secondary stage leaves many slots without a ray (pixels that missed, points
  * facing away from the light).  grt_compact_rays lists the active rays of a

@@ -540,6 +540,166 @@ void orc_trace_any(const orc_frame* f, const uint32_t* offsets, const uint32_t* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* kd-tree traversal (the comparison accelerator)                            */
+/* ------------------------------------------------------------------------ */
+#define ORC_KD_MAX_DEPTH 64
+
+/* KdTreeAccStruct::traceNearestGeometry / findLeaf (KdTreeAccStruct.cpp:
+ * 122-201) over a flattened tree: node k is words [2k, 2k+1] -- inner
+ * {split bits, (left << 2) | axis} with the right child at left + 1, leaf
+ * {first element, 0x80000000 | count}; elements[] are triangle ids.  The ray
+ * is clipped to box[0..5] (min xyz, max xyz) like the grid; d < tnear takes
+ * the far child, d > tfar the near child, else the near child with
+ * tfar = min(d, tfar) and the far child stacked with [max(d, tnear), tfar]
+ * (selects as written, so a NaN d keeps the interval).  Closest hit: leaf
+ * triangles tested with the grid's window, the leaf's tfar as the exit
+ * bound; the first leaf with a hit ends the ray.  Any hit: (tmin, tmax]. */
+static void kd_trace_one(const float box[6], const uint32_t* nodes, const uint32_t* elements,
+                         const float* verts, const float o[3], const float d[3], float tmin,
+                         float tmax, int any, int32_t* hid, float* ht, float* hu, float* hv,
+                         uint8_t* occ, orc_counters* cnt)
+{
+    int32_t id = -1;
+    float best = INFINITY, bu = 0.0f, bv = 0.0f;
+    int hit_any = 0;
+    uint32_t leaves = 0, tests = 0;
+    float inv[3];
+    for (int i = 0; i < 3; ++i)
+        inv[i] = 1.0f / d[i];
+    float tn = tmin, tf = tmax;
+    for (int i = 0; i < 3; ++i) {
+        const float a = (box[i] - o[i]) * inv[i];
+        const float b = (box[3 + i] - o[i]) * inv[i];
+        tn = tmax_f(tn, tmin_f(a, b));
+        tf = tmin_f(tf, tmax_f(a, b));
+    }
+    if (tn <= tf) {
+        uint32_t st_node[ORC_KD_MAX_DEPTH];
+        float st_tn[ORC_KD_MAX_DEPTH], st_tf[ORC_KD_MAX_DEPTH];
+        int sp = 0;
+        uint32_t node = 0;
+        for (;;) {
+            while (!(nodes[2 * node + 1] & 0x80000000u)) {
+                const uint32_t w = nodes[2 * node + 1];
+                const int axis = (int)(w & 3u);
+                const uint32_t left = w >> 2;
+                float split;
+                memcpy(&split, &nodes[2 * node], 4);
+                const float dist = (split - o[axis]) * inv[axis];
+                const int neg = signbit(d[axis]) != 0;
+                const uint32_t front = left + (neg ? 1u : 0u);
+                const uint32_t back = left + (neg ? 0u : 1u);
+                if (dist < tn) {
+                    node = back;
+                } else if (dist > tf) {
+                    node = front;
+                } else {
+                    if (sp < ORC_KD_MAX_DEPTH) {
+                        st_node[sp] = back;
+                        st_tn[sp] = dist > tn ? dist : tn;
+                        st_tf[sp] = tf;
+                        ++sp;
+                    }
+                    node = front;
+                    tf = dist < tf ? dist : tf;
+                }
+            }
+            ++leaves;
+            const uint32_t first = nodes[2 * node];
+            const uint32_t end = first + (nodes[2 * node + 1] & 0x7fffffffu);
+            if (any) {
+                for (uint32_t k = first; k < end; ++k) {
+                    ++tests;
+                    float t, u, v;
+                    if (mt_core(verts + 9ull * elements[k], o, d, &t, &u, &v) && t > tmin &&
+                        t <= tmax) {
+                        hit_any = 1;
+                        break;
+                    }
+                }
+                if (hit_any)
+                    break;
+            } else {
+                const float bound = tf * ORC_EXIT_SCALE;
+                for (uint32_t k = first; k < end; ++k) {
+                    const uint32_t tri = elements[k];
+                    ++tests;
+                    float t, u, v;
+                    if (mt_core(verts + 9ull * tri, o, d, &t, &u, &v) && t > tmin && t <= tmax &&
+                        t <= bound && t < best) {
+                        best = t;
+                        id = (int32_t)tri;
+                        bu = u;
+                        bv = v;
+                    }
+                }
+                if (id >= 0)
+                    break;
+            }
+            if (sp == 0)
+                break;
+            --sp;
+            node = st_node[sp];
+            tn = st_tn[sp];
+            tf = st_tf[sp];
+        }
+    }
+    if (any) {
+        *occ = (uint8_t)hit_any;
+    } else {
+        *hid = id;
+        *ht = best;
+        *hu = bu;
+        *hv = bv;
+    }
+    if (cnt) {
+        cnt->cells = leaves;
+        cnt->tests = tests;
+    }
+}
+
+void orc_kd_trace_closest(const float box[6], const uint32_t* nodes, const uint32_t* elements,
+                          const float* verts, uint32_t n_rays,
+                          const float* ox, const float* oy, const float* oz,
+                          const float* dx, const float* dy, const float* dz,
+                          const float* tmin, const float* tmax,
+                          int32_t* hit_id, float* hit_t, float* hit_u, float* hit_v,
+                          orc_counters* cnt, int n_threads, int chunk)
+{
+    if (chunk < 1)
+        chunk = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, chunk) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
+#endif
+    for (int64_t i = 0; i < (int64_t)n_rays; ++i) {
+        const float o[3] = {ox[i], oy[i], oz[i]};
+        const float d[3] = {dx[i], dy[i], dz[i]};
+        kd_trace_one(box, nodes, elements, verts, o, d, tmin[i], tmax[i], 0, &hit_id[i],
+                     &hit_t[i], &hit_u[i], &hit_v[i], NULL, cnt ? &cnt[i] : NULL);
+    }
+}
+
+void orc_kd_trace_any(const float box[6], const uint32_t* nodes, const uint32_t* elements,
+                      const float* verts, uint32_t n_rays,
+                      const float* ox, const float* oy, const float* oz,
+                      const float* dx, const float* dy, const float* dz,
+                      const float* tmin, const float* tmax,
+                      uint8_t* occluded, orc_counters* cnt, int n_threads, int chunk)
+{
+    if (chunk < 1)
+        chunk = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, chunk) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
+#endif
+    for (int64_t i = 0; i < (int64_t)n_rays; ++i) {
+        const float o[3] = {ox[i], oy[i], oz[i]};
+        const float d[3] = {dx[i], dy[i], dz[i]};
+        kd_trace_one(box, nodes, elements, verts, o, d, tmin[i], tmax[i], 1, NULL, NULL, NULL,
+                     NULL, &occluded[i], cnt ? &cnt[i] : NULL);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* a10-a12: secondary rays                                                   */
 /* ------------------------------------------------------------------------ */
 

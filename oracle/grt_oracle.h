@@ -119,6 +119,25 @@ void orc_trace_any(const orc_frame* f, const uint32_t* offsets, const uint32_t* 
                    const float* tmin, const float* tmax,
                    uint8_t* occluded, orc_counters* cnt, int n_threads, int chunk);
 
+/* ---- kd-tree traversal (comparison accelerator) ---------------------------- */
+/* box: min xyz, max xyz of the tree; nodes: 2 words per node (inner {split
+ * bits, (left << 2) | axis}, leaf {first element, 0x80000000 | count});
+ * elements: triangle id per leaf element.  Counters: {leaves visited, tests}.
+ * Trees deeper than 64 levels are not supported. */
+void orc_kd_trace_closest(const float box[6], const uint32_t* nodes, const uint32_t* elements,
+                          const float* verts, uint32_t n_rays,
+                          const float* ox, const float* oy, const float* oz,
+                          const float* dx, const float* dy, const float* dz,
+                          const float* tmin, const float* tmax,
+                          int32_t* hit_id, float* hit_t, float* hit_u, float* hit_v,
+                          orc_counters* cnt, int n_threads, int chunk);
+void orc_kd_trace_any(const float box[6], const uint32_t* nodes, const uint32_t* elements,
+                      const float* verts, uint32_t n_rays,
+                      const float* ox, const float* oy, const float* oz,
+                      const float* dx, const float* dy, const float* dz,
+                      const float* tmin, const float* tmax,
+                      uint8_t* occluded, orc_counters* cnt, int n_threads, int chunk);
+
 /* Single Moller-Trumbore test (exposed for known-answer tests). Returns 1 on
  * hit inside (tmin, tmax] and writes t,u,v. */
 int orc_mt(const float* v9, const float o[3], const float d[3], float tmin, float tmax,

@@ -83,6 +83,10 @@ def lib():
                                   + [_P] * 4 + [_P, ctypes.c_int, ctypes.c_int]),
             "orc_trace_any": (None, [ctypes.POINTER(Frame), _P, _P, _P, _U32] + [_P] * 8
                               + [_P, _P, ctypes.c_int, ctypes.c_int]),
+            "orc_kd_trace_closest": (None, [_P, _P, _P, _P, _U32] + [_P] * 8 + [_P] * 4
+                                     + [_P, ctypes.c_int, ctypes.c_int]),
+            "orc_kd_trace_any": (None, [_P, _P, _P, _P, _U32] + [_P] * 8
+                                 + [_P, _P, ctypes.c_int, ctypes.c_int]),
             "orc_dda_cells": (_U32, [ctypes.POINTER(Frame), _F * 3, _F * 3, _F, _F, _P, _U32]),
             "orc_mt": (ctypes.c_int, [_P, _F * 3, _F * 3, _F, _F, ctypes.POINTER(_F),
                                       ctypes.POINTER(_F), ctypes.POINTER(_F)]),
@@ -251,6 +255,55 @@ def trace_any(frame: dict, offsets, refs, verts, rays: dict, threads: int = 0,
     out = {"occluded": occ}
     if counters:
         out["cells"] = cnt[:, 0].copy()
+        out["tests"] = cnt[:, 1].copy()
+    return out
+
+
+def _kd_args(box, nodes, elements):
+    box = np.ascontiguousarray(np.concatenate([np.asarray(box[0], np.float32),
+                                               np.asarray(box[1], np.float32)]), np.float32)
+    nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+    elements = np.ascontiguousarray(elements, np.uint32)
+    if elements.size == 0:
+        elements = np.zeros(1, np.uint32)
+    return box, nodes, elements
+
+
+def kd_trace_closest(box, nodes, elements, verts, rays: dict, threads: int = 0,
+                     counters: bool = False, chunk: int = 64):
+    """Closest hit through a flattened kd-tree; box = (bmin, bmax)."""
+    box, nodes, elements = _kd_args(box, nodes, elements)
+    verts = _f32(verts).reshape(-1, 9)
+    r = _rays(rays)
+    n = r["ox"].shape[0]
+    hid = np.empty(n, np.int32)
+    ht, hu, hv = (np.empty(n, np.float32) for _ in range(3))
+    cnt = np.empty((n, 2), np.uint32) if counters else None
+    threads = threads or lib().orc_max_threads()
+    lib().orc_kd_trace_closest(_p(box), _p(nodes), _p(elements), _p(verts), n,
+                               *[_p(r[k]) for k in RAY_FIELDS], _p(hid), _p(ht), _p(hu), _p(hv),
+                               _p(cnt), threads, chunk)
+    out = {"id": hid, "t": ht, "u": hu, "v": hv}
+    if counters:
+        out["leaves"] = cnt[:, 0].copy()
+        out["tests"] = cnt[:, 1].copy()
+    return out
+
+
+def kd_trace_any(box, nodes, elements, verts, rays: dict, threads: int = 0,
+                 counters: bool = False, chunk: int = 64):
+    box, nodes, elements = _kd_args(box, nodes, elements)
+    verts = _f32(verts).reshape(-1, 9)
+    r = _rays(rays)
+    n = r["ox"].shape[0]
+    occ = np.empty(n, np.uint8)
+    cnt = np.empty((n, 2), np.uint32) if counters else None
+    threads = threads or lib().orc_max_threads()
+    lib().orc_kd_trace_any(_p(box), _p(nodes), _p(elements), _p(verts), n,
+                           *[_p(r[k]) for k in RAY_FIELDS], _p(occ), _p(cnt), threads, chunk)
+    out = {"occluded": occ}
+    if counters:
+        out["leaves"] = cnt[:, 0].copy()
         out["tests"] = cnt[:, 1].copy()
     return out
 
