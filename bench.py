@@ -72,10 +72,11 @@ def parse():
                         "references left (0 = off)")
     p.add_argument("--lam", type=float, default=None,
                    help="grid resolution factor; default: tuned over %s" % (LAMBDAS,))
-    p.add_argument("--mode", default="frame", choices=["frame", "serve", "batch", "ao"],
+    p.add_argument("--mode", default="frame", choices=["frame", "serve", "batch", "ao", "kdtree"],
                    help="frame: the stadium frame (default, headline line, with a short "
                         "serve-vs-plain latency section); serve / batch / ao: the serving, "
-                        "batching and ambient-occlusion workloads on their own")
+                        "batching and ambient-occlusion workloads on their own; kdtree: the "
+                        "kd-tree comparison accelerator against the grid on the stadium frame")
     p.add_argument("--no-serve", action="store_true", help="frame mode without the latency run")
     p.add_argument("--serve-requests", type=int, default=120)
     p.add_argument("--profile", action="store_true",
@@ -444,6 +445,8 @@ def main():
             res["triangles"] = tris
         elif args.mode == "batch":
             res = perf.batch_config()
+        elif args.mode == "kdtree":
+            res = perf.kdtree_config(lam=args.lam or 256.0)
         else:
             res = perf.ao_config()
         print(json.dumps({"mode": args.mode, **res}))

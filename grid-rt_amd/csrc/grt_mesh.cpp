@@ -13,7 +13,10 @@
 //                        the fan (c0, ci, ci+1); missing normals are (0,0,0);
 //                        usemtl selects the MTL material's Kd as the
 //                        triangles' diffuse colour
-//   procedural scenes    rtdb::loadSphere (rtdb.cpp:142-192) and a unit cube
+//   procedural scenes    rtdb::loadSphere (rtdb.cpp:142-192), a unit cube, and
+//                        the layouts of rtdb.cpp:9-57,215-287 (triangle, logo,
+//                        evenly spaced triangles, the mixed scene, 125 copies
+//                        of a model), the teapot given as a mesh
 //
 // Everything here is host code; the result is 9 floats of vertices, 9 of
 // normals and 3 of diffuse colour per triangle.
@@ -509,6 +512,195 @@ int grt_mesh_add_cube(grt_mesh m)
                 grt_mesh_vertex(m, c[k][0], c[k][1], c[k][2]);
             grt_mesh_end(m);
         }
+    return GRT_OK;
+}
+
+// rtdb::loadTriangle (rtdb.cpp:9-18): (0,0,0), (1,0,0), (0,1,0), normal +z.
+int grt_mesh_add_triangle(grt_mesh m)
+{
+    if (!mesh_ok(m, "grt_mesh_add_triangle"))
+        return GRT_INVALID_PARAM;
+    if (m->open)
+        return set_error(GRT_INVALID_PARAM, "grt_mesh_add_triangle: a primitive is open");
+    grt_mesh_normal(m, 0.0f, 0.0f, 1.0f);
+    grt_mesh_begin(m, GRT_TRIANGLES);
+    grt_mesh_vertex(m, 0.0f, 0.0f, 0.0f);
+    grt_mesh_vertex(m, 1.0f, 0.0f, 0.0f);
+    grt_mesh_vertex(m, 0.0f, 1.0f, 0.0f);
+    return grt_mesh_end(m);
+}
+
+// rtdb::loadLogo (rtdb.cpp:20-57): three triangles in the z = 0 plane, normal
+// +z.  The reference coloured each vertex; colour is per triangle here, so
+// the logo takes the current colour.
+int grt_mesh_add_logo(grt_mesh m)
+{
+    if (!mesh_ok(m, "grt_mesh_add_logo"))
+        return GRT_INVALID_PARAM;
+    if (m->open)
+        return set_error(GRT_INVALID_PARAM, "grt_mesh_add_logo: a primitive is open");
+    static const float p[9][2] = {{0.0f, 0.0f}, {1.0f, 0.0f}, {0.0f, 1.0f},
+                                  {2.0f, 0.0f}, {3.0f, 0.0f}, {3.0f, 1.0f},
+                                  {1.5f, 0.0f}, {2.5f, 1.0f}, {0.5f, 1.0f}};
+    grt_mesh_normal(m, 0.0f, 0.0f, 1.0f);
+    grt_mesh_begin(m, GRT_TRIANGLES);
+    for (const auto& q : p)
+        grt_mesh_vertex(m, q[0], q[1], 0.0f);
+    return grt_mesh_end(m);
+}
+
+// Append model's triangles through m's current transform, as if each had been
+// added as a GRT_TRIANGLES primitive: vertices as points, normals through the
+// inverse transpose and renormalised, the degenerate filter applied again,
+// each triangle keeping its diffuse colour.
+int grt_mesh_add_mesh(grt_mesh m, grt_mesh model)
+{
+    if (!mesh_ok(m, "grt_mesh_add_mesh") || !mesh_ok(model, "grt_mesh_add_mesh"))
+        return GRT_INVALID_PARAM;
+    if (m->open)
+        return set_error(GRT_INVALID_PARAM, "grt_mesh_add_mesh: a primitive is open");
+    if (m == model)
+        return set_error(GRT_INVALID_PARAM, "grt_mesh_add_mesh: a mesh cannot add itself");
+    const size_t nt = model->source.size();
+    const float saved_color[3] = {m->color[0], m->color[1], m->color[2]};
+    const float saved_normal[3] = {m->normal[0], m->normal[1], m->normal[2]};
+    for (size_t t = 0; t < nt; ++t) {
+        const float* v = &model->verts[9 * t];
+        const float* n = &model->normals[9 * t];
+        const float* kd = &model->diffuse[3 * t];
+        m->color[0] = kd[0];
+        m->color[1] = kd[1];
+        m->color[2] = kd[2];
+        grt_mesh_begin(m, GRT_TRIANGLES);
+        for (int k = 0; k < 3; ++k) {
+            grt_mesh_normal(m, n[3 * k], n[3 * k + 1], n[3 * k + 2]);
+            grt_mesh_vertex(m, v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+        }
+        grt_mesh_end(m);
+    }
+    std::memcpy(m->color, saved_color, sizeof(saved_color));
+    std::memcpy(m->normal, saved_normal, sizeof(saved_normal));
+    return GRT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The reference's MatrixStack::scale/translate post-multiply the current
+// matrix (MatrixStack.cpp:26-38): the new operation applies to a vertex first.
+void affine_mul(const float a[12], const float b[12], float out[12])
+{
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+            out[4 * r + c] = (a[4 * r] * b[c] + a[4 * r + 1] * b[4 + c]) + a[4 * r + 2] * b[8 + c];
+        out[4 * r + 3] = ((a[4 * r] * b[3] + a[4 * r + 1] * b[7]) + a[4 * r + 2] * b[11]) +
+                         a[4 * r + 3];
+    }
+}
+
+void translated(const float base[12], float x, float y, float z, float out[12])
+{
+    const float t[12] = {1, 0, 0, x, 0, 1, 0, y, 0, 0, 1, z};
+    affine_mul(base, t, out);
+}
+
+void scaled(const float base[12], float s, float out[12])
+{
+    const float t[12] = {s, 0, 0, 0, 0, s, 0, 0, 0, 0, s, 0};
+    affine_mul(base, t, out);
+}
+
+// Run `add` with m's transform set to xf, then restore the transform.
+template <class F>
+int with_transform(grt_mesh m, const float xf[12], F add)
+{
+    float saved[12];
+    std::memcpy(saved, m->xf, sizeof(saved));
+    int rc = grt_mesh_set_transform(m, xf);
+    if (rc == GRT_OK)
+        rc = add();
+    const int rc2 = grt_mesh_set_transform(m, saved);
+    return rc != GRT_OK ? rc : rc2;
+}
+
+}  // namespace
+
+extern "C" {
+
+// rtdb::loadEvenSpaced (rtdb.cpp:215-233): the unit triangle at x, y, z in
+// {0, 1.5, 3} (27 copies, x fastest).
+int grt_mesh_add_even_spaced(grt_mesh m)
+{
+    if (!mesh_ok(m, "grt_mesh_add_even_spaced"))
+        return GRT_INVALID_PARAM;
+    const float base[12] = {m->xf[0], m->xf[1], m->xf[2],  m->xf[3],
+                            m->xf[4], m->xf[5], m->xf[6],  m->xf[7],
+                            m->xf[8], m->xf[9], m->xf[10], m->xf[11]};
+    for (int z = 0; z < 3; ++z)
+        for (int y = 0; y < 3; ++y)
+            for (int x = 0; x < 3; ++x) {
+                float xf[12];
+                translated(base, 1.5f * (float)x, 1.5f * (float)y, 1.5f * (float)z, xf);
+                int rc = with_transform(m, xf, [&]() { return grt_mesh_add_triangle(m); });
+                if (rc != GRT_OK)
+                    return rc;
+            }
+    return GRT_OK;
+}
+
+// rtdb::loadMixed (rtdb.cpp:235-266): the logo at (1, 2, 2.5); unit cubes at
+// (0,5,0), (5,5,0), (5,5,5), (0,5,5); and `model` (the reference's teapot)
+// scaled by 5 with offsets (0,0,0), (5,0,0), (5,0,5), (0,0,5) before the
+// scale.  The translations accumulate as in the reference.
+int grt_mesh_add_mixed(grt_mesh m, grt_mesh model)
+{
+    if (!mesh_ok(m, "grt_mesh_add_mixed") || !mesh_ok(model, "grt_mesh_add_mixed"))
+        return GRT_INVALID_PARAM;
+    const float base[12] = {m->xf[0], m->xf[1], m->xf[2],  m->xf[3],
+                            m->xf[4], m->xf[5], m->xf[6],  m->xf[7],
+                            m->xf[8], m->xf[9], m->xf[10], m->xf[11]};
+    float xf[12], next[12];
+    translated(base, 1.0f, 2.0f, 2.5f, xf);
+    int rc = with_transform(m, xf, [&]() { return grt_mesh_add_logo(m); });
+    static const float steps[4][3] = {{0, 5, 0}, {5, 0, 0}, {0, 0, 5}, {-5, 0, 0}};
+    std::memcpy(xf, base, sizeof(xf));
+    for (int k = 0; k < 4 && rc == GRT_OK; ++k) {
+        translated(xf, steps[k][0], steps[k][1], steps[k][2], next);
+        std::memcpy(xf, next, sizeof(xf));
+        rc = with_transform(m, xf, [&]() { return grt_mesh_add_cube(m); });
+    }
+    static const float tsteps[4][3] = {{0, 0, 0}, {5, 0, 0}, {0, 0, 5}, {-5, 0, 0}};
+    scaled(base, 5.0f, xf);
+    for (int k = 0; k < 4 && rc == GRT_OK; ++k) {
+        translated(xf, tsteps[k][0], tsteps[k][1], tsteps[k][2], next);
+        std::memcpy(xf, next, sizeof(xf));
+        rc = with_transform(m, xf, [&]() { return grt_mesh_add_mesh(m, model); });
+    }
+    return rc;
+}
+
+// rtdb::loadTeapots125 (rtdb.cpp:268-287): 125 copies of `model` (the
+// reference's teapot, 125 x 2256 = 282k triangles there), each scaled by 5
+// after a translation by (2x, 2y, 2z), x, y, z in 0..4 (x fastest).
+int grt_mesh_add_teapots125(grt_mesh m, grt_mesh model)
+{
+    if (!mesh_ok(m, "grt_mesh_add_teapots125") || !mesh_ok(model, "grt_mesh_add_teapots125"))
+        return GRT_INVALID_PARAM;
+    const float base[12] = {m->xf[0], m->xf[1], m->xf[2],  m->xf[3],
+                            m->xf[4], m->xf[5], m->xf[6],  m->xf[7],
+                            m->xf[8], m->xf[9], m->xf[10], m->xf[11]};
+    float s5[12];
+    scaled(base, 5.0f, s5);
+    for (int z = 0; z < 5; ++z)
+        for (int y = 0; y < 5; ++y)
+            for (int x = 0; x < 5; ++x) {
+                float xf[12];
+                translated(s5, (float)(x * 2), (float)(y * 2), (float)(z * 2), xf);
+                int rc = with_transform(m, xf, [&]() { return grt_mesh_add_mesh(m, model); });
+                if (rc != GRT_OK)
+                    return rc;
+            }
     return GRT_OK;
 }
 

@@ -203,16 +203,25 @@ grt_grid_info HipGridAccStruct::gridInfo() const
     return info;
 }
 
+void HipGridAccStruct::traceBatch(const grt_rays& rays, const grt_hits* hits, uint8_t* occluded,
+                                  uint32_t n, uint32_t image_width)
+{
+    if (hits)
+        check(grt_trace_closest(scene_, &rays, hits, n, image_width, nullptr));
+    else
+        check(grt_trace_any(scene_, &rays, occluded, n, image_width, nullptr));
+}
+
 void HipGridAccStruct::traceNearest(const grt_rays& rays, const grt_hits& hits, uint32_t n,
                                     uint32_t image_width)
 {
-    check(grt_trace_closest(scene_, &rays, &hits, n, image_width, nullptr));
+    traceBatch(rays, &hits, nullptr, n, image_width);
 }
 
 void HipGridAccStruct::traceAny(const grt_rays& rays, uint8_t* occluded, uint32_t n,
                                 uint32_t image_width)
 {
-    check(grt_trace_any(scene_, &rays, occluded, n, image_width, nullptr));
+    traceBatch(rays, nullptr, occluded, n, image_width);
 }
 
 // One ray through the batch path: 8 ray floats + 4 hit words + 1 flag byte.
@@ -228,7 +237,7 @@ void HipGridAccStruct::traceOne(Ray& ray, bool any, Hit& hit, bool& found)
     hip_ok(hipMemcpyAsync(f, in, sizeof(in), hipMemcpyHostToDevice, s), "per-ray upload");
     if (any) {
         uint8_t* occ = reinterpret_cast<uint8_t*>(f + 12);
-        check(grt_trace_any(scene_, &r, occ, 1, 0, nullptr));
+        traceBatch(r, nullptr, occ, 1, 0);
         uint8_t o = 0;
         hip_ok(hipMemcpyAsync(&o, occ, 1, hipMemcpyDeviceToHost, s), "per-ray result");
         check(grt_synchronize(ctx_));
@@ -236,7 +245,7 @@ void HipGridAccStruct::traceOne(Ray& ray, bool any, Hit& hit, bool& found)
         return;
     }
     grt_hits h = {reinterpret_cast<int32_t*>(f + 8), f + 9, f + 10, f + 11};
-    check(grt_trace_closest(scene_, &r, &h, 1, 0, nullptr));
+    traceBatch(r, &h, nullptr, 1, 0);
     float out[4];
     hip_ok(hipMemcpyAsync(out, f + 8, sizeof(out), hipMemcpyDeviceToHost, s), "per-ray result");
     check(grt_synchronize(ctx_));
@@ -335,7 +344,10 @@ void HipGridAccStructBuilder::receiveParameter(int paramId, void* paramValue)
     }
 }
 
-void HipGridAccStructBuilder::buildGeometry(Geometry* g)
+// Expand a geometry's indexed triangles into 9 floats of vertices (and of
+// normals when every vertex has one), as CudaRenderer.cpp:86-147 did to float4,
+// and upload them into the accelerator's scene.
+static void upload_geometry(Geometry* g, grt_scene scene)
 {
     if (!g)
         throw Error(GRT_INVALID_PARAM, "buildGeometry: NULL geometry");
@@ -361,16 +373,82 @@ void HipGridAccStructBuilder::buildGeometry(Geometry* g)
                 n[t * 9 + k * 3 + 2] = q.z;
             }
         }
+    check(grt_scene_upload_triangles(scene, v.data(), (uint32_t)nt));
+    if (with_normals)
+        check(grt_scene_upload_normals(scene, n.data(), (uint32_t)nt));
+}
+
+void HipGridAccStructBuilder::buildGeometry(Geometry* g)
+{
+    if (!g)
+        throw Error(GRT_INVALID_PARAM, "buildGeometry: NULL geometry");
     HipGridAccStruct* acc = dynamic_cast<HipGridAccStruct*>(g->accStruct.get());
-    if (!acc) {
+    if (!acc || dynamic_cast<HipKdTreeAccStruct*>(acc)) {
         acc = new HipGridAccStruct(ctx_);
         g->accStruct = acc;
     }
-    check(grt_scene_upload_triangles(acc->scene_, v.data(), (uint32_t)nt));
-    if (with_normals)
-        check(grt_scene_upload_normals(acc->scene_, n.data(), (uint32_t)nt));
+    upload_geometry(g, acc->scene_);
     check(grt_grid_build(acc->scene_, lambda_));
     const grt_grid_info info = acc->gridInfo();
+    Aabb box;
+    box.minv.set(info.bmin[0], info.bmin[1], info.bmin[2]);
+    box.maxv.set(info.bmax[0], info.bmax[1], info.bmax[2]);
+    acc->setBoundingBox(box);
+}
+
+// ---- HipKdTreeAccStruct / HipKdTreeAccStructBuilder --------------------------
+
+HipKdTreeAccStruct::~HipKdTreeAccStruct()
+{
+    if (tree_)
+        grt_kdtree_destroy(tree_);
+}
+
+void HipKdTreeAccStruct::clear()
+{
+    if (tree_)
+        grt_kdtree_destroy(tree_);
+    tree_ = nullptr;
+    HipGridAccStruct::clear();
+}
+
+grt_kdtree_info HipKdTreeAccStruct::treeInfo() const
+{
+    if (!tree_)
+        throw Error(GRT_NOT_READY, "HipKdTreeAccStruct: no tree built");
+    grt_kdtree_info info;
+    check(grt_kdtree_info_get(tree_, &info));
+    return info;
+}
+
+void HipKdTreeAccStruct::traceBatch(const grt_rays& rays, const grt_hits* hits, uint8_t* occluded,
+                                    uint32_t n, uint32_t image_width)
+{
+    if (!tree_)
+        throw Error(GRT_NOT_READY, "HipKdTreeAccStruct: no tree built");
+    if (hits)
+        check(grt_kdtree_trace_closest(tree_, &rays, hits, n, image_width, nullptr));
+    else
+        check(grt_kdtree_trace_any(tree_, &rays, occluded, n, image_width, nullptr));
+}
+
+HipKdTreeAccStructBuilder::HipKdTreeAccStructBuilder(grt_context ctx) : ctx_(ctx) {}
+
+void HipKdTreeAccStructBuilder::buildGeometry(Geometry* g)
+{
+    if (!g)
+        throw Error(GRT_INVALID_PARAM, "buildGeometry: NULL geometry");
+    HipKdTreeAccStruct* acc = dynamic_cast<HipKdTreeAccStruct*>(g->accStruct.get());
+    if (!acc) {
+        acc = new HipKdTreeAccStruct(ctx_);
+        g->accStruct = acc;
+    }
+    upload_geometry(g, acc->scene_);
+    if (acc->tree_)
+        grt_kdtree_destroy(acc->tree_);
+    acc->tree_ = nullptr;
+    check(grt_kdtree_build(acc->scene_, &acc->tree_));
+    const grt_kdtree_info info = acc->treeInfo();
     Aabb box;
     box.minv.set(info.bmin[0], info.bmin[1], info.bmin[2]);
     box.maxv.set(info.bmax[0], info.bmax[1], info.bmax[2]);

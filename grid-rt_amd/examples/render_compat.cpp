@@ -5,11 +5,14 @@
 // traces a few pixels one ray at a time (traceNearestInstance) and checks them
 // against the frame.
 //
-//   render_compat SCENE OUT WIDTH HEIGHT LAMBDA EYE_X EYE_Y EYE_Z AT_X AT_Y AT_Z
+//   render_compat SCENE OUT WIDTH HEIGHT LAMBDA EYE_X EYE_Y EYE_Z AT_X AT_Y AT_Z [kd]
 //
 // SCENE: uint32 n, then 9n floats of vertices and 9n floats of normals.
 // OUT:   int32 id[w*h], float t[w*h], u[w*h], v[w*h], uint32 rgba[w*h],
 //        then the grid's n[3] (int32) and bmin[3], bmax[3] (float).
+// With "kd" the geometry is built by the kd-tree builder plugin instead
+// (KdTreeAccStructBuilder), and the trailer holds n = {-1,-1,-1} and the
+// tree's box.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,8 +28,9 @@ static int fail(const char* msg)
 
 int main(int argc, char** argv)
 {
-    if (argc != 12)
-        return fail("usage: SCENE OUT WIDTH HEIGHT LAMBDA EYE(3) AT(3)");
+    if (argc != 12 && !(argc == 13 && std::strcmp(argv[12], "kd") == 0))
+        return fail("usage: SCENE OUT WIDTH HEIGHT LAMBDA EYE(3) AT(3) [kd]");
+    const bool kd = argc == 13;
     const uint32_t width = (uint32_t)std::atoi(argv[3]), height = (uint32_t)std::atoi(argv[4]);
     float lambda = (float)std::atof(argv[5]);
     float eye[3], at[3];
@@ -62,9 +66,16 @@ int main(int argc, char** argv)
                 geom->normals[i].set(nv[3 * i], nv[3 * i + 1], nv[3 * i + 2]);
                 geom->triDesc[t].v[k] = (uint32_t)i;
             }
-        grt::ref_ptr<grt::HipGridAccStructBuilder> builder(new grt::HipGridAccStructBuilder(ctx));
-        builder->receiveParameter(grt::GRT_PARAM_LAMBDA, &lambda);
-        builder->buildGeometry(geom.get());
+        if (kd) {
+            grt::ref_ptr<grt::HipKdTreeAccStructBuilder> builder(
+                new grt::HipKdTreeAccStructBuilder(ctx));
+            builder->buildGeometry(geom.get());
+        } else {
+            grt::ref_ptr<grt::HipGridAccStructBuilder> builder(
+                new grt::HipGridAccStructBuilder(ctx));
+            builder->receiveParameter(grt::GRT_PARAM_LAMBDA, &lambda);
+            builder->buildGeometry(geom.get());
+        }
 
         grt::ref_ptr<grt::PinholeCamera> cam(new grt::PinholeCamera);
         cam->setLookAt(eye[0], eye[1], eye[2], at[0], at[1], at[2], 0.0f, 1.0f, 0.0f);
@@ -100,8 +111,18 @@ int main(int argc, char** argv)
                 rc = 2;
             }
         }
-        const grt_grid_info info =
-            static_cast<grt::HipGridAccStruct*>(geom->accStruct.get())->gridInfo();
+        grt_grid_info info;
+        if (kd) {
+            const grt_kdtree_info ki =
+                static_cast<grt::HipKdTreeAccStruct*>(geom->accStruct.get())->treeInfo();
+            for (int k = 0; k < 3; ++k) {
+                info.n[k] = -1;
+                info.bmin[k] = ki.bmin[k];
+                info.bmax[k] = ki.bmax[k];
+            }
+        } else {
+            info = static_cast<grt::HipGridAccStruct*>(geom->accStruct.get())->gridInfo();
+        }
         FILE* o = std::fopen(argv[2], "wb");
         if (!o)
             throw grt::Error(GRT_INVALID_PARAM, "cannot open output");

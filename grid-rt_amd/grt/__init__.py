@@ -150,6 +150,12 @@ _SIGS = {
     "grt_mesh_load_obj": (ctypes.c_int, [_P, ctypes.c_char_p, _U32]),
     "grt_mesh_add_sphere": (ctypes.c_int, [_P, _U32, _U32]),
     "grt_mesh_add_cube": (ctypes.c_int, [_P]),
+    "grt_mesh_add_triangle": (ctypes.c_int, [_P]),
+    "grt_mesh_add_logo": (ctypes.c_int, [_P]),
+    "grt_mesh_add_mesh": (ctypes.c_int, [_P, _P]),
+    "grt_mesh_add_even_spaced": (ctypes.c_int, [_P]),
+    "grt_mesh_add_mixed": (ctypes.c_int, [_P, _P]),
+    "grt_mesh_add_teapots125": (ctypes.c_int, [_P, _P]),
     "grt_mesh_info": (ctypes.c_int, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     "grt_mesh_copy": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "grt_scene_upload_mesh": (ctypes.c_int, [_P, _P]),
@@ -522,6 +528,25 @@ class Mesh:
 
     def add_cube(self):
         check(lib().grt_mesh_add_cube(self.h))
+
+    # rtdb's layouts (rtdb.cpp:9-57,215-287); `model` stands for the teapot.
+    def add_triangle(self):
+        check(lib().grt_mesh_add_triangle(self.h))
+
+    def add_logo(self):
+        check(lib().grt_mesh_add_logo(self.h))
+
+    def add_mesh(self, model: "Mesh"):
+        check(lib().grt_mesh_add_mesh(self.h, model.h))
+
+    def add_even_spaced(self):
+        check(lib().grt_mesh_add_even_spaced(self.h))
+
+    def add_mixed(self, model: "Mesh"):
+        check(lib().grt_mesh_add_mixed(self.h, model.h))
+
+    def add_teapots125(self, model: "Mesh"):
+        check(lib().grt_mesh_add_teapots125(self.h, model.h))
 
     def info(self):
         n, d = ctypes.c_uint32(), ctypes.c_uint32()

@@ -8,7 +8,8 @@ import time
 
 import numpy as np
 
-from . import Batch, Context, Graph, HitBatch, RayBatch, Scene, serve_run, stream_create
+from . import (Batch, Context, Graph, HitBatch, KdTree, RayBatch, Scene, camera_lookat,
+               gen_cluster_transforms, gen_stadium, serve_run, stream_create)
 from . import workloads as W
 from .autotune import Autotuner
 
@@ -409,3 +410,112 @@ def ao_config(lams=(4.0, 8.0, 16.0), reps: int = 3, width: int = 1024, height: i
             "ao_trace_gbs": round(gbs, 1), "ao_roofline_frac": round(gbs / 8000.0, 4),
             "build_algorithmic_gbs": round(build_bytes / (build_ms * 1e-3) / 1e9, 1),
             "autotune": tuner.cache["ao-16M"]["tried"]}
+
+
+# ---------------------------------------------------------------------------
+# kd-tree vs grid on the stadium frame
+# ---------------------------------------------------------------------------
+def kdtree_config(lam: float = 256.0, reps: int = 5, width: int = 1920, height: int = 1080):
+    """The kd-tree comparison accelerator against the uniform grid on the
+    stadium frame (2^20 triangles in 256 clusters + the ground plane, frame 0's
+    rigid cluster transforms): the same four traversal stages -- primary,
+    shadow, mirror bounce, bounce shadow, rays generated from the grid's hits --
+    timed per launch (HIP events, median of `reps`) through the grid (best of
+    one ray per lane, pooled tests and 2 / 16 lanes per ray) and through the
+    kd-tree, with the per-ray work of each (cells or leaves, triangle tests) and
+    both build times (the grid's device rebuild, the kd-tree's host build)."""
+    torch = _torch()
+    ctx = Context(0)
+    seed = W.SEED_STADIUM
+    v, nrm, centers = gen_stadium(seed, 256, 4096, (1 << 20) - 2)
+    sc = Scene(ctx)
+    sc.upload(v, nrm)
+    sc.transform_clusters(gen_cluster_transforms(seed, 0, centers, 0.05), 4096, (1 << 20) - 2)
+    sc.build_grid(lam)
+    info = sc.grid_info()
+    sc.reserve_grid(int(info["m"] * 1.25) + 4096, int(info["r"] * 1.5) + 4096)
+    kd = KdTree(sc)
+    kinfo = kd.info()
+    n = width * height
+    cam = camera_lookat((0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0.0, 1.0, 0.0), 60.0, width, height)
+    prim, hits = RayBatch.empty(n), HitBatch.empty(n)
+    ctx.gen_primary(cam, prim)
+    sc.trace_closest(prim, hits, image_width=width)
+    shadow, refl, rhits, shadow2 = RayBatch.empty(n), RayBatch.empty(n), HitBatch.empty(n), \
+        RayBatch.empty(n)
+    sc.gen_shadow(prim, hits, W.LIGHT, 1e-4, shadow)
+    sc.gen_reflection(prim, hits, 1e-4, refl)
+    sc.trace_closest(refl, rhits, image_width=width)
+    sc.gen_shadow(refl, rhits, W.LIGHT, 1e-4, shadow2)
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    out_h = HitBatch.empty(n)
+    cnt = torch.empty(2 * n, dtype=torch.int32, device="cuda:0")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn):
+        fn()
+        ms = []
+        for _ in range(reps):
+            ev[0].record()
+            fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms.append(ev[0].elapsed_time(ev[1]))
+        return float(np.median(ms))
+
+    stages = {"primary": (prim, True), "shadow": (shadow, False), "bounce": (refl, True),
+              "bounce_shadow": (shadow2, False)}
+    out = {"triangles": int(v.shape[0]), "lambda": lam,
+           "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"]},
+           "kdtree": {k: kinfo[k] for k in ("nodes", "leaves", "empty_leaves", "elements",
+                                            "depth")},
+           "kdtree_host_build_ms": round(kinfo["build_ms"], 1), "stages": {}}
+    ev_b = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev_b[0].record()
+    sc.rebuild_grid_async(lam)
+    ev_b[1].record()
+    torch.cuda.synchronize()
+    out["grid_device_rebuild_ms"] = round(ev_b[0].elapsed_time(ev_b[1]), 3)
+    tot = {"grid": 0.0, "kd": 0.0}
+    n_rays = 0
+    for name, (rays, closest) in stages.items():
+        active = rays.f["tmin"] <= rays.f["tmax"]
+        n_act = int(active.sum().item())
+        n_rays += n_act
+        grid_ms = {}
+        for mode, param in (("simple", 16), ("pooled", 16), ("split", 2), ("split", 16)):
+            ctx.set_trace_mode(mode, param)
+            if closest:
+                grid_ms[f"{mode}{param if mode == 'split' else ''}"] = timed(
+                    lambda: sc.trace_closest(rays, out_h, image_width=width))
+            else:
+                grid_ms[f"{mode}{param if mode == 'split' else ''}"] = timed(
+                    lambda: sc.trace_any(rays, occ, image_width=width))
+        ctx.set_trace_mode("simple", 16)
+        if closest:
+            kd_ms = timed(lambda: kd.trace_closest(rays, out_h, image_width=width))
+            kd.trace_closest(rays, out_h, image_width=width, counters=cnt)
+        else:
+            kd_ms = timed(lambda: kd.trace_any(rays, occ, image_width=width))
+            kd.trace_any(rays, occ, image_width=width, counters=cnt)
+        kc = cnt.view(-1, 2)[active].double().mean(0)
+        if closest:
+            sc.trace_closest(rays, out_h, image_width=width, counters=cnt)
+        else:
+            sc.trace_any(rays, occ, image_width=width, counters=cnt)
+        gc = cnt.view(-1, 2)[active].double().mean(0)
+        best = min(grid_ms, key=grid_ms.get)
+        tot["grid"] += grid_ms[best]
+        tot["kd"] += kd_ms
+        out["stages"][name] = {
+            "rays": n_act, "grid_ms": round(grid_ms[best], 3), "grid_schedule": best,
+            "grid_ms_by_schedule": {k: round(x, 3) for k, x in grid_ms.items()},
+            "kd_ms": round(kd_ms, 3),
+            "grid_mrays_s": round(n_act / (grid_ms[best] * 1e-3) / 1e6, 1),
+            "kd_mrays_s": round(n_act / (kd_ms * 1e-3) / 1e6, 1),
+            "grid_cells_tests_per_ray": [round(float(gc[0]), 2), round(float(gc[1]), 2)],
+            "kd_leaves_tests_per_ray": [round(float(kc[0]), 2), round(float(kc[1]), 2)]}
+    out["traversal_ms"] = {k: round(x, 3) for k, x in tot.items()}
+    out["traversal_mrays_s"] = {k: round(n_rays / (x * 1e-3) / 1e6, 1) for k, x in tot.items()}
+    kd.close()
+    return out

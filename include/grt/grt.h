@@ -175,6 +175,25 @@ int grt_mesh_load_obj(grt_mesh mesh, const char* path, uint32_t flags);
  * through the current transform. */
 int grt_mesh_add_sphere(grt_mesh mesh, uint32_t slices, uint32_t stacks);
 int grt_mesh_add_cube(grt_mesh mesh);
+/* rtdb's scene layouts, through the current transform (the reference's
+ * MatrixStack post-multiplies: a layout's translation applies before the
+ * current transform).  grt_mesh_add_triangle: rtdb::loadTriangle
+ * (rtdb.cpp:9-18), (0,0,0) (1,0,0) (0,1,0), normal +z.  grt_mesh_add_logo:
+ * rtdb::loadLogo (rtdb.cpp:20-57), three triangles in z = 0 in the current
+ * colour.  grt_mesh_add_mesh: model's triangles as GRT_TRIANGLES primitives
+ * (normals through the inverse transpose, the degenerate filter again, each
+ * keeping its colour).  grt_mesh_add_even_spaced: rtdb::loadEvenSpaced
+ * (rtdb.cpp:215-233), the triangle at x, y, z in {0, 1.5, 3}.
+ * grt_mesh_add_mixed: rtdb::loadMixed (rtdb.cpp:235-266) with `model` in
+ * place of the teapot (whose data table is not shipped: load it, e.g. from an
+ * OBJ).  grt_mesh_add_teapots125: rtdb::loadTeapots125 (rtdb.cpp:268-287),
+ * 125 copies of `model` scaled by 5 at offsets 2 x (x, y, z), x, y, z < 5. */
+int grt_mesh_add_triangle(grt_mesh mesh);
+int grt_mesh_add_logo(grt_mesh mesh);
+int grt_mesh_add_mesh(grt_mesh mesh, grt_mesh model);
+int grt_mesh_add_even_spaced(grt_mesh mesh);
+int grt_mesh_add_mixed(grt_mesh mesh, grt_mesh model);
+int grt_mesh_add_teapots125(grt_mesh mesh, grt_mesh model);
 /* The degenerate filter on one triangle (9 floats): 1 kept, 0 dropped. */
 int grt_triangle_valid(const float v9[9]);
 /* Kept triangles, and triangles assembled but dropped as degenerate. */

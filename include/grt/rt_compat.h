@@ -279,13 +279,38 @@ public:
 
 protected:
     ~HipGridAccStruct() override;
+    // The batch trace every call above goes through: closest hit into hits,
+    // or any hit into occluded when hits is NULL.
+    virtual void traceBatch(const grt_rays& rays, const grt_hits* hits, uint8_t* occluded,
+                            uint32_t n, uint32_t image_width);
+    grt_context ctx_;
+    grt_scene scene_ = nullptr;
 
 private:
     friend class HipGridAccStructBuilder;
+    friend class HipKdTreeAccStructBuilder;
     void traceOne(Ray& ray, bool any, Hit& hit, bool& found);
-    grt_context ctx_;
-    grt_scene scene_ = nullptr;
     void* one_ = nullptr;   // device scratch for the per-ray calls
+};
+
+// The kd-tree of one geometry (rtp::KdTreeAccStruct, KdTreeAccStruct.cpp:
+// 53-240): the same calls as HipGridAccStruct, traced through a surface-area-
+// heuristic kd-tree (grt_kdtree_*) instead of the grid.
+class HipKdTreeAccStruct : public HipGridAccStruct {
+public:
+    explicit HipKdTreeAccStruct(grt_context ctx) : HipGridAccStruct(ctx) {}
+    void clear() override;
+    grt_kdtree tree() const { return tree_; }
+    grt_kdtree_info treeInfo() const;
+
+protected:
+    ~HipKdTreeAccStruct() override;
+    void traceBatch(const grt_rays& rays, const grt_hits* hits, uint8_t* occluded, uint32_t n,
+                    uint32_t image_width) override;
+
+private:
+    friend class HipKdTreeAccStructBuilder;
+    grt_kdtree tree_ = nullptr;
 };
 
 // Device grid build (UniformGridAccStructBuilder::buildGeometry,
@@ -302,6 +327,18 @@ public:
 private:
     grt_context ctx_;
     float lambda_ = 5.0f;
+};
+
+// kd-tree build (KdTreeAccStructBuilder::buildGeometry,
+// KdTreeAccStructBuilder.cpp:21-28, with TriangleTreeBuilder's SAH build):
+// the geometry's accStruct becomes a HipKdTreeAccStruct.
+class HipKdTreeAccStructBuilder : public IAccStructBuilder {
+public:
+    explicit HipKdTreeAccStructBuilder(grt_context ctx);
+    void buildGeometry(Geometry* geometry) override;
+
+private:
+    grt_context ctx_;
 };
 
 // rtc::CudaRenderer (CudaRenderer.cpp:9-209): newFrame reads the camera,

@@ -169,7 +169,10 @@ def test_cabi_cell_layout_and_validation(cabi):
 RENDER = os.path.join(os.path.dirname(grt.__file__), "..", "_build", "render_compat")
 
 
-def test_cpp_plugin_classes_render(gpu_ctx, c1_scene, tmp_path):
+@pytest.mark.parametrize("accel", ["grid", "kd"])
+def test_cpp_plugin_classes_render(gpu_ctx, c1_scene, tmp_path, accel):
+    """The CudaRenderer frame through the plugin classes, with the grid builder
+    plugin or the kd-tree builder plugin (whose closest hits are the grid's)."""
     v, nrm = c1_scene
     w, h, lam = 256, 200, 5.0
     scene = tmp_path / "scene.bin"
@@ -179,6 +182,8 @@ def test_cpp_plugin_classes_render(gpu_ctx, c1_scene, tmp_path):
         np.ascontiguousarray(nrm, np.float32).tofile(f)
     out = tmp_path / "frame.bin"
     cmd = [RENDER, str(scene), str(out), str(w), str(h), str(lam), *map(str, EYE), *map(str, AT)]
+    if accel == "kd":
+        cmd.append("kd")
     print("running", " ".join(cmd), flush=True)
     p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stdout + p.stderr
@@ -192,9 +197,14 @@ def test_cpp_plugin_classes_render(gpu_ctx, c1_scene, tmp_path):
     gn = words[5 * n:5 * n + 3].view(np.int32)
     gb = words[5 * n + 3:].view(np.float32)
     fr = orc.grid_frame(v, lam)
-    assert list(gn) == list(fr["n"])
-    assert_same(gb[:3], np.asarray(fr["bmin"], np.float32), "grid bmin")
-    assert_same(gb[3:], np.asarray(fr["bmax"], np.float32), "grid bmax")
+    if accel == "kd":
+        assert list(gn) == [-1, -1, -1]
+        assert_same(gb[:3], v.reshape(-1, 3).min(axis=0), "kd-tree bmin")
+        assert_same(gb[3:], v.reshape(-1, 3).max(axis=0), "kd-tree bmax")
+    else:
+        assert list(gn) == list(fr["n"])
+        assert_same(gb[:3], np.asarray(fr["bmin"], np.float32), "grid bmin")
+        assert_same(gb[3:], np.asarray(fr["bmax"], np.float32), "grid bmax")
     _, offs, refs = orc.grid_build(v, fr)
     cam = grt.camera_lookat(EYE, AT, (0, 1, 0), 60.0, w, h)
     o, img = _oracle_frame(v, nrm, fr, offs, refs, cam)

@@ -616,6 +616,51 @@ def test_mesh_scene_parity(gpu_ctx, tmp_path):
     check_any(sc, fr, offs, refs, v, sh)
 
 
+@pytest.mark.parametrize("layout", ["even_spaced", "mixed", "teapots125"])
+def test_rtdb_layout_scene_parity(gpu_ctx, tmp_path, layout):
+    """rtdb's scene layouts (rtdb.cpp:215-287) assembled on the host -- with a
+    model read from an OBJ in place of the teapot -- through
+    grt_scene_upload_mesh, the device grid build and the traversal, against
+    the oracle on the mesh's own arrays."""
+    src = grt.Mesh()
+    src.set_transform([0.5, 0, 0, 0.25, 0, 0.25, 0, 0.5, 0, 0, 0.75, 0.125])
+    src.add_cube()
+    src.set_transform([0.3, 0, 0, 0.5, 0, 0.3, 0, 0.2, 0, 0, 0.3, 0.5])
+    src.add_sphere(12, 10)
+    v0 = src.arrays()[0].reshape(-1, 3, 3)
+    lines = ["v %.9g %.9g %.9g" % tuple(p) for tri in v0 for p in tri]
+    lines += ["f %d %d %d" % (3 * k + 1, 3 * k + 2, 3 * k + 3) for k in range(len(v0))]
+    (tmp_path / "model.obj").write_text("\n".join(lines) + "\n")
+    model = grt.Mesh()
+    model.load_obj(tmp_path / "model.obj", drop_degenerate=True)
+    m = grt.Mesh()
+    if layout == "even_spaced":
+        m.add_even_spaced()
+        eye, at = (1.5, 1.5, -6.0), (1.5, 1.5, 1.5)
+    elif layout == "mixed":
+        m.add_mixed(model)
+        eye, at = (14.0, 12.0, -25.0), (12.0, 3.0, 12.0)
+    else:
+        m.add_teapots125(model)
+        eye, at = (-30.0, 25.0, -30.0), (25.0, 25.0, 25.0)
+    v, nrm, _, _ = m.arrays()
+    sc = grt.Scene(gpu_ctx)
+    sc.upload_mesh(m)
+    sc.build_grid(5.0)
+    fr = orc.grid_frame(v, 5.0)
+    _, offs, refs = orc.grid_build(v, fr)
+    assert_frame(sc.grid_info(), fr)
+    _, go, gr = sc.download_grid()
+    assert_same(go, offs, f"{layout} grid offsets")
+    assert_same(gr, refs, f"{layout} grid refs")
+    cam = grt.camera_lookat(eye, at, (0, 1, 0), 50.0, 200, 150)
+    rays = orc.primary_rays(grt.camera_dict(cam))
+    g, o = check_closest(sc, fr, offs, refs, v, rays, image_width=cam.width)
+    assert np.count_nonzero(o["id"] >= 0) > 500
+    sh = orc.gen_shadow(rays, o, nrm, (1000.0, 1000.0, 1000.0), 1e-3)
+    check_any(sc, fr, offs, refs, v, sh)
+
+
 # ---------------------------------------------------------------------------
 # Instances: rays through each instance's inverse matrix, nearest hit kept
 # ---------------------------------------------------------------------------

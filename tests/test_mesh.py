@@ -267,3 +267,116 @@ def test_plugin_primitive_builder(tmp_path):
     vx, nx, ny, nz = (float.fromhex(x) for x in lines[4].split())
     assert vx == 3.0                                           # 2 * 1 + 1
     assert np.allclose([nx, ny, nz], np.array([0.5, 1, 0]) / math.sqrt(1.25), atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# rtdb scene layouts (rtdb.cpp:9-57, 215-287)
+# ---------------------------------------------------------------------------
+UNIT_TRI = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+
+
+def test_rtdb_triangle_and_logo():
+    m = grt.Mesh()
+    m.add_triangle()
+    m.add_logo()
+    t, n, _, _ = _tris(m)
+    assert m.info() == (4, 0)
+    np.testing.assert_array_equal(t[0], UNIT_TRI)
+    logo = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[2, 0, 0], [3, 0, 0], [3, 1, 0]],
+                     [[1.5, 0, 0], [2.5, 1, 0], [0.5, 1, 0]]], np.float32)
+    np.testing.assert_array_equal(t[1:], logo)
+    assert np.all(n.reshape(-1, 3) == [0, 0, 1])
+
+
+def test_rtdb_even_spaced_post_multiplies_the_transform():
+    """27 unit triangles at (1.5 x, 1.5 y, 1.5 z), x fastest; under a current
+    transform M the layout's translation applies first: M(T(v))."""
+    m = grt.Mesh()
+    base = np.array([2, 0, 0, 10, 0, 2, 0, 0, 0, 0, 2, 0], np.float32)
+    m.set_transform(base)
+    m.add_even_spaced()
+    t, n, _, _ = _tris(m)
+    assert m.info() == (27, 0)
+    k = 0
+    for z in range(3):
+        for y in range(3):
+            for x in range(3):
+                off = np.float32(1.5) * np.array([x, y, z], np.float32)
+                want = 2 * (UNIT_TRI + off) + np.array([10, 0, 0], np.float32)
+                np.testing.assert_array_equal(t[k], want)
+                k += 1
+    assert np.all(n.reshape(-1, 3) == [0, 0, 1])
+    # the current transform is left as it was
+    m.add_triangle()
+    np.testing.assert_array_equal(_tris(m)[0][-1], 2 * UNIT_TRI + np.float32([10, 0, 0]))
+
+
+def _model():
+    """A stand-in for the teapot with no near-degenerate triangles (a sphere's
+    pole triangles collapse once offset, and the filter then drops them): a
+    cube and a fan."""
+    s = grt.Mesh()
+    s.set_color(0.2, 0.4, 0.6)
+    s.set_transform([0.5, 0, 0, 0.25, 0, 0.25, 0, 0.5, 0, 0, 0.75, 0.125])
+    s.add_cube()
+    s.set_transform(None)
+    s.primitive(grt.TRIANGLE_FAN, [[0, 0, 0], [1, 0, 0.25], [1, 1, 0.5], [0, 1, 0.25]])
+    return s
+
+
+def test_rtdb_mixed_scene():
+    model = _model()
+    mv, mn, _, _ = model.arrays()
+    nm = mv.shape[0]
+    m = grt.Mesh()
+    m.add_mixed(model)
+    t, n, kd, _ = _tris(m)
+    assert m.info()[0] == 3 + 4 * 12 + 4 * nm
+    # the logo at (1, 2, 2.5)
+    np.testing.assert_array_equal(t[0], UNIT_TRI + np.float32([1, 2, 2.5]))
+    # cubes centred at (0,5,0), (5,5,0), (5,5,5), (0,5,5): the translations accumulate
+    for c, centre in enumerate([(0, 5, 0), (5, 5, 0), (5, 5, 5), (0, 5, 5)]):
+        cube = t[3 + 12 * c: 3 + 12 * (c + 1)].reshape(-1, 3)
+        np.testing.assert_array_equal(cube.min(axis=0), np.float32(centre) - 0.5)
+        np.testing.assert_array_equal(cube.max(axis=0), np.float32(centre) + 0.5)
+    # the model scaled by 5 after offsets (0,0,0), (5,0,0), (5,0,5), (0,0,5)
+    first = 3 + 48
+    for c, off in enumerate([(0, 0, 0), (5, 0, 0), (5, 0, 5), (0, 0, 5)]):
+        copy = t[first + nm * c: first + nm * (c + 1)]
+        want = np.float32(5) * (mv.reshape(-1, 3, 3) + np.float32(off))
+        np.testing.assert_allclose(copy, want, rtol=0, atol=2e-6)
+        np.testing.assert_allclose(n[first + nm * c: first + nm * (c + 1)],
+                                   mn.reshape(-1, 3, 3), atol=1e-6)
+        assert np.all(kd[first + nm * c: first + nm * (c + 1)] == np.float32([0.2, 0.4, 0.6]))
+
+
+def test_rtdb_teapots125_from_an_obj_model(tmp_path):
+    """125 copies of a model read from an OBJ (the teapot's place), each
+    5 x (v + 2 (x, y, z)), x fastest."""
+    src = _model()
+    v, _, _, _ = src.arrays()
+    lines = []
+    for tri in v.reshape(-1, 3, 3):
+        for p in tri:
+            lines.append("v %.9g %.9g %.9g" % tuple(p))
+    for k in range(v.shape[0]):
+        lines.append("f %d %d %d" % (3 * k + 1, 3 * k + 2, 3 * k + 3))
+    (tmp_path / "model.obj").write_text("\n".join(lines) + "\n")
+    model = grt.Mesh()
+    model.load_obj(tmp_path / "model.obj", drop_degenerate=True)
+    mv = model.arrays()[0].reshape(-1, 3, 3)
+    nm = mv.shape[0]
+    assert nm == v.shape[0]
+    m = grt.Mesh()
+    m.add_teapots125(model)
+    t = _tris(m)[0]
+    assert m.info()[0] == 125 * nm
+    k = 0
+    for z in range(5):
+        for y in range(5):
+            for x in range(5):
+                want = np.float32(5) * (mv + np.float32([2 * x, 2 * y, 2 * z]))
+                np.testing.assert_allclose(t[nm * k: nm * (k + 1)], want, rtol=0, atol=1e-5)
+                k += 1
+    with pytest.raises(grt.GrtError):
+        m.add_mesh(m)
