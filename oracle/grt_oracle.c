@@ -952,6 +952,13 @@ static inline int mt_f64(const float* v9, const double o[3], const double d[3],
     return (*u >= 0.0) && (*v >= 0.0) && (*u + *v <= 1.0);
 }
 
+/* Both brute-force passes take the rays in blocks of ORC_BRUTE_BLOCK: each
+ * triangle is read once per block and tested against every ray of it, so a
+ * scene far larger than the caches streams once per block instead of once per
+ * ray.  Each ray still sees the triangles in ascending order, so the result is
+ * that of the plain double loop. */
+#define ORC_BRUTE_BLOCK 32
+
 void orc_brute_closest_f64(const float* verts, uint32_t n_tris, uint32_t n_rays,
                            const float* ox, const float* oy, const float* oz,
                            const float* dx, const float* dy, const float* dz,
@@ -959,34 +966,53 @@ void orc_brute_closest_f64(const float* verts, uint32_t n_tris, uint32_t n_rays,
                            int32_t* id, double* t1, double* t2, double* u, double* v,
                            int n_threads)
 {
+    const int64_t n_blocks = ((int64_t)n_rays + ORC_BRUTE_BLOCK - 1) / ORC_BRUTE_BLOCK;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
 #endif
-    for (int64_t i = 0; i < (int64_t)n_rays; ++i) {
-        const double o[3] = {ox[i], oy[i], oz[i]};
-        const double d[3] = {dx[i], dy[i], dz[i]};
-        const double lo = tmin[i], hi = tmax[i];
-        double b1 = INFINITY, b2 = INFINITY, bu = 0.0, bv = 0.0;
-        int32_t bid = -1;
+    for (int64_t blk = 0; blk < n_blocks; ++blk) {
+        const int64_t i0 = blk * ORC_BRUTE_BLOCK;
+        const int nb = (int)((int64_t)n_rays - i0 < ORC_BRUTE_BLOCK ? (int64_t)n_rays - i0
+                                                                     : ORC_BRUTE_BLOCK);
+        double o[ORC_BRUTE_BLOCK][3], d[ORC_BRUTE_BLOCK][3], lo[ORC_BRUTE_BLOCK],
+            hi[ORC_BRUTE_BLOCK];
+        double b1[ORC_BRUTE_BLOCK], b2[ORC_BRUTE_BLOCK], bu[ORC_BRUTE_BLOCK], bv[ORC_BRUTE_BLOCK];
+        int32_t bid[ORC_BRUTE_BLOCK];
+        for (int j = 0; j < nb; ++j) {
+            const int64_t i = i0 + j;
+            o[j][0] = ox[i], o[j][1] = oy[i], o[j][2] = oz[i];
+            d[j][0] = dx[i], d[j][1] = dy[i], d[j][2] = dz[i];
+            lo[j] = tmin[i];
+            hi[j] = tmax[i];
+            b1[j] = b2[j] = INFINITY;
+            bu[j] = bv[j] = 0.0;
+            bid[j] = -1;
+        }
         for (uint32_t k = 0; k < n_tris; ++k) {
-            double t, uu, vv;
-            if (!mt_f64(verts + 9ull * k, o, d, &t, &uu, &vv) || !(t > lo) || !(t <= hi))
-                continue;
-            if (t < b1) {
-                b2 = b1;
-                b1 = t;
-                bid = (int32_t)k;
-                bu = uu;
-                bv = vv;
-            } else if (t < b2) {
-                b2 = t;
+            const float* tri = verts + 9ull * k;
+            for (int j = 0; j < nb; ++j) {
+                double t, uu, vv;
+                if (!mt_f64(tri, o[j], d[j], &t, &uu, &vv) || !(t > lo[j]) || !(t <= hi[j]))
+                    continue;
+                if (t < b1[j]) {
+                    b2[j] = b1[j];
+                    b1[j] = t;
+                    bid[j] = (int32_t)k;
+                    bu[j] = uu;
+                    bv[j] = vv;
+                } else if (t < b2[j]) {
+                    b2[j] = t;
+                }
             }
         }
-        id[i] = bid;
-        t1[i] = b1;
-        t2[i] = b2;
-        u[i] = bu;
-        v[i] = bv;
+        for (int j = 0; j < nb; ++j) {
+            const int64_t i = i0 + j;
+            id[i] = bid[j];
+            t1[i] = b1[j];
+            t2[i] = b2[j];
+            u[i] = bu[j];
+            v[i] = bv[j];
+        }
     }
 }
 
@@ -996,23 +1022,39 @@ void orc_brute_any_f64(const float* verts, uint32_t n_tris, uint32_t n_rays,
                        const float* tmin, const float* tmax,
                        uint8_t* occluded, double* t_near, int n_threads)
 {
+    const int64_t n_blocks = ((int64_t)n_rays + ORC_BRUTE_BLOCK - 1) / ORC_BRUTE_BLOCK;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
 #endif
-    for (int64_t i = 0; i < (int64_t)n_rays; ++i) {
-        const double o[3] = {ox[i], oy[i], oz[i]};
-        const double d[3] = {dx[i], dy[i], dz[i]};
-        const double lo = tmin[i], hi = tmax[i];
-        double best = INFINITY;
-        for (uint32_t k = 0; k < n_tris; ++k) {
-            double t, uu, vv;
-            if (!mt_f64(verts + 9ull * k, o, d, &t, &uu, &vv) || !(t > lo))
-                continue;
-            if (t < best)
-                best = t;
+    for (int64_t blk = 0; blk < n_blocks; ++blk) {
+        const int64_t i0 = blk * ORC_BRUTE_BLOCK;
+        const int nb = (int)((int64_t)n_rays - i0 < ORC_BRUTE_BLOCK ? (int64_t)n_rays - i0
+                                                                     : ORC_BRUTE_BLOCK);
+        double o[ORC_BRUTE_BLOCK][3], d[ORC_BRUTE_BLOCK][3], lo[ORC_BRUTE_BLOCK],
+            best[ORC_BRUTE_BLOCK];
+        for (int j = 0; j < nb; ++j) {
+            const int64_t i = i0 + j;
+            o[j][0] = ox[i], o[j][1] = oy[i], o[j][2] = oz[i];
+            d[j][0] = dx[i], d[j][1] = dy[i], d[j][2] = dz[i];
+            lo[j] = tmin[i];
+            best[j] = INFINITY;
         }
-        t_near[i] = best;
-        occluded[i] = (uint8_t)(lo <= hi && best <= hi);
+        for (uint32_t k = 0; k < n_tris; ++k) {
+            const float* tri = verts + 9ull * k;
+            for (int j = 0; j < nb; ++j) {
+                double t, uu, vv;
+                if (!mt_f64(tri, o[j], d[j], &t, &uu, &vv) || !(t > lo[j]))
+                    continue;
+                if (t < best[j])
+                    best[j] = t;
+            }
+        }
+        for (int j = 0; j < nb; ++j) {
+            const int64_t i = i0 + j;
+            const double hi = tmax[i];
+            t_near[i] = best[j];
+            occluded[i] = (uint8_t)(lo[j] <= hi && best[j] <= hi);
+        }
     }
 }
 

@@ -106,3 +106,83 @@ def test_fp64_error_stadium():
     _check(prim)
     _check(shadow)
     _check(bounce)
+
+
+def _merge(reports):
+    out = {}
+    for r in reports:
+        for k, x in r.items():
+            if k.startswith("t_rel") or k == "uv_abs_max":
+                out[k] = max(out.get(k, 0.0), x)
+            else:
+                out[k] = out.get(k, 0) + x
+    return out
+
+
+def _scene_rows(v, n, lam, cam, n_rays, seed, light=(1000.0, 1000.0, 1000.0)):
+    """Primary, shadow and mirror-bounce reports on n_rays random pixels."""
+    f = orc.grid_frame(v, lam)
+    _, offs, refs = orc.grid_build(v, f)
+    rays_all = orc.primary_rays(grt.camera_dict(cam))
+    idx = np.sort(rng(seed).choice(len(rays_all["ox"]), n_rays, replace=False))
+    rays = _sample(rays_all, idx)
+    hits, prim = closest_report(v, f, offs, refs, rays)
+    sh = orc.gen_shadow(rays, hits, n, light, 1e-4)
+    shadow = any_report(v, f, offs, refs, sh)
+    refl = orc.gen_reflection(rays, hits, n, 1e-4)
+    _, bounce = closest_report(v, f, offs, refs, refl)
+    return prim, shadow, bounce
+
+
+def test_fp64_error_batch64():
+    """The 64 batched scenes (2k..500k triangles, lambda 2, the tuned batch
+    grid): 48 random pixels per scene of its own square image, with their
+    shadow rays and mirror bounces."""
+    from grt import workloads as W
+    rows = [[], [], []]
+    for i, (tris, side, _depth) in enumerate(W.batch_jobs()):
+        v, n = W.sphere_scene(W.SEED_BATCH + 1000 + i, tris)
+        for r, rep in zip(rows, _scene_rows(v, n, 2.0, W.camera(side, side), 48, i)):
+            r.append(rep)
+    prim, shadow, bounce = (_merge(r) for r in rows)
+    print("\nbatch-64 primary", prim, "\nbatch-64 shadow", shadow, "\nbatch-64 bounce", bounce)
+    _check(prim)
+    _check(shadow)
+    _check(bounce)
+
+
+def test_fp64_error_serve16():
+    """The 16 resident serving scenes (10k..4M triangles, lambda 8): 64 random
+    pixels of a 1920 x 1080 request per scene, with shadow rays and bounces."""
+    from grt import workloads as W
+    rows = [[], [], []]
+    for i, nt in enumerate(W.serve_scene_sizes()):
+        v, n = W.sphere_scene(W.SEED_SERVE + 100 + i, nt)
+        for r, rep in zip(rows, _scene_rows(v, n, 8.0, W.camera(1920, 1080), 64, 50 + i)):
+            r.append(rep)
+    prim, shadow, bounce = (_merge(r) for r in rows)
+    print("\nserve-16 primary", prim, "\nserve-16 shadow", shadow, "\nserve-16 bounce", bounce)
+    _check(prim)
+    _check(shadow)
+    _check(bounce)
+
+
+def test_fp64_error_ao16m():
+    """2^24 triangles (lambda 4, the tuned AO grid): 256 random pixels' primary
+    rays of the 1024 x 512 frame, and 512 random AO rays of the frame's
+    8 per pixel (tmax = 0.05 x scene extent)."""
+    from grt import workloads as W
+    v, n = W.ao_scene()
+    f = orc.grid_frame(v, 4.0)
+    _, offs, refs = orc.grid_build(v, f)
+    rays_all = orc.primary_rays(grt.camera_dict(W.ao_camera(1024, 512)))
+    idx = np.sort(rng(21).choice(len(rays_all["ox"]), 256, replace=False))
+    _, prim = closest_report(v, f, offs, refs, _sample(rays_all, idx))
+    hits_all = orc.trace_closest(f, offs, refs, v, rays_all)
+    extent = float(np.max(np.asarray(f["bmax"]) - np.asarray(f["bmin"])))
+    ao_all = orc.gen_ao(rays_all, hits_all, n, 8, W.SEED_AO, 1e-4, 0.05 * extent)
+    aidx = np.sort(rng(22).choice(len(ao_all["ox"]), 512, replace=False))
+    ao = any_report(v, f, offs, refs, _sample(ao_all, aidx))
+    print("\nao-16M primary", prim, "\nao-16M AO", ao)
+    _check(prim)
+    _check(ao)

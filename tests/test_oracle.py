@@ -532,3 +532,106 @@ def test_headlight_known_answers():
     assert orc.headlight((0, 0, -1.0), facing, 0, 0.3, 0.3) == 0xFF000000
     # a miss is white
     assert orc.headlight((0, 0, 1.0), facing, -1, 0.0, 0.0) == 0xFFFFFFFF
+
+
+# ---------------------------------------------------------------------------
+# The reference CPU tracer's Wald test and the specification's Moller-Trumbore
+# ---------------------------------------------------------------------------
+def _tri_accel(v0, v1, v2):
+    """TriAccel::buildFrom (Triangle.cpp:11-55) in binary32."""
+    f = np.float32
+    b = (v2 - v0).astype(f)
+    c = (v1 - v0).astype(f)
+    n = np.array([f(c[1] * b[2]) - f(c[2] * b[1]), f(c[2] * b[0]) - f(c[0] * b[2]),
+                  f(c[0] * b[1]) - f(c[1] * b[0])], f)
+    if abs(n[0]) > abs(n[1]) and abs(n[0]) > abs(n[2]):
+        k = 0
+    elif abs(n[1]) > abs(n[2]):
+        k = 1
+    else:
+        k = 2
+    u, w = (k + 1) % 3, (k + 2) % 3
+    acc = {"k": k, "n_u": f(n[u] / n[k]), "n_v": f(n[w] / n[k])}
+    nl = (n * f(f(1.0) / n[k])).astype(f)
+    acc["n_d"] = f(f(f(v0[0] * nl[0]) + f(v0[1] * nl[1])) + f(v0[2] * nl[2]))
+    inv = f(f(1.0) / f(f(b[u] * c[w]) - f(b[w] * c[u])))
+    acc["b_nu"], acc["b_nv"] = f(-b[w] * inv), f(b[u] * inv)
+    acc["b_d"] = f(f(f(b[w] * v0[u]) - f(b[u] * v0[w])) * inv)
+    acc["c_nu"], acc["c_nv"] = f(c[w] * inv), f(-c[u] * inv)
+    acc["c_d"] = f(f(f(c[u] * v0[w]) - f(c[w] * v0[u])) * inv)
+    return acc
+
+
+def _hit_wald(acc, o, d):
+    """RayTriIntersection::hitWald (RayTriIntersection.cpp:12-48) without the
+    interval test: (f, lambda, mu), or None when outside the triangle."""
+    f = np.float32
+    k = acc["k"]
+    u, w = (k + 1) % 3, (k + 2) % 3
+    nd = f(f(1.0) / f(f(d[k] + f(acc["n_u"] * d[u])) + f(acc["n_v"] * d[w])))
+    dist = f(nd * f(f(f(acc["n_d"] - o[k]) - f(acc["n_u"] * o[u])) - f(acc["n_v"] * o[w])))
+    hu = f(o[u] + f(dist * d[u]))
+    hv = f(o[w] + f(dist * d[w]))
+    lam = f(f(f(hu * acc["b_nu"]) + f(hv * acc["b_nv"])) + acc["b_d"])
+    mue = f(f(f(hu * acc["c_nu"]) + f(hv * acc["c_nv"])) + acc["c_d"])
+    if lam < 0 or mue < 0 or f(f(1.0) - lam) - mue < 0:
+        return None
+    return dist, lam, mue
+
+
+def test_wald_barycentrics_are_the_mt_weights():
+    """The reference CPU grid path records hitWald's lambda, mu as the hit's
+    v1Coord, v2Coord (RayTriIntersection.cpp:44-47, called from
+    UniformGridAccStruct.cpp:373); the specification's Moller-Trumbore returns
+    u, v as the weights of vertices 1 and 2.  On random well-shaped triangles
+    hit at random interior points, at least 17 degrees off grazing (|cos| >=
+    0.3, where neither binary32 test loses more than a few ulps to
+    conditioning), both agree with the float64 solution of o + t d = v0 +
+    u e1 + v e2 to 2e-5 and with each other to 3e-5 (99% within 5e-6); t
+    agrees to 1e-5 relative."""
+    r = rng(77)
+    dl, dm = [], []
+    for _ in range(3000):
+        v = r.uniform(-1, 1, (3, 3)).astype(np.float32)
+        e1, e2 = v[1] - v[0], v[2] - v[0]
+        cr = np.cross(e1.astype(np.float64), e2.astype(np.float64))
+        if np.linalg.norm(cr) < 0.05 or min(np.linalg.norm(e1), np.linalg.norm(e2)) < 0.2:
+            continue
+        a, b = r.uniform(0.05, 0.9, 2)
+        if a + b > 0.95:
+            a, b = 0.95 - b, 0.95 - a
+            if a < 0.05 or b < 0.05:
+                continue
+        p = v[0].astype(np.float64) + a * e1 + b * e2
+        o = (p + r.normal(size=3) * 2.0).astype(np.float32)
+        d = (p - o).astype(np.float32)
+        cosang = abs(np.dot(cr / np.linalg.norm(cr), d / np.linalg.norm(d)))
+        if cosang < 0.3:
+            continue
+        ok, t, u, vv = orc.mt(v.reshape(9), o, d)
+        w = _hit_wald(_tri_accel(v[0], v[1], v[2]), o, d)
+        assert ok and w is not None, (v, o, d)
+        f, lam, mue = w
+        V, O, D = v.astype(np.float64), o.astype(np.float64), d.astype(np.float64)
+        t64, u64, v64 = np.linalg.solve(np.stack([-D, V[1] - V[0], V[2] - V[0]], 1), O - V[0])
+        assert abs(f - t) <= 1e-5 * abs(t), (f, t)
+        assert max(abs(u - u64), abs(vv - v64)) <= 2e-5, (u, vv, u64, v64)
+        assert max(abs(lam - u64), abs(mue - v64)) <= 2e-5, (lam, mue, u64, v64)
+        assert abs(lam - u) <= 3e-5 and abs(mue - vv) <= 3e-5, (lam, u, mue, vv)
+        dl.append(abs(lam - u))
+        dm.append(abs(mue - vv))
+    assert len(dl) > 1000
+    assert np.percentile(dl, 99) <= 5e-6 and np.percentile(dm, 99) <= 5e-6
+
+
+def test_wald_weights_swapped_would_fail():
+    """The pin above is not vacuous: on an asymmetric hit, lambda and mu
+    differ, so a swapped (v1Coord, v2Coord) assignment would be caught."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    o = np.array([0.2, 0.6, -1.0], np.float32)
+    d = np.array([0, 0, 1], np.float32)
+    ok, t, u, vv = orc.mt(v.reshape(9), o, d)
+    assert ok
+    f, lam, mue = _hit_wald(_tri_accel(v[0], v[1], v[2]), o, d)
+    assert (u, vv) == (np.float32(0.2), np.float32(0.6))
+    assert (lam, mue) == (u, vv) and lam != mue and f == t == 1.0
