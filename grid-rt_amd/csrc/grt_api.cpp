@@ -11,6 +11,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "grt/grt.h"
 #include "grt_internal.h"
@@ -205,6 +206,145 @@ static int enqueue_scatter_sort(grt_scene_t* sc)
     return hip_check(launch_gather_refs(fr, sc->refs.as<uint32_t>(), sc->tri.as<float4>(),
                                         sc->trir.as<float4>(), sc->refs_cap, s),
                      "grid records");
+}
+
+// ---- second grid level ------------------------------------------------------
+static G2Build g2_args(grt_scene_t* sc, bool with_outputs)
+{
+    G2Build b;
+    b.fr = sc->frame.as<DevFrame>();
+    b.g2 = sc->g2frame.as<DevFrame>();
+    b.offsets = sc->offsets.as<uint32_t>();
+    b.refs = sc->refs.as<uint32_t>();
+    b.verts = sc->verts.as<float>();
+    b.lambda2 = sc->g2_lambda;
+    b.min_refs = sc->g2_min_refs;
+    b.tile_m = sc->g2tiles.as<uint32_t>();
+    b.tile_d = b.tile_m + (sc->cells_cap / kScanTile + 1);
+    b.cellrec = sc->cellrec.as<uint4>();
+    b.dense = sc->g2dense.as<uint32_t>();
+    b.dense_refs = b.dense + sc->g2_dense_cap;
+    b.sub_offsets = with_outputs ? sc->sub_offsets.as<uint32_t>() : nullptr;
+    b.sub_refs = with_outputs ? sc->sub_refs.as<uint32_t>() : nullptr;
+    return b;
+}
+
+// Cell records and tile sums for the top grid's capacity.
+static int g2_reserve_top(grt_scene_t* sc)
+{
+    int rc;
+    if ((rc = sc->g2frame.reserve(sc->ctx, sizeof(DevFrame), true)) != GRT_OK)
+        return rc;
+    if ((rc = sc->cellrec.reserve(sc->ctx, (size_t)sc->cells_cap * 16, false)) != GRT_OK)
+        return rc;
+    if ((rc = sc->g2tiles.reserve(sc->ctx, 2 * ((size_t)sc->cells_cap / kScanTile + 1) * 4,
+                                  false)) != GRT_OK)
+        return rc;
+    sc->g2_top_cap = sc->cells_cap;
+    return GRT_OK;
+}
+
+// The dense-cell list (and its reference counts) for `dense` cells.
+static int g2_reserve_dense(grt_scene_t* sc, uint32_t dense)
+{
+    const uint64_t dc = (uint64_t)dense + dense / 8 + 1024;
+    if (dc > 0x7ffffff0ull)
+        return set_error(GRT_CAPACITY, "%u dense cells are too many", dense);
+    if (dc <= sc->g2_dense_cap && sc->g2dense.ptr)
+        return GRT_OK;
+    int rc;
+    if ((rc = sc->g2dense.reserve(sc->ctx, 2 * dc * 4, false)) != GRT_OK)
+        return rc;
+    sc->g2_dense_cap = (uint32_t)dc;
+    return GRT_OK;
+}
+
+static int g2_reserve_sub(grt_scene_t* sc, uint32_t cells, uint32_t refs)
+{
+    const uint64_t cc = (uint64_t)cells + cells / 8 + 4096, rc64 = (uint64_t)refs + refs / 8 + 4096;
+    if (cc > 0xfffffff0ull || rc64 > 0xfffffff0ull)
+        return set_error(GRT_CAPACITY, "sub-grids of %u cells / %u references are too large",
+                         cells, refs);
+    int rc;
+    if (cc > sc->g2_cells_cap || !sc->sub_offsets.ptr) {
+        if ((rc = sc->sub_offsets.reserve(sc->ctx, (cc + 1) * 4, false)) != GRT_OK)
+            return rc;
+        sc->g2_cells_cap = (uint32_t)cc;
+    }
+    if (rc64 > sc->g2_refs_cap || !sc->sub_refs.ptr) {
+        if ((rc = sc->sub_refs.reserve(sc->ctx, rc64 * 4, false)) != GRT_OK)
+            return rc;
+        if ((rc = sc->sub_trir.reserve(sc->ctx, rc64 * 48, false)) != GRT_OK)
+            return rc;
+        sc->g2_refs_cap = (uint32_t)rc64;
+    }
+    return GRT_OK;
+}
+
+// Enqueue the second level after a top build on the same stream (no host
+// synchronisation); skipped when its buffers do not cover the top capacity.
+static int g2_enqueue(grt_scene_t* sc)
+{
+    sc->g2_built = false;
+    if (!(sc->g2_lambda > 0.0f))
+        return GRT_OK;
+    if (sc->g2_top_cap < sc->cells_cap || !sc->sub_offsets.ptr || !sc->sub_refs.ptr ||
+        !sc->g2dense.ptr)
+        return GRT_OK;
+    const G2Build b = g2_args(sc, true);
+    const int rc = hip_check(launch_grid2_build(b, sc->cells_cap, sc->g2_cells_cap,
+                                                sc->g2_refs_cap, sc->g2_dense_cap,
+                                                sc->tri.as<float4>(), sc->sub_trir.as<float4>(),
+                                                sc->ctx->stream),
+                             "sub-grid build");
+    sc->g2_built = rc == GRT_OK;
+    return rc;
+}
+
+// Synchronous second level: plan with unbounded capacity, size the buffers
+// with headroom, then build.
+static int g2_build_sync(grt_scene_t* sc)
+{
+    sc->g2_built = false;
+    if (!(sc->g2_lambda > 0.0f))
+        return GRT_OK;
+    int rc;
+    if ((rc = g2_reserve_top(sc)) != GRT_OK)
+        return rc;
+    hipStream_t s = sc->ctx->stream;
+    DevFrame g2;
+    auto fetch = [&]() {
+        hipError_t e = hipMemcpyAsync(&g2, sc->g2frame.ptr, sizeof(g2), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        return hip_check(e, "reading the sub-grid totals");
+    };
+    // The dense-cell count first (its list sizes the per-cell pass), then
+    // the sub-cell and reference totals.
+    const uint32_t dense0 = sc->g2dense.ptr ? sc->g2_dense_cap : 0u;
+    if ((rc = g2_reserve_dense(sc, dense0)) != GRT_OK)
+        return rc;
+    for (int pass = 0; pass < 2; ++pass) {
+        if ((rc = hip_check(launch_grid2_plan(g2_args(sc, false), sc->cells_cap, 0xffffffffu,
+                                              0xffffffffu, sc->g2_dense_cap, s),
+                            "sub-grid plan")) != GRT_OK)
+            return rc;
+        if ((rc = fetch()) != GRT_OK)
+            return rc;
+        if (g2.status == 0)
+            break;
+        if ((rc = g2_reserve_dense(sc, (uint32_t)g2.n[0])) != GRT_OK)
+            return rc;
+    }
+    if (g2.status != 0)
+        return set_error(GRT_CAPACITY, "sub-grid plan status 0x%x", g2.status);
+    if ((rc = g2_reserve_dense(sc, (uint32_t)g2.n[0])) != GRT_OK)
+        return rc;
+    if ((rc = g2_reserve_sub(sc, g2.m, g2.r)) != GRT_OK)
+        return rc;
+    if ((rc = g2_enqueue(sc)) != GRT_OK)
+        return rc;
+    return hip_check(hipStreamSynchronize(s), "sub-grid build");
 }
 
 int scene_common_buffers(grt_scene_t* sc)
@@ -615,7 +755,7 @@ int grt_grid_build(grt_scene sc, float lambda)
     sc->info_host = host;
     sc->lambda = lambda;
     sc->grid_valid = true;
-    return GRT_OK;
+    return g2_build_sync(sc);
 }
 
 int grt_grid_reserve(grt_scene sc, uint32_t cells, uint32_t refs)
@@ -625,7 +765,11 @@ int grt_grid_reserve(grt_scene sc, uint32_t cells, uint32_t refs)
     int rc;
     if ((rc = grid_reserve_cells(sc, cells)) != GRT_OK)
         return rc;
-    return grid_reserve_refs(sc, refs);
+    if ((rc = grid_reserve_refs(sc, refs)) != GRT_OK)
+        return rc;
+    if (sc->g2_lambda > 0.0f && sc->g2_top_cap < sc->cells_cap)
+        return g2_reserve_top(sc);
+    return GRT_OK;
 }
 
 int grt_grid_rebuild_async(grt_scene sc, float lambda)
@@ -653,7 +797,7 @@ int grt_grid_rebuild_async(grt_scene sc, float lambda)
     sc->lambda = lambda;
     sc->grid_valid = true;
     sc->info_stale = true;
-    return GRT_OK;
+    return g2_enqueue(sc);
 }
 
 int grt_grid_check(grt_scene sc)
@@ -670,6 +814,22 @@ int grt_grid_check(grt_scene sc)
     sc->info_stale = false;
     // A rebuild over the reference capacity zeroed the counts on the device
     // (the scan's apply pass), so the next build needs no clean-up here.
+    if (host.status == 0 && sc->g2_built) {
+        DevFrame g2;
+        hipError_t e = hipMemcpyAsync(&g2, sc->g2frame.ptr, sizeof(g2), hipMemcpyDeviceToHost,
+                                      sc->ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(sc->ctx->stream);
+        if (e != hipSuccess)
+            return hip_check(e, "reading the sub-grid status");
+        // Over its capacity the second level is left out (every cell record
+        // says "no sub-grid"): traces stay correct on the top grid alone.
+        if (g2.status != 0)
+            return set_error(GRT_CAPACITY,
+                             "sub-grids exceeded their capacity (status 0x%x: %u cells, %u "
+                             "references); this frame traced the top grid alone",
+                             g2.status, g2.m, g2.r);
+    }
     if (host.status != 0) {
         sc->grid_valid = false;
         return set_error(GRT_CAPACITY, "grid rebuild exceeded capacity (status 0x%x)",
@@ -735,8 +895,83 @@ int grt_grid_upload(grt_scene sc, const grt_grid_info* info, const uint32_t* off
         return hip_check(e, "grid upload");
     sc->info_host = f;
     sc->grid_valid = true;
+    sc->g2_built = false;   // an uploaded grid has no second level
     sc->info_stale = false;
     return GRT_OK;
+}
+
+int grt_grid_set_subgrid(grt_scene sc, float lambda2, uint32_t min_refs)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_set_subgrid: NULL scene");
+    if (!(lambda2 >= 0.0f) || !std::isfinite(lambda2))
+        return set_error(GRT_INVALID_PARAM, "grt_grid_set_subgrid: lambda2 must be >= 0");
+    if (lambda2 > 0.0f && min_refs < 2)
+        return set_error(GRT_INVALID_PARAM, "grt_grid_set_subgrid: min_refs must be >= 2");
+    sc->g2_lambda = lambda2;
+    sc->g2_min_refs = min_refs;
+    sc->g2_built = false;   // takes effect with the next build
+    return GRT_OK;
+}
+
+int grt_subgrid_info_get(grt_scene sc, grt_subgrid_info* out)
+{
+    if (!sc || !out)
+        return set_error(GRT_INVALID_PARAM, "grt_subgrid_info_get: NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    out->lambda2 = sc->g2_lambda;
+    out->min_refs = sc->g2_min_refs;
+    out->built = sc->g2_built ? 1u : 0u;
+    if (!sc->g2_built)
+        return GRT_OK;
+    DevFrame g2;
+    hipError_t e = hipMemcpyAsync(&g2, sc->g2frame.ptr, sizeof(g2), hipMemcpyDeviceToHost,
+                                  sc->ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(sc->ctx->stream);
+    if (e != hipSuccess)
+        return hip_check(e, "reading the sub-grid totals");
+    out->cells = g2.m;
+    out->refs = g2.r;
+    out->status = g2.status;
+    return GRT_OK;
+}
+
+int grt_subgrid_download(grt_scene sc, uint32_t* cellsub_host, uint32_t* sub_offsets_host,
+                         uint32_t* sub_refs_host)
+{
+    if (!sc)
+        return set_error(GRT_INVALID_PARAM, "grt_subgrid_download: NULL scene");
+    grt_subgrid_info si;
+    int rc = grt_subgrid_info_get(sc, &si);
+    if (rc != GRT_OK)
+        return rc;
+    if (!si.built || si.status != 0)
+        return set_error(GRT_NOT_READY, "grt_subgrid_download: no second level built");
+    grt_grid_info gi;
+    if ((rc = grt_grid_info_get(sc, &gi)) != GRT_OK)
+        return rc;
+    hipStream_t s = sc->ctx->stream;
+    hipError_t e = hipSuccess;
+    if (cellsub_host) {
+        std::vector<uint32_t> rec((size_t)gi.m * 4);
+        e = hipMemcpyAsync(rec.data(), sc->cellrec.ptr, rec.size() * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        for (size_t c = 0; e == hipSuccess && c < gi.m; ++c) {
+            cellsub_host[2 * c] = rec[4 * c + 3] ? rec[4 * c + 2] : 0u;
+            cellsub_host[2 * c + 1] = rec[4 * c + 3];
+        }
+    }
+    if (e == hipSuccess && sub_offsets_host)
+        e = hipMemcpyAsync(sub_offsets_host, sc->sub_offsets.ptr, ((size_t)si.cells + 1) * 4,
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && sub_refs_host && si.refs)
+        e = hipMemcpyAsync(sub_refs_host, sc->sub_refs.ptr, (size_t)si.refs * 4,
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    return hip_check(e, "sub-grid download");
 }
 
 int grt_grid_info_get(grt_scene sc, grt_grid_info* out)
@@ -813,6 +1048,12 @@ static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint
     a.counters = counters;
     a.coop = sc->ctx->coop;
     a.pool = sc->ctx->trace_mode == GRT_TRACE_POOLED ? 1u : 0u;
+    if (sc->g2_built) {
+        a.g2 = sc->g2frame.as<DevFrame>();
+        a.cellrec = sc->cellrec.as<uint4>();
+        a.sub_offsets = sc->sub_offsets.as<uint32_t>();
+        a.sub_trir = sc->sub_trir.as<float4>();
+    }
     return a;
 }
 
@@ -822,6 +1063,10 @@ static int dispatch_closest(grt_scene sc, const TraceArgs& a, const grt_rays* ra
 {
     const HitSoA h{hits->id, hits->t, hits->u, hits->v};
     grt_context_t* ctx = sc->ctx;
+    if (a.cellrec)
+        return hip_check(launch_trace2(a, to_soa(rays), h, nullptr, ctx->block_closest,
+                                       ctx->stream),
+                         "trace_closest");
     if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
         return hip_check(launch_trace_persistent(a, to_soa(rays), h, nullptr, false,
                                                  sc->small_counters.as<uint32_t>() + 2,
@@ -840,6 +1085,10 @@ static int dispatch_any(grt_scene sc, const TraceArgs& a, const grt_rays* rays,
                         uint8_t* occluded_dev)
 {
     grt_context_t* ctx = sc->ctx;
+    if (a.cellrec)
+        return hip_check(launch_trace2(a, to_soa(rays), HitSoA{}, occluded_dev, ctx->block_any,
+                                       ctx->stream),
+                         "trace_any");
     if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
         return hip_check(launch_trace_persistent(a, to_soa(rays), HitSoA{}, occluded_dev, true,
                                                  sc->small_counters.as<uint32_t>() + 2,

@@ -83,6 +83,37 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* block_sums,
                            DevFrame* tmp_frame, hipStream_t s);
 
+// ---- second grid level (grt_grid2.hip) ----------------------------------------
+constexpr uint32_t kSubMax = 16;   // sub-cells per axis of a dense top cell, at most
+struct G2Build {
+    const DevFrame* fr;         // top frame (m cells, status)
+    DevFrame* g2;               // level 2: m = sub-cells, r = sub-references, status
+    const uint32_t* offsets;    // top offsets[m + 1]
+    const uint32_t* refs;       // top references
+    const float* verts;         // 9 floats per triangle (current pose)
+    float lambda2;
+    uint32_t min_refs;
+    uint32_t* tile_m;           // per 4096-cell tile: sub-cells, scanned in place
+    uint32_t* tile_d;           // per tile: dense cells, scanned in place
+    uint4* cellrec;             // per top cell {first ref, end ref, first sub-cell, r0|r1<<8|r2<<16}
+    uint32_t* dense;            // dense top cells in cell order [dense_cap]
+    uint32_t* dense_refs;       // per dense cell: sub-references, then their prefix [dense_cap]
+    uint32_t* sub_offsets;      // [sub-cells + 1]
+    uint32_t* sub_refs;
+};
+// Plan, dense list and per-cell counts: g2->m, g2->n[0], g2->r receive the
+// sub-cell, dense-cell and sub-reference totals, g2->status the capacities
+// exceeded (b.sub_offsets may be NULL).  tile arrays: top_cells_cap /
+// kScanTile + 1 entries each.
+hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
+                             uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s);
+// The whole second level after the top build on the same stream: the plan,
+// then sub-cell offsets, references and their 48-byte records.  Over a
+// capacity, g2->status is set and every cell record says "no sub-grid".
+hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
+                              uint32_t sub_refs_cap, uint32_t dense_cap, const float4* tri,
+                              float4* sub_trir, hipStream_t s);
+
 // ---- traversal and ray generation (grt_traverse.hip, grt_raygen.hip) ---------
 struct TraceArgs {
     const DevFrame* fr;
@@ -99,9 +130,19 @@ struct TraceArgs {
                             // test per lane (0)
     const uint32_t* list = nullptr;        // or the rays to trace: slot s traces list[s] ...
     const uint32_t* list_count = nullptr;  // ... for s < *list_count (a device value) and s < n
+    // Two-level grid (all set, or cellrec NULL for the top grid alone).  The
+    // sub-grids are used only when g2->status is 0.
+    const DevFrame* g2 = nullptr;
+    const uint4* cellrec = nullptr;        // per top cell {refs begin, end, first sub-cell, r}
+    const uint32_t* sub_offsets = nullptr;
+    const float4* sub_trir = nullptr;
 };
 hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
                                 uint32_t block, hipStream_t s);
+// Two-level traversal (a.cellrec set), one ray per lane: closest hit into
+// hits, or any hit into occluded when it is set.
+hipError_t launch_trace2(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                         uint8_t* occluded, uint32_t block, hipStream_t s);
 hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occluded,
                             uint32_t block, hipStream_t s);
 // Persistent traversal (closest hit, or any hit when `any`): lanes refill

@@ -1278,6 +1278,186 @@ static uint32_t ray_grid(uint32_t n, uint32_t img_w, uint32_t block)
     return (uint32_t)((slot_count(n, img_w) + block - 1) / block);
 }
 
+// ---------------------------------------------------------------------------
+// Two-level grid: one ray per lane
+// ---------------------------------------------------------------------------
+//
+// The top DDA as in the one-level kernels; a top cell whose record carries a
+// sub-grid resolution is walked by a DDA of its own from the point where the
+// ray entered the cell (the clipped entry, then each top cell's exit), with
+// hits bounded by the sub-cell's exit; other cells test their references
+// bounded by the cell's exit.  The first (sub-)cell holding a hit ends a
+// closest-hit ray, the first triangle in (tmin, tmax] an any-hit ray.
+// Counters: top cells plus sub-cells visited, triangle tests.
+
+// Tests records [b, e) of `rec`: closest (bound) or any.  True when the ray is
+// done (a hit in this cell, or an occluder).
+template <bool ANY>
+__device__ __forceinline__ bool trace2_cell(const float4* __restrict__ rec, uint32_t b,
+                                            uint32_t e, float o0, float o1, float o2, float d0,
+                                            float d1, float d2, float tmin, float tmax,
+                                            float bound, int32_t& id, float& best, float& bu,
+                                            float& bv, uint32_t& tests)
+{
+    if (b >= e)
+        return false;
+    float4 A = rec[3ull * b], B = rec[3ull * b + 1], C = rec[3ull * b + 2];
+    for (uint32_t k = b; k < e; ++k) {
+        const float4 cA = A, cB = B, cC = C;
+        if (k + 1 < e) {
+            const uint64_t q = 3ull * (k + 1);
+            A = rec[q];
+            B = rec[q + 1];
+            C = rec[q + 2];
+        }
+        ++tests;
+        if (ANY) {
+            if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax))
+                return true;
+        } else {
+            closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu, bv);
+        }
+    }
+    return !ANY && id >= 0;
+}
+
+__device__ __forceinline__ int32_t sub_vox(float val, float bmin, float inv, int32_t r)
+{
+    float x = (val - bmin) * inv;
+    x = x > 0.0f ? x : 0.0f;
+    const float hi = (float)(r - 1);
+    x = x < hi ? x : hi;
+    return (int32_t)x;
+}
+
+template <bool ANY>
+__global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA hits,
+                                                uint8_t* __restrict__ occluded)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid;
+    const uint32_t i = trace_ray(a, slot, valid);
+    if (!valid)
+        return;
+    const DevFrame* __restrict__ fr = a.fr;
+    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
+    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
+    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
+    int32_t id = -1;
+    float best = INFINITY, bu = 0.0f, bv = 0.0f;
+    bool occ = false;
+    uint32_t cells = 0, tests = 0;
+    Dda s;
+    // Over its capacity the second level's records are not complete: the top
+    // grid alone then.
+    const bool sub_ok = a.g2->status == 0;
+    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
+        const float i0 = 1.0f / d0, i1 = 1.0f / d1, i2 = 1.0f / d2;
+        // the clipped entry, as dda_setup computes it
+        float t_enter = tmin;
+        t_enter = tmax_f(t_enter, tmin_f((fr->bmin[0] - o0) * i0, (fr->bmax[0] - o0) * i0));
+        t_enter = tmax_f(t_enter, tmin_f((fr->bmin[1] - o1) * i1, (fr->bmax[1] - o1) * i1));
+        t_enter = tmax_f(t_enter, tmin_f((fr->bmin[2] - o2) * i2, (fr->bmax[2] - o2) * i2));
+        const bool n0 = __builtin_signbit(d0), n1 = __builtin_signbit(d1),
+                   n2 = __builtin_signbit(d2);
+        const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+        for (;;) {
+            ++cells;
+            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+            const uint4 rec = a.cellrec[ci];
+            bool done = false;
+            if (sub_ok && rec.w != 0) {
+                const int32_t r0 = (int32_t)(rec.w & 255u), r1 = (int32_t)((rec.w >> 8) & 255u),
+                              r2 = (int32_t)((rec.w >> 16) & 255u);
+                const float sb0 = (float)s.cx * fr->cell[0] + fr->bmin[0];
+                const float sb1 = (float)s.cy * fr->cell[1] + fr->bmin[1];
+                const float sb2 = (float)s.cz * fr->cell[2] + fr->bmin[2];
+                const float sc0 = fr->cell[0] / (float)r0, sc1 = fr->cell[1] / (float)r1,
+                            sc2 = fr->cell[2] / (float)r2;
+                const float si0 = 1.0f / sc0, si1 = 1.0f / sc1, si2 = 1.0f / sc2;
+                int32_t qx = sub_vox(o0 + d0 * t_enter, sb0, si0, r0);
+                int32_t qy = sub_vox(o1 + d1 * t_enter, sb1, si1, r1);
+                int32_t qz = sub_vox(o2 + d2 * t_enter, sb2, si2, r2);
+                const int32_t sx = n0 ? -1 : 1, sy = n1 ? -1 : 1, sz = n2 ? -1 : 1;
+                const int32_t ox = n0 ? -1 : r0, oy = n1 ? -1 : r1, oz = n2 ? -1 : r2;
+                float tx = ((float)(qx + (n0 ? 0 : 1)) * sc0 + sb0 - o0) * i0;
+                float ty = ((float)(qy + (n1 ? 0 : 1)) * sc1 + sb1 - o1) * i1;
+                float tz = ((float)(qz + (n2 ? 0 : 1)) * sc2 + sb2 - o2) * i2;
+                const float dtx = fabsf(sc0 * i0), dty = fabsf(sc1 * i1), dtz = fabsf(sc2 * i2);
+                for (;;) {
+                    ++cells;
+                    const float sexit = tmin_f(tmin_f(tx, ty), tz);
+                    const uint32_t si = rec.z + (uint32_t)qx +
+                                        (uint32_t)r0 * ((uint32_t)qy + (uint32_t)r1 * (uint32_t)qz);
+                    const uint32_t kb = a.sub_offsets[si], ke = a.sub_offsets[si + 1];
+                    if (trace2_cell<ANY>(a.sub_trir, kb, ke, o0, o1, o2, d0, d1, d2, tmin, tmax,
+                                         sexit * kExitScale, id, best, bu, bv, tests)) {
+                        done = true;
+                        occ = ANY;
+                        break;
+                    }
+                    if (sexit > tmax) {
+                        done = true;
+                        break;
+                    }
+                    if (tx < ty && tx < tz) {
+                        qx += sx;
+                        if (qx == ox)
+                            break;
+                        tx += dtx;
+                    } else if (ty < tz) {
+                        qy += sy;
+                        if (qy == oy)
+                            break;
+                        ty += dty;
+                    } else {
+                        qz += sz;
+                        if (qz == oz)
+                            break;
+                        tz += dtz;
+                    }
+                }
+            } else if (trace2_cell<ANY>(a.trir, rec.x, rec.y, o0, o1, o2, d0, d1, d2, tmin, tmax,
+                                        exit_t * kExitScale, id, best, bu, bv, tests)) {
+                done = true;
+                occ = ANY;
+            }
+            if (done || exit_t > tmax)
+                break;
+            t_enter = exit_t;
+            if (!dda_step(s))
+                break;
+        }
+    }
+    if (ANY) {
+        occluded[i] = occ ? 1 : 0;
+    } else {
+        hits.id[i] = id;
+        hits.t[i] = best;
+        hits.u[i] = bu;
+        hits.v[i] = bv;
+    }
+    if (a.counters) {
+        a.counters[2ull * i] = cells;
+        a.counters[2ull * i + 1] = tests;
+    }
+}
+
+hipError_t launch_trace2(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                         uint8_t* occluded, uint32_t block, hipStream_t s)
+{
+    (void)block;
+    if (a.n == 0)
+        return hipSuccess;
+    const uint32_t g = ray_grid(a.n, a.image_width, 128);
+    if (occluded)
+        hipLaunchKernelGGL(k_trace2<true>, dim3(g), dim3(128), 0, s, a, rays, hits, occluded);
+    else
+        hipLaunchKernelGGL(k_trace2<false>, dim3(g), dim3(128), 0, s, a, rays, hits, occluded);
+    return hipGetLastError();
+}
+
 hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
                                 uint32_t block, hipStream_t s)
 {

@@ -56,6 +56,12 @@ class GridInfo(ctypes.Structure):
                 ("n", ctypes.c_int32 * 3), ("m", ctypes.c_uint32), ("r", ctypes.c_uint32)]
 
 
+class SubgridInfo(ctypes.Structure):
+    _fields_ = [("lambda2", ctypes.c_float), ("min_refs", ctypes.c_uint32),
+                ("built", ctypes.c_uint32), ("cells", ctypes.c_uint32),
+                ("refs", ctypes.c_uint32), ("status", ctypes.c_uint32)]
+
+
 class KdTreeInfo(ctypes.Structure):
     _fields_ = [("bmin", ctypes.c_float * 3), ("bmax", ctypes.c_float * 3),
                 ("nodes", ctypes.c_uint32), ("leaves", ctypes.c_uint32),
@@ -91,6 +97,9 @@ _U32 = ctypes.c_uint32
 _F = ctypes.c_float
 _SIGS = {
     "grt_last_error": (ctypes.c_char_p, []),
+    "grt_grid_set_subgrid": (ctypes.c_int, [_P, _F, _U32]),
+    "grt_subgrid_info_get": (ctypes.c_int, [_P, ctypes.POINTER(SubgridInfo)]),
+    "grt_subgrid_download": (ctypes.c_int, [_P, _P, _P, _P]),
     "grt_kdtree_build": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "grt_kdtree_build_host": (ctypes.c_int, [_P, _U32, ctypes.POINTER(_P)]),
     "grt_kdtree_destroy": (ctypes.c_int, [_P]),
@@ -623,6 +632,26 @@ class Scene:
 
     def check_grid(self):
         check(lib().grt_grid_check(self.h))
+
+    def set_subgrid(self, lambda2: float, min_refs: int = 8):
+        """Second grid level from the next build on (lambda2 = 0: off)."""
+        check(lib().grt_grid_set_subgrid(self.h, lambda2, min_refs))
+
+    def subgrid_info(self) -> dict:
+        i = SubgridInfo()
+        check(lib().grt_subgrid_info_get(self.h, ctypes.byref(i)))
+        return {"lambda2": float(i.lambda2), "min_refs": int(i.min_refs), "built": bool(i.built),
+                "cells": int(i.cells), "refs": int(i.refs), "status": int(i.status)}
+
+    def download_subgrid(self):
+        """(cellsub [m, 2], sub_offsets, sub_refs) of the second level."""
+        si = self.subgrid_info()
+        m = self.grid_info()["m"]
+        cellsub = np.empty((m, 2), np.uint32)
+        offs = np.empty(si["cells"] + 1, np.uint32)
+        refs = np.empty(max(si["refs"], 1), np.uint32)
+        check(lib().grt_subgrid_download(self.h, _ptr(cellsub), _ptr(offs), _ptr(refs)))
+        return cellsub, offs, refs[:si["refs"]]
 
     def upload_grid(self, frame: dict, offsets: np.ndarray, refs: np.ndarray):
         gi = GridInfo()

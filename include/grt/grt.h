@@ -226,6 +226,38 @@ int grt_grid_check(grt_scene scene);
 int grt_grid_upload(grt_scene scene, const grt_grid_info* info, const uint32_t* offsets_host,
                     const uint32_t* refs_host);
 int grt_grid_info_get(grt_scene scene, grt_grid_info* out);
+/* Second grid level (a two-level grid; not in the reference, whose uniform
+ * grid walks every empty cell of a "teapot in a stadium").  From the next
+ * grt_grid_build / grt_grid_rebuild_async on, each top cell holding at least
+ * min_refs references gets a uniform sub-grid spanning the cell, r0 x r1 x r2
+ * cells from the top frame's rule applied to the cell (x = lambda2 n / cell
+ * volume, r_a = ceil(cell_a cbrt(x)), at most 16 per axis), rebuilt on the
+ * device with the top grid.  grt_trace_closest / grt_trace_any and the _list
+ * traces then walk both levels, one ray per lane whatever the trace mode: a
+ * subdivided cell is crossed by a DDA of its own, hits bounded by the
+ * sub-cell's exit.  The closest hit is the one-level grid's triangle, t, u
+ * and v except where two hits lie within the exit bound's 2^-20 of each other;
+ * counters count top cells plus sub-cells.  The top grid stays complete, so
+ * batches, serving and instance traces (which use it alone) are unchanged.
+ * lambda2 = 0 turns the level off.  Its buffers are sized by grt_grid_build
+ * (and grt_grid_reserve); an asynchronous rebuild that outgrows them traces
+ * the top grid alone and grt_grid_check reports GRT_CAPACITY. */
+int grt_grid_set_subgrid(grt_scene scene, float lambda2, uint32_t min_refs);
+typedef struct {
+    float lambda2;
+    uint32_t min_refs;
+    uint32_t built;      /* the last build produced the second level */
+    uint32_t cells;      /* sub-cells */
+    uint32_t refs;       /* references in sub-cells */
+    uint32_t status;     /* 0, or the capacity the last build exceeded */
+} grt_subgrid_info;
+int grt_subgrid_info_get(grt_scene scene, grt_subgrid_info* out);
+/* cellsub_host: 2 words per top cell {first sub-cell, r0 | r1 << 8 | r2 << 16
+ * (0: no sub-grid)}; sub-cell (x, y, z) of a cell is first + x + r0 (y + r1 z);
+ * sub_offsets_host: cells + 1 words; sub_refs_host: refs words, ascending
+ * triangle ids per sub-cell (any pointer may be NULL). */
+int grt_subgrid_download(grt_scene scene, uint32_t* cellsub_host, uint32_t* sub_offsets_host,
+                         uint32_t* sub_refs_host);
 int grt_grid_download(grt_scene scene, uint32_t* offsets_host, uint32_t* refs_host);
 
 /* ---- tracing ---------------------------------------------------------- */

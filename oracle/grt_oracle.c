@@ -540,6 +540,347 @@ void orc_trace_any(const orc_frame* f, const uint32_t* offsets, const uint32_t* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Two-level grid                                                            */
+/* ------------------------------------------------------------------------ */
+/* A top cell c holding n_c >= min_refs references gets a uniform sub-grid of
+ * r0 x r1 x r2 cells spanning exactly the cell: the top frame's rule applied
+ * to the cell -- x = (lambda2 n_c) / ((cell0 cell1) cell2), f = cbrt(x),
+ * r_a = ceil(cell_a f) clamped to [1, rmax] -- and none when r = (1,1,1).
+ * Its frame: bmin_a = (float)c_a cell_a + bmin_a (the DDA's boundary formula),
+ * cell'_a = cell_a / r_a, inv'_a = 1 / cell'_a.  A reference goes into every
+ * sub-cell of its triangle's AABB box in that frame (clamp, truncate), with
+ * ascending triangle ids per sub-cell.  cellsub[2c] = first sub-cell of c in
+ * the global sub-cell order (0 without a sub-grid), cellsub[2c+1] = r0 |
+ * r1 << 8 | r2 << 16 (0: no sub-grid); sub-cell (x, y, z) of c is
+ * cellsub[2c] + x + r0 (y + r1 z). */
+typedef struct {
+    float bmin[3], cell[3], inv[3];
+    int32_t r[3];
+} sub_frame;
+
+static inline int sub_frame_of(const orc_frame* f, uint32_t c, uint32_t packed, sub_frame* s)
+{
+    if (packed == 0)
+        return 0;
+    const uint32_t nx = (uint32_t)f->n[0], ny = (uint32_t)f->n[1];
+    const uint32_t cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    const uint32_t cc[3] = {cx, cy, cz};
+    for (int a = 0; a < 3; ++a) {
+        s->r[a] = (int32_t)((packed >> (8 * a)) & 0xffu);
+        s->bmin[a] = (float)cc[a] * f->cell[a] + f->bmin[a];
+        s->cell[a] = f->cell[a] / (float)s->r[a];
+        s->inv[a] = 1.0f / s->cell[a];
+    }
+    return 1;
+}
+
+static inline int32_t sub_vox(const sub_frame* s, float val, int a)
+{
+    float x = (val - s->bmin[a]) * s->inv[a];
+    x = x > 0.0f ? x : 0.0f;
+    const float hi = (float)(s->r[a] - 1);
+    x = x < hi ? x : hi;
+    return (int32_t)x;
+}
+
+uint64_t orc_grid2_plan(const orc_frame* f, const uint32_t* offsets, uint32_t min_refs,
+                        float lambda2, uint32_t rmax, uint32_t* cellsub)
+{
+    const float vol = (f->cell[0] * f->cell[1]) * f->cell[2];
+    uint64_t total = 0;
+    for (uint32_t c = 0; c < f->m; ++c) {
+        const uint32_t n = offsets[c + 1] - offsets[c];
+        uint32_t packed = 0, m = 1;
+        if (n >= min_refs && n > 0) {
+            const float fac = orc_cbrtf((lambda2 * (float)n) / vol);
+            uint32_t r[3];
+            for (int a = 0; a < 3; ++a) {
+                float x = ceilf(f->cell[a] * fac);
+                x = x < 1.0f ? 1.0f : x;
+                x = x > (float)rmax ? (float)rmax : x;
+                r[a] = (uint32_t)x;
+                m *= r[a];
+            }
+            if (m > 1)
+                packed = r[0] | (r[1] << 8) | (r[2] << 16);
+        }
+        cellsub[2 * c] = packed ? (uint32_t)total : 0u;
+        cellsub[2 * c + 1] = packed;
+        if (packed)
+            total += m;
+    }
+    return total;
+}
+
+static inline void tri_sub_range(const sub_frame* s, const float* v9, int32_t lo[3],
+                                 int32_t hi[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        float x = v9[a], y = v9[3 + a], z = v9[6 + a];
+        float mn = x, mx = x;
+        mn = y < mn ? y : mn;
+        mx = y > mx ? y : mx;
+        mn = z < mn ? z : mn;
+        mx = z > mx ? z : mx;
+        lo[a] = sub_vox(s, mn, a);
+        hi[a] = sub_vox(s, mx, a);
+    }
+}
+
+/* Sub-cell counts (n_sub entries) and their total. */
+uint64_t orc_grid2_count(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                         const float* verts, const uint32_t* cellsub, uint32_t* counts,
+                         uint64_t n_sub)
+{
+    memset(counts, 0, sizeof(uint32_t) * (size_t)n_sub);
+    uint64_t total = 0;
+    for (uint32_t c = 0; c < f->m; ++c) {
+        sub_frame s;
+        if (!sub_frame_of(f, c, cellsub[2 * c + 1], &s))
+            continue;
+        const uint32_t base = cellsub[2 * c];
+        for (uint32_t k = offsets[c]; k < offsets[c + 1]; ++k) {
+            int32_t lo[3], hi[3];
+            tri_sub_range(&s, verts + 9ull * refs[k], lo, hi);
+            for (int32_t z = lo[2]; z <= hi[2]; ++z)
+                for (int32_t y = lo[1]; y <= hi[1]; ++y)
+                    for (int32_t x = lo[0]; x <= hi[0]; ++x) {
+                        counts[base + (uint32_t)x + (uint32_t)s.r[0] *
+                                                        ((uint32_t)y + (uint32_t)s.r[1] * (uint32_t)z)]++;
+                        ++total;
+                    }
+        }
+    }
+    return total;
+}
+
+void orc_grid2_fill(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                    const float* verts, const uint32_t* cellsub, const uint32_t* counts,
+                    uint64_t n_sub, uint32_t* sub_offsets, uint32_t* sub_refs)
+{
+    uint32_t acc = 0;
+    for (uint64_t q = 0; q < n_sub; ++q) {
+        sub_offsets[q] = acc;
+        acc += counts[q];
+    }
+    sub_offsets[n_sub] = acc;
+    uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n_sub ? n_sub : 1));
+    memcpy(cur, sub_offsets, sizeof(uint32_t) * (size_t)n_sub);
+    for (uint32_t c = 0; c < f->m; ++c) {
+        sub_frame s;
+        if (!sub_frame_of(f, c, cellsub[2 * c + 1], &s))
+            continue;
+        const uint32_t base = cellsub[2 * c];
+        for (uint32_t k = offsets[c]; k < offsets[c + 1]; ++k) {
+            int32_t lo[3], hi[3];
+            tri_sub_range(&s, verts + 9ull * refs[k], lo, hi);
+            for (int32_t z = lo[2]; z <= hi[2]; ++z)
+                for (int32_t y = lo[1]; y <= hi[1]; ++y)
+                    for (int32_t x = lo[0]; x <= hi[0]; ++x)
+                        sub_refs[cur[base + (uint32_t)x + (uint32_t)s.r[0] *
+                                                             ((uint32_t)y + (uint32_t)s.r[1] *
+                                                                                (uint32_t)z)]++] = refs[k];
+        }
+    }
+    free(cur);
+}
+
+/* The DDA inside one sub-grid from t_enter, with the ray's inverse direction:
+ * the same set-up and single-axis step as the top level. */
+typedef struct {
+    int32_t c[3], step[3], out[3];
+    float tnext[3], tdelta[3];
+} sub_dda;
+
+static inline void sub_dda_setup(const sub_frame* s, const float o[3], const float d[3],
+                                 const float inv[3], float t_enter, sub_dda* q)
+{
+    for (int a = 0; a < 3; ++a) {
+        const float p = o[a] + d[a] * t_enter;
+        q->c[a] = sub_vox(s, p, a);
+        const int neg = signbit(d[a]) != 0;
+        q->step[a] = neg ? -1 : 1;
+        q->out[a] = neg ? -1 : s->r[a];
+        const float w = (float)(q->c[a] + (neg ? 0 : 1)) * s->cell[a] + s->bmin[a];
+        q->tnext[a] = (w - o[a]) * inv[a];
+        q->tdelta[a] = fabsf(s->cell[a] * inv[a]);
+    }
+}
+
+static inline int sub_dda_step(sub_dda* q)
+{
+    int a;
+    if (q->tnext[0] < q->tnext[1] && q->tnext[0] < q->tnext[2])
+        a = 0;
+    else if (q->tnext[1] < q->tnext[2])
+        a = 1;
+    else
+        a = 2;
+    q->c[a] += q->step[a];
+    if (q->c[a] == q->out[a])
+        return 0;
+    q->tnext[a] += q->tdelta[a];
+    return 1;
+}
+
+/* One ray through the two-level grid.  Closest hit: the first cell or
+ * sub-cell holding a hit ends the ray, hits bounded by that (sub-)cell's exit;
+ * any hit: the first triangle in (tmin, tmax].  Counters: top cells plus
+ * sub-cells visited, triangle tests. */
+static void grid2_trace_one(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                            const uint32_t* cellsub, const uint32_t* sub_offsets,
+                            const uint32_t* sub_refs, const float* verts, const float o[3],
+                            const float d[3], float tmin, float tmax, int any, int32_t* hid,
+                            float* ht, float* hu, float* hv, uint8_t* occ, orc_counters* cnt)
+{
+    int32_t id = -1;
+    float best = INFINITY, bu = 0.0f, bv = 0.0f;
+    int found_any = 0, done = 0;
+    uint32_t cells = 0, tests = 0;
+    dda_state st;
+    float inv[3];
+    for (int a = 0; a < 3; ++a)
+        inv[a] = 1.0f / d[a];
+    if (dda_setup(f, o, d, tmin, tmax, &st)) {
+        float t_enter = tmin;
+        for (int a = 0; a < 3; ++a) {   /* the clipped entry, as dda_setup computed it */
+            const float p = (f->bmin[a] - o[a]) * inv[a], q = (f->bmax[a] - o[a]) * inv[a];
+            t_enter = tmax_f(t_enter, tmin_f(p, q));
+        }
+        const uint32_t nx = (uint32_t)f->n[0], ny = (uint32_t)f->n[1];
+        for (;;) {
+            ++cells;
+            const float exit_t = tmin_f(tmin_f(st.tnext[0], st.tnext[1]), st.tnext[2]);
+            const uint32_t ci =
+                (uint32_t)st.c[0] + nx * ((uint32_t)st.c[1] + ny * (uint32_t)st.c[2]);
+            sub_frame sf;
+            if (sub_frame_of(f, ci, cellsub[2 * ci + 1], &sf)) {
+                const uint32_t base = cellsub[2 * ci];
+                sub_dda q;
+                sub_dda_setup(&sf, o, d, inv, t_enter, &q);
+                for (;;) {
+                    ++cells;
+                    const float sexit = tmin_f(tmin_f(q.tnext[0], q.tnext[1]), q.tnext[2]);
+                    const uint32_t si = base + (uint32_t)q.c[0] +
+                                        (uint32_t)sf.r[0] * ((uint32_t)q.c[1] +
+                                                             (uint32_t)sf.r[1] * (uint32_t)q.c[2]);
+                    const uint32_t b = sub_offsets[si], e = sub_offsets[si + 1];
+                    const float bound = sexit * ORC_EXIT_SCALE;
+                    for (uint32_t k = b; k < e; ++k) {
+                        const uint32_t tri = sub_refs[k];
+                        ++tests;
+                        float t, u, v;
+                        if (!mt_core(verts + 9ull * tri, o, d, &t, &u, &v) || !(t > tmin) ||
+                            !(t <= tmax))
+                            continue;
+                        if (any) {
+                            found_any = 1;
+                            break;
+                        }
+                        if (t <= bound && t < best) {
+                            best = t;
+                            id = (int32_t)tri;
+                            bu = u;
+                            bv = v;
+                        }
+                    }
+                    if (found_any || id >= 0 || sexit > tmax) {
+                        done = 1;
+                        break;
+                    }
+                    if (!sub_dda_step(&q))
+                        break;
+                }
+            } else {
+                const uint32_t b = offsets[ci], e = offsets[ci + 1];
+                const float bound = exit_t * ORC_EXIT_SCALE;
+                for (uint32_t k = b; k < e; ++k) {
+                    const uint32_t tri = refs[k];
+                    ++tests;
+                    float t, u, v;
+                    if (!mt_core(verts + 9ull * tri, o, d, &t, &u, &v) || !(t > tmin) ||
+                        !(t <= tmax))
+                        continue;
+                    if (any) {
+                        found_any = 1;
+                        break;
+                    }
+                    if (t <= bound && t < best) {
+                        best = t;
+                        id = (int32_t)tri;
+                        bu = u;
+                        bv = v;
+                    }
+                }
+                if (found_any || id >= 0)
+                    done = 1;
+            }
+            if (done || exit_t > tmax)
+                break;
+            t_enter = exit_t;
+            if (!dda_step(&st))
+                break;
+        }
+    }
+    if (any) {
+        *occ = (uint8_t)found_any;
+    } else {
+        *hid = id;
+        *ht = best;
+        *hu = bu;
+        *hv = bv;
+    }
+    if (cnt) {
+        cnt->cells = cells;
+        cnt->tests = tests;
+    }
+}
+
+void orc_grid2_trace_closest(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                             const uint32_t* cellsub, const uint32_t* sub_offsets,
+                             const uint32_t* sub_refs, const float* verts, uint32_t n_rays,
+                             const float* ox, const float* oy, const float* oz,
+                             const float* dx, const float* dy, const float* dz,
+                             const float* tmin, const float* tmax,
+                             int32_t* hit_id, float* hit_t, float* hit_u, float* hit_v,
+                             orc_counters* cnt, int n_threads, int chunk)
+{
+    if (chunk < 1)
+        chunk = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, chunk) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
+#endif
+    for (int64_t i = 0; i < (int64_t)n_rays; ++i) {
+        const float o[3] = {ox[i], oy[i], oz[i]};
+        const float d[3] = {dx[i], dy[i], dz[i]};
+        grid2_trace_one(f, offsets, refs, cellsub, sub_offsets, sub_refs, verts, o, d, tmin[i],
+                        tmax[i], 0, &hit_id[i], &hit_t[i], &hit_u[i], &hit_v[i], NULL,
+                        cnt ? &cnt[i] : NULL);
+    }
+}
+
+void orc_grid2_trace_any(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                         const uint32_t* cellsub, const uint32_t* sub_offsets,
+                         const uint32_t* sub_refs, const float* verts, uint32_t n_rays,
+                         const float* ox, const float* oy, const float* oz,
+                         const float* dx, const float* dy, const float* dz,
+                         const float* tmin, const float* tmax,
+                         uint8_t* occluded, orc_counters* cnt, int n_threads, int chunk)
+{
+    if (chunk < 1)
+        chunk = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, chunk) num_threads(n_threads > 1 ? n_threads : 1) if (n_threads > 1)
+#endif
+    for (int64_t i = 0; i < (int64_t)n_rays; ++i) {
+        const float o[3] = {ox[i], oy[i], oz[i]};
+        const float d[3] = {dx[i], dy[i], dz[i]};
+        grid2_trace_one(f, offsets, refs, cellsub, sub_offsets, sub_refs, verts, o, d, tmin[i],
+                        tmax[i], 1, NULL, NULL, NULL, NULL, &occluded[i], cnt ? &cnt[i] : NULL);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* kd-tree traversal (the comparison accelerator)                            */
 /* ------------------------------------------------------------------------ */
 #define ORC_KD_MAX_DEPTH 64

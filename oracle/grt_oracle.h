@@ -119,6 +119,34 @@ void orc_trace_any(const orc_frame* f, const uint32_t* offsets, const uint32_t* 
                    const float* tmin, const float* tmax,
                    uint8_t* occluded, orc_counters* cnt, int n_threads, int chunk);
 
+/* ---- two-level grid --------------------------------------------------------- */
+/* Sub-grids for top cells holding at least min_refs references (resolution
+ * from lambda2, at most rmax per axis).  cellsub: 2 words per top cell {first
+ * sub-cell, r0 | r1 << 8 | r2 << 16 (0: none)}; returns the sub-cell total. */
+uint64_t orc_grid2_plan(const orc_frame* f, const uint32_t* offsets, uint32_t min_refs,
+                        float lambda2, uint32_t rmax, uint32_t* cellsub);
+uint64_t orc_grid2_count(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                         const float* verts, const uint32_t* cellsub, uint32_t* counts,
+                         uint64_t n_sub);
+void orc_grid2_fill(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                    const float* verts, const uint32_t* cellsub, const uint32_t* counts,
+                    uint64_t n_sub, uint32_t* sub_offsets, uint32_t* sub_refs);
+void orc_grid2_trace_closest(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                             const uint32_t* cellsub, const uint32_t* sub_offsets,
+                             const uint32_t* sub_refs, const float* verts, uint32_t n_rays,
+                             const float* ox, const float* oy, const float* oz,
+                             const float* dx, const float* dy, const float* dz,
+                             const float* tmin, const float* tmax,
+                             int32_t* hit_id, float* hit_t, float* hit_u, float* hit_v,
+                             orc_counters* cnt, int n_threads, int chunk);
+void orc_grid2_trace_any(const orc_frame* f, const uint32_t* offsets, const uint32_t* refs,
+                         const uint32_t* cellsub, const uint32_t* sub_offsets,
+                         const uint32_t* sub_refs, const float* verts, uint32_t n_rays,
+                         const float* ox, const float* oy, const float* oz,
+                         const float* dx, const float* dy, const float* dz,
+                         const float* tmin, const float* tmax,
+                         uint8_t* occluded, orc_counters* cnt, int n_threads, int chunk);
+
 /* ---- kd-tree traversal (comparison accelerator) ---------------------------- */
 /* box: min xyz, max xyz of the tree; nodes: 2 words per node (inner {split
  * bits, (left << 2) | axis}, leaf {first element, 0x80000000 | count});

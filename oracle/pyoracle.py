@@ -83,6 +83,15 @@ def lib():
                                   + [_P] * 4 + [_P, ctypes.c_int, ctypes.c_int]),
             "orc_trace_any": (None, [ctypes.POINTER(Frame), _P, _P, _P, _U32] + [_P] * 8
                               + [_P, _P, ctypes.c_int, ctypes.c_int]),
+            "orc_grid2_plan": (ctypes.c_uint64, [ctypes.POINTER(Frame), _P, _U32, _F, _U32, _P]),
+            "orc_grid2_count": (ctypes.c_uint64, [ctypes.POINTER(Frame), _P, _P, _P, _P, _P,
+                                                  ctypes.c_uint64]),
+            "orc_grid2_fill": (None, [ctypes.POINTER(Frame), _P, _P, _P, _P, _P,
+                                      ctypes.c_uint64, _P, _P]),
+            "orc_grid2_trace_closest": (None, [ctypes.POINTER(Frame)] + [_P] * 6 + [_U32]
+                                        + [_P] * 8 + [_P] * 4 + [_P, ctypes.c_int, ctypes.c_int]),
+            "orc_grid2_trace_any": (None, [ctypes.POINTER(Frame)] + [_P] * 6 + [_U32]
+                                    + [_P] * 8 + [_P, _P, ctypes.c_int, ctypes.c_int]),
             "orc_kd_trace_closest": (None, [_P, _P, _P, _P, _U32] + [_P] * 8 + [_P] * 4
                                      + [_P, ctypes.c_int, ctypes.c_int]),
             "orc_kd_trace_any": (None, [_P, _P, _P, _P, _U32] + [_P] * 8
@@ -252,6 +261,69 @@ def trace_any(frame: dict, offsets, refs, verts, rays: dict, threads: int = 0,
     threads = threads or lib().orc_max_threads()
     lib().orc_trace_any(ctypes.byref(f), _p(offsets), _p(refs), _p(verts), n,
                         *[_p(r[k]) for k in RAY_FIELDS], _p(occ), _p(cnt), threads, chunk)
+    out = {"occluded": occ}
+    if counters:
+        out["cells"] = cnt[:, 0].copy()
+        out["tests"] = cnt[:, 1].copy()
+    return out
+
+
+def grid2_build(verts, frame: dict, offsets, refs, min_refs: int, lambda2: float, rmax: int):
+    """Sub-grids of a built top grid: (cellsub [m, 2], sub_offsets, sub_refs)."""
+    verts = _f32(verts).reshape(-1, 9)
+    f = frame_struct(frame)
+    offsets = np.ascontiguousarray(offsets, np.uint32)
+    refs = np.ascontiguousarray(refs, np.uint32)
+    cellsub = np.empty((f.m, 2), np.uint32)
+    n_sub = lib().orc_grid2_plan(ctypes.byref(f), _p(offsets), min_refs, lambda2, rmax,
+                                 _p(cellsub))
+    counts = np.empty(max(int(n_sub), 1), np.uint32)
+    r2 = lib().orc_grid2_count(ctypes.byref(f), _p(offsets), _p(refs), _p(verts), _p(cellsub),
+                               _p(counts), n_sub)
+    sub_offsets = np.empty(int(n_sub) + 1, np.uint32)
+    sub_refs = np.empty(max(int(r2), 1), np.uint32)
+    lib().orc_grid2_fill(ctypes.byref(f), _p(offsets), _p(refs), _p(verts), _p(cellsub),
+                         _p(counts), n_sub, _p(sub_offsets), _p(sub_refs))
+    return cellsub, sub_offsets, sub_refs[:int(r2)]
+
+
+def _grid2_call(name, frame, offsets, refs, g2, verts, rays, extra, threads, chunk):
+    f = frame_struct(frame)
+    cellsub, sub_offsets, sub_refs = g2
+    arrs = [np.ascontiguousarray(a, np.uint32) for a in (offsets, refs, cellsub, sub_offsets,
+                                                        sub_refs if len(sub_refs) else
+                                                        np.zeros(1, np.uint32))]
+    verts = _f32(verts).reshape(-1, 9)
+    r = _rays(rays)
+    n = r["ox"].shape[0]
+    threads = threads or lib().orc_max_threads()
+    getattr(lib(), name)(ctypes.byref(f), *[_p(a) for a in arrs], _p(verts), n,
+                         *[_p(r[k]) for k in RAY_FIELDS], *[_p(a) for a in extra], threads, chunk)
+    return n
+
+
+def grid2_trace_closest(frame, offsets, refs, g2, verts, rays: dict, threads: int = 0,
+                        counters: bool = False, chunk: int = 64):
+    n = rays["ox"].shape[0]
+    hid = np.empty(n, np.int32)
+    ht, hu, hv = (np.empty(n, np.float32) for _ in range(3))
+    cnt = np.empty((n, 2), np.uint32) if counters else None
+    _grid2_call("orc_grid2_trace_closest", frame, offsets, refs, g2, verts, rays,
+                [hid, ht, hu, hv, cnt], threads, chunk)
+    out = {"id": hid, "t": ht, "u": hu, "v": hv}
+    if counters:
+        out["cells"] = cnt[:, 0].copy()
+        out["tests"] = cnt[:, 1].copy()
+    return out
+
+
+def grid2_trace_any(frame, offsets, refs, g2, verts, rays: dict, threads: int = 0,
+                    counters: bool = False, chunk: int = 64):
+    n = rays["ox"].shape[0]
+    occ = np.empty(n, np.uint8)
+    cnt = np.empty((n, 2), np.uint32) if counters else None
+    _grid2_call("orc_grid2_trace_any", frame, offsets, refs, g2, verts, rays, [occ, cnt],
+                threads, chunk)
     out = {"occluded": occ}
     if counters:
         out["cells"] = cnt[:, 0].copy()
