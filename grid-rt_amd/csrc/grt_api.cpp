@@ -221,21 +221,29 @@ static G2Build g2_args(grt_scene_t* sc, bool with_outputs)
     b.min_refs = sc->g2_min_refs;
     b.tile_m = sc->g2tiles.as<uint32_t>();
     b.tile_d = b.tile_m + (sc->cells_cap / kScanTile + 1);
-    b.cellrec = sc->cellrec.as<uint4>();
+    b.cellrec = sc->cellrec.as<uint2>();
     b.dense = sc->g2dense.as<uint32_t>();
     b.dense_refs = b.dense + sc->g2_dense_cap;
     b.sub_offsets = with_outputs ? sc->sub_offsets.as<uint32_t>() : nullptr;
     b.sub_refs = with_outputs ? sc->sub_refs.as<uint32_t>() : nullptr;
+    b.box = sc->g2box.as<uint32_t>();
+    b.big = sc->g2big.as<uint32_t>();
+    b.big_cap = sc->g2_big_cap;
     return b;
 }
 
-// Cell records and tile sums for the top grid's capacity.
+// Cell records, tile sums and reference boxes for the top grid's capacities.
 static int g2_reserve_top(grt_scene_t* sc)
 {
     int rc;
+    if (sc->g2_box_cap < sc->refs_cap || !sc->g2box.ptr) {
+        if ((rc = sc->g2box.reserve(sc->ctx, (size_t)sc->refs_cap * 4 + 4, false)) != GRT_OK)
+            return rc;
+        sc->g2_box_cap = sc->refs_cap;
+    }
     if ((rc = sc->g2frame.reserve(sc->ctx, sizeof(DevFrame), true)) != GRT_OK)
         return rc;
-    if ((rc = sc->cellrec.reserve(sc->ctx, (size_t)sc->cells_cap * 16, false)) != GRT_OK)
+    if ((rc = sc->cellrec.reserve(sc->ctx, (size_t)sc->cells_cap * 8, false)) != GRT_OK)
         return rc;
     if ((rc = sc->g2tiles.reserve(sc->ctx, 2 * ((size_t)sc->cells_cap / kScanTile + 1) * 4,
                                   false)) != GRT_OK)
@@ -262,7 +270,7 @@ static int g2_reserve_dense(grt_scene_t* sc, uint32_t dense)
 static int g2_reserve_sub(grt_scene_t* sc, uint32_t cells, uint32_t refs)
 {
     const uint64_t cc = (uint64_t)cells + cells / 8 + 4096, rc64 = (uint64_t)refs + refs / 8 + 4096;
-    if (cc > 0xfffffff0ull || rc64 > 0xfffffff0ull)
+    if (cc > 0x7ffffff0ull || rc64 > 0xfffffff0ull)
         return set_error(GRT_CAPACITY, "sub-grids of %u cells / %u references are too large",
                          cells, refs);
     int rc;
@@ -276,6 +284,11 @@ static int g2_reserve_sub(grt_scene_t* sc, uint32_t cells, uint32_t refs)
             return rc;
         if ((rc = sc->sub_trir.reserve(sc->ctx, rc64 * 48, false)) != GRT_OK)
             return rc;
+        // sub-cells of more than 1024 references: at most refs / 1025 of them
+        const uint64_t bc = rc64 / 1025 + 1;
+        if ((rc = sc->g2big.reserve(sc->ctx, bc * 4, false)) != GRT_OK)
+            return rc;
+        sc->g2_big_cap = (uint32_t)bc;
         sc->g2_refs_cap = (uint32_t)rc64;
     }
     return GRT_OK;
@@ -288,8 +301,8 @@ static int g2_enqueue(grt_scene_t* sc)
     sc->g2_built = false;
     if (!(sc->g2_lambda > 0.0f))
         return GRT_OK;
-    if (sc->g2_top_cap < sc->cells_cap || !sc->sub_offsets.ptr || !sc->sub_refs.ptr ||
-        !sc->g2dense.ptr)
+    if (sc->g2_top_cap < sc->cells_cap || sc->g2_box_cap < sc->refs_cap || !sc->sub_offsets.ptr ||
+        !sc->sub_refs.ptr || !sc->g2dense.ptr || !sc->g2big.ptr)
         return GRT_OK;
     const G2Build b = g2_args(sc, true);
     const int rc = hip_check(launch_grid2_build(b, sc->cells_cap, sc->g2_cells_cap,
@@ -325,7 +338,7 @@ static int g2_build_sync(grt_scene_t* sc)
     if ((rc = g2_reserve_dense(sc, dense0)) != GRT_OK)
         return rc;
     for (int pass = 0; pass < 2; ++pass) {
-        if ((rc = hip_check(launch_grid2_plan(g2_args(sc, false), sc->cells_cap, 0xffffffffu,
+        if ((rc = hip_check(launch_grid2_plan(g2_args(sc, false), sc->cells_cap, 0x7ffffff0u,
                                               0xffffffffu, sc->g2_dense_cap, s),
                             "sub-grid plan")) != GRT_OK)
             return rc;
@@ -767,7 +780,7 @@ int grt_grid_reserve(grt_scene sc, uint32_t cells, uint32_t refs)
         return rc;
     if ((rc = grid_reserve_refs(sc, refs)) != GRT_OK)
         return rc;
-    if (sc->g2_lambda > 0.0f && sc->g2_top_cap < sc->cells_cap)
+    if (sc->g2_lambda > 0.0f && (sc->g2_top_cap < sc->cells_cap || sc->g2_box_cap < sc->refs_cap))
         return g2_reserve_top(sc);
     return GRT_OK;
 }
@@ -954,13 +967,14 @@ int grt_subgrid_download(grt_scene sc, uint32_t* cellsub_host, uint32_t* sub_off
     hipStream_t s = sc->ctx->stream;
     hipError_t e = hipSuccess;
     if (cellsub_host) {
-        std::vector<uint32_t> rec((size_t)gi.m * 4);
+        std::vector<uint32_t> rec((size_t)gi.m * 2);
         e = hipMemcpyAsync(rec.data(), sc->cellrec.ptr, rec.size() * 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess)
             e = hipStreamSynchronize(s);
         for (size_t c = 0; e == hipSuccess && c < gi.m; ++c) {
-            cellsub_host[2 * c] = rec[4 * c + 3] ? rec[4 * c + 2] : 0u;
-            cellsub_host[2 * c + 1] = rec[4 * c + 3];
+            const bool sub = (rec[2 * c] >> 31) != 0;
+            cellsub_host[2 * c] = sub ? rec[2 * c] & 0x7fffffffu : 0u;
+            cellsub_host[2 * c + 1] = sub ? rec[2 * c + 1] : 0u;
         }
     }
     if (e == hipSuccess && sub_offsets_host)
@@ -1050,7 +1064,7 @@ static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint
     a.pool = sc->ctx->trace_mode == GRT_TRACE_POOLED ? 1u : 0u;
     if (sc->g2_built) {
         a.g2 = sc->g2frame.as<DevFrame>();
-        a.cellrec = sc->cellrec.as<uint4>();
+        a.cellrec = sc->cellrec.as<uint2>();
         a.sub_offsets = sc->sub_offsets.as<uint32_t>();
         a.sub_trir = sc->sub_trir.as<float4>();
     }

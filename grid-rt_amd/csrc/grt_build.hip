@@ -23,7 +23,6 @@ namespace grt {
 constexpr uint32_t kSmallCells = 64;
 constexpr uint32_t kHugeCells = 65536;
 constexpr uint32_t kHugeMerge = 4;   // huge triangles applied together in one cell walk
-constexpr uint32_t kSortTile = 16384;   // LDS tile of the per-cell sort (64 KB)
 
 // ---------------------------------------------------------------------------
 // 1. Frame
@@ -890,74 +889,6 @@ __global__ __launch_bounds__(256) void k_compact_write(uint32_t n, uint32_t img_
 // ---------------------------------------------------------------------------
 // 5. Per-cell sort
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
-{
-    const uint32_t lo = a < b ? a : b;
-    const uint32_t hi = a < b ? b : a;
-    a = lo;
-    b = hi;
-}
-
-// Bitonic layer (k = 2^LK, j = 2^LJ) over N registers of one thread, then the
-// next smaller j: element i is compared with i^j, ascending when (i & k) == 0.
-template <int N, int LK, int LJ>
-__device__ __forceinline__ void thread_bitonic_layer(uint32_t (&a)[N])
-{
-    constexpr int k = 1 << LK, j = 1 << LJ;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        if ((i & j) == 0) {
-            if ((i & k) == 0)
-                cswap(a[i], a[i | j]);
-            else
-                cswap(a[i | j], a[i]);
-        }
-    }
-    if constexpr (LJ > 0)
-        thread_bitonic_layer<N, LK, LJ - 1>(a);
-}
-
-template <int N, int LK>
-__device__ __forceinline__ void thread_bitonic(uint32_t (&a)[N])
-{
-    thread_bitonic_layer<N, LK, LK - 1>(a);
-    if constexpr ((1 << LK) < N)
-        thread_bitonic<N, LK + 1>(a);
-}
-
-// A cell of 2..16 references sorted by one thread in registers (slots past
-// the count hold 0xffffffff, which sorts last).
-template <int N>
-__device__ __forceinline__ void sort_cell_thread(uint32_t* __restrict__ seg, uint32_t cnt)
-{
-    uint32_t a[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-        a[k] = (uint32_t)k < cnt ? seg[k] : 0xffffffffu;
-    bool sorted = true;
-#pragma unroll
-    for (int k = 1; k < N; ++k)
-        sorted = sorted && (a[k - 1] <= a[k]);
-    if (sorted)
-        return;
-    thread_bitonic<N, 1>(a);
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-        if ((uint32_t)k < cnt)
-            seg[k] = a[k];
-}
-
-__device__ __forceinline__ void sort_small_cell(const uint32_t* __restrict__ offsets,
-                                                uint32_t* __restrict__ refs, uint32_t c)
-{
-    const uint32_t b = offsets[c];
-    const uint32_t cnt = offsets[c + 1] - b;
-    if (cnt <= 8)
-        sort_cell_thread<8>(refs + b, cnt);
-    else
-        sort_cell_thread<16>(refs + b, cnt);
-}
-
 // Cells of 2..kSortSmallCells references (lists.small), one thread per cell.
 __global__ __launch_bounds__(256) void k_sort_small(const DevFrame* __restrict__ fr,
                                                     const uint32_t* __restrict__ offsets,
@@ -972,140 +903,7 @@ __global__ __launch_bounds__(256) void k_sort_small(const DevFrame* __restrict__
         sort_small_cell(offsets, refs, lists.small[li]);
 }
 
-// Bitonic sort written with ascending comparators only (the first comparator
-// of each merge mirrors the block: i <-> i^(k-1)), so the virtual padding up to
-// a power of two is +inf at the top and never needs storage: a comparator whose
-// upper index is >= n is a no-op.
-__device__ __forceinline__ void cex(uint32_t* a, uint32_t i, uint32_t j)
-{
-    const uint32_t x = a[i], y = a[j];
-    if (y < x) {
-        a[i] = y;
-        a[j] = x;
-    }
-}
 
-// Full sort of lds[0..n), n <= kSortTile, by the calling block.
-__device__ void lds_sort(uint32_t* lds, uint32_t n)
-{
-    uint32_t p = 2;
-    while (p < n)
-        p <<= 1;
-    for (uint32_t k = 2; k <= p; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < (p >> 1); i += blockDim.x) {
-                const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-                const uint32_t hi = (j == (k >> 1)) ? (lo ^ (k - 1)) : (lo | j);
-                const uint32_t a = lo < hi ? lo : hi, b = lo < hi ? hi : lo;
-                if (b < n)
-                    cex(lds, a, b);
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Half-cleaners j = jmax .. 1 on lds[0..len) where the tile starts at global
-// index `g0` of a cell of n elements (comparators mirror nothing here).
-__device__ void lds_half_cleaners(uint32_t* lds, uint32_t len, uint32_t jmax)
-{
-    for (uint32_t j = jmax; j > 0; j >>= 1) {
-        for (uint32_t i = threadIdx.x; i < (kSortTile >> 1); i += blockDim.x) {
-            const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-            const uint32_t hi = lo | j;
-            if (hi < len)
-                cex(lds, lo, hi);
-        }
-        __syncthreads();
-    }
-}
-
-// Global comparator pass over refs[b..b+n) by one block.
-__device__ void global_pass(uint32_t* __restrict__ a, uint32_t n, uint32_t p, uint32_t k,
-                            uint32_t j)
-{
-    for (uint32_t i = threadIdx.x; i < (p >> 1); i += blockDim.x) {
-        const uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-        const uint32_t hi = (j == (k >> 1)) ? (lo ^ (k - 1)) : (lo | j);
-        const uint32_t x = lo < hi ? lo : hi, y = lo < hi ? hi : lo;
-        if (y < n)
-            cex(a, x, y);
-    }
-    __syncthreads();
-}
-
-// One wave sorts one cell of up to 64*E references in registers: element
-// i = e*64 + lane sits in lane `lane`, register e; the bitonic network compares
-// i with i^j ascending when (i & k) == 0 -- across lanes by shuffles (j < 64),
-// within a lane between registers (j >= 64).  Slots past n hold 0xffffffff,
-// which sorts last.
-// One comparator layer (k = 2^LK, j = 2^LJ), then the next smaller j.
-template <int E, int LK, int LJ>
-__device__ __forceinline__ void bitonic_layer(uint32_t (&x)[E], uint32_t lane)
-{
-    constexpr uint32_t k = 1u << LK, j = 1u << LJ;
-    if constexpr (j >= 64u) {
-        constexpr uint32_t je = j >> 6;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (((uint32_t)e & je) == 0) {
-                const int f = e ^ (int)je;
-                const bool up = (((uint32_t)e * 64u) & k) == 0;
-                const uint32_t lo = x[e] < x[f] ? x[e] : x[f];
-                const uint32_t hi = x[e] < x[f] ? x[f] : x[e];
-                x[e] = up ? lo : hi;
-                x[f] = up ? hi : lo;
-            }
-        }
-    } else {
-        const bool lower = (lane & j) == 0;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint32_t o = (uint32_t)__shfl_xor((int)x[e], (int)j, 64);
-            const bool up = ((((uint32_t)e * 64u) | lane) & k) == 0;
-            const uint32_t lo = x[e] < o ? x[e] : o;
-            const uint32_t hi = x[e] < o ? o : x[e];
-            x[e] = (lower == up) ? lo : hi;
-        }
-    }
-    if constexpr (LJ > 0)
-        bitonic_layer<E, LK, LJ - 1>(x, lane);
-}
-
-// Merge stages k = 2^LK .. 64*E.
-template <int E, int LK>
-__device__ __forceinline__ void bitonic_stages(uint32_t (&x)[E], uint32_t lane)
-{
-    bitonic_layer<E, LK, LK - 1>(x, lane);
-    if constexpr ((1u << LK) < 64u * E)
-        bitonic_stages<E, LK + 1>(x, lane);
-}
-
-// One wave sorts one cell of up to 64*E references in registers: element
-// i = e*64 + lane sits in lane `lane`, register e; the bitonic network compares
-// i with i^j ascending when (i & k) == 0 -- across lanes by shuffles (j < 64),
-// within a lane between registers (j >= 64).  Slots past n hold 0xffffffff,
-// which sorts last.
-template <int E>
-__device__ __forceinline__ void wave_sort(uint32_t* __restrict__ seg, uint32_t n)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t x[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t i = (uint32_t)e * 64u + lane;
-        x[e] = i < n ? seg[i] : 0xffffffffu;
-    }
-    bitonic_stages<E, 1>(x, lane);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t i = (uint32_t)e * 64u + lane;
-        if (i < n)
-            seg[i] = x[e];
-    }
-}
-
-constexpr uint32_t kSortWaveMax = 4096;
 
 // Split lists.large by size class (see SortLists).  The waves of a block
 // reserve their runs in an LDS count per class, and the block then takes one
@@ -1185,57 +983,33 @@ __global__ __launch_bounds__(256) void k_sort_wave(const DevFrame* __restrict__ 
     }
 }
 
-// Sort of one cell segment seg[0..n), n > kSortWaveMax, by the whole block:
-// LDS bitonic sort when it fits a tile; otherwise each tile is sorted in LDS
-// and the merge stages with k > kSortTile run their j >= kSortTile
-// comparators in global memory and the rest tile by tile in LDS.
-__device__ void block_sort_cell(uint32_t* __restrict__ seg, uint32_t n, uint32_t* lds)
-{
-    if (n <= kSortTile) {
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-            lds[i] = seg[i];
-        __syncthreads();
-        lds_sort(lds, n);
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-            seg[i] = lds[i];
-        __syncthreads();
-        return;
-    }
-    uint32_t p = kSortTile;
-    while (p < n)
-        p <<= 1;
-    for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
-        const uint32_t len = (n - t0) < kSortTile ? (n - t0) : kSortTile;
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
-            lds[i] = seg[t0 + i];
-        __syncthreads();
-        lds_sort(lds, len);
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
-            seg[t0 + i] = lds[i];
-        __syncthreads();
-    }
-    for (uint32_t k = 2 * kSortTile; k <= p; k <<= 1) {
-        for (uint32_t j = k >> 1; j >= kSortTile; j >>= 1)
-            global_pass(seg, n, p, k, j);
-        for (uint32_t t0 = 0; t0 < n; t0 += kSortTile) {
-            const uint32_t len = (n - t0) < kSortTile ? (n - t0) : kSortTile;
-            for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
-                lds[i] = seg[t0 + i];
-            __syncthreads();
-            lds_half_cleaners(lds, len, kSortTile >> 1);
-            for (uint32_t i = threadIdx.x; i < len; i += blockDim.x)
-                seg[t0 + i] = lds[i];
-            __syncthreads();
-        }
-    }
-}
-
-// Cells of more than kSortWaveMax references (size class 4), one block each.
-__global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__ fr,
+// Cells of 1025..kSortWaveMax references (size class 3), one 256-thread
+// block each: four register-sorted runs merged through LDS.
+__global__ __launch_bounds__(256) void k_sort_block(const DevFrame* __restrict__ fr,
                                                     const uint32_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ refs, SortLists lists)
 {
-    __shared__ uint32_t lds[kSortTile];
+    __shared__ uint32_t lds[2 * kSortWaveMax];
+    if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
+        return;
+    uint32_t cnt = lists.counts[2 + 3];
+    cnt = cnt < lists.large_cap ? cnt : lists.large_cap;
+    const uint32_t* __restrict__ list = lists.cls + (size_t)3 * lists.large_cap;
+    for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        const uint32_t c = list[li];
+        const uint32_t b = offsets[c];
+        block_merge_sort<256>(refs + b, offsets[c + 1] - b, lds, lds + kSortWaveMax);
+    }
+}
+
+// Cells of more than kSortWaveMax references (size class 4), one 1024-thread
+// block each: up to kSortTile by sixteen register-sorted runs merged through
+// LDS, longer ones by the tiled bitonic sort.
+__global__ __launch_bounds__(1024) void k_sort_large(const DevFrame* __restrict__ fr,
+                                                     const uint32_t* __restrict__ offsets,
+                                                     uint32_t* __restrict__ refs, SortLists lists)
+{
+    __shared__ uint32_t lds[2 * kSortTile];
     if (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver))
         return;
     uint32_t cnt = lists.counts[2 + 4];
@@ -1244,7 +1018,11 @@ __global__ __launch_bounds__(256) void k_sort_large(const DevFrame* __restrict__
     for (uint32_t li = blockIdx.x; li < cnt; li += gridDim.x) {
         const uint32_t c = list[li];
         const uint32_t b = offsets[c];
-        block_sort_cell(refs + b, offsets[c + 1] - b, lds);
+        const uint32_t n = offsets[c + 1] - b;
+        if (n <= kSortTile)
+            block_merge_sort<1024>(refs + b, n, lds, lds + kSortTile);
+        else
+            block_sort_cell(refs + b, n, lds);
     }
 }
 
@@ -1487,8 +1265,8 @@ hipError_t launch_grid_sort(const DevFrame* fr, const uint32_t* offsets, uint32_
     hipLaunchKernelGGL((k_sort_wave<1, 0>), dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
     hipLaunchKernelGGL((k_sort_wave<4, 1>), dim3(2048), dim3(256), 0, s, fr, offsets, refs, lists);
     hipLaunchKernelGGL((k_sort_wave<16, 2>), dim3(1024), dim3(256), 0, s, fr, offsets, refs, lists);
-    hipLaunchKernelGGL((k_sort_wave<64, 3>), dim3(512), dim3(256), 0, s, fr, offsets, refs, lists);
-    hipLaunchKernelGGL(k_sort_large, dim3(512), dim3(256), 0, s, fr, offsets, refs, lists);
+    hipLaunchKernelGGL(k_sort_block, dim3(1024), dim3(256), 0, s, fr, offsets, refs, lists);
+    hipLaunchKernelGGL(k_sort_large, dim3(256), dim3(1024), 0, s, fr, offsets, refs, lists);
     return hipGetLastError();
 }
 

@@ -14,26 +14,23 @@
 //
 // Work layout: the plan of every top cell comes from 4096-cell tiles (16
 // consecutive cells per thread); the dense cells -- few, and clustered where
-// the scene is -- then get one 256-thread block each.  In such a block thread
-// j owns sub-cell row j (y = j % r1, z = j / r1; r1 r2 <= 256 rows) and keeps
-// the row's counters or write cursors in registers, while the cell's
-// references stream through LDS in ascending order, 1024 at a time, as packed
-// 4-bit sub-cell ranges.  A thread appends a reference to its own rows only,
-// in stream order, so every sub-cell's list comes out ascending without
-// atomics or a sort.
+// the scene is -- then get one 1024-thread block each, one reference per
+// thread: the count pass adds to LDS counters over the reference's sub-cell
+// box (and keeps the box, 4 bits per bound, at the reference's position), the
+// fill pass scatters through LDS cursors, and every sub-cell's ids are then
+// sorted ascending -- by a thread, a wave or, past 1024 ids, a block.
 //   k_g2_plan<false>  per tile: sub-cells and dense cells (tile sums)
 //   k_g2_scan_tiles   exclusive scan of both; totals, capacity
 //   k_g2_plan<true>   per tile: cell records, the ordered dense-cell list
-//   k_g2_cells<false> per dense cell: its sub-references
+//   k_g2_count        per dense cell: boxes, sub-cell counts, sub-references
 //   k_g2_scan_dense   exclusive scan of those; total, capacity
-//   k_g2_cells<true>  per dense cell: sub-cell offsets and references
+//   k_g2_fill         per dense cell: sub-cell offsets, references, sorts
+//   k_g2_sort_big     sub-cells of more than 1024 references
 //   k_gather_refs     48-byte record of every sub-reference
 #include "grt_device.h"
 #include "grt_launch.h"
 
 namespace grt {
-
-constexpr uint32_t kG2Chunk = 1024;   // references staged in LDS per round
 
 // Packed resolution r0 | r1 << 8 | r2 << 16 of a top cell with n references
 // (0: no sub-grid) and its sub-cell count m.
@@ -68,79 +65,6 @@ __device__ __forceinline__ uint32_t g2_vox(float val, float bmin, float inv, uin
     return (uint32_t)x;
 }
 
-// One dense cell's references, streamed through LDS in order: COUNT adds to the
-// owned row's counters, else appends to the owned row's cursors.  Ranges are
-// read 4 references per 16-byte LDS load (the chunk's last nk % 4 one by one).
-// A range has lo <= hi on every axis, so the unsigned tests x - lo <= hi - lo
-// are exact.
-template <bool COUNT>
-__device__ __forceinline__ void g2_ref(uint32_t p, uint32_t t, uint32_t y, uint32_t z,
-                                       uint32_t (&cnt)[kSubMax], uint32_t* sub_refs)
-{
-    const uint32_t ylo = (p >> 8) & 15u, yhi = (p >> 12) & 15u;
-    const uint32_t zlo = (p >> 16) & 15u, zhi = (p >> 20) & 15u;
-    if (y - ylo <= yhi - ylo && z - zlo <= zhi - zlo) {
-        const uint32_t xlo = p & 15u, xw = ((p >> 4) & 15u) - xlo;
-#pragma unroll
-        for (uint32_t x = 0; x < kSubMax; ++x) {
-            const bool in = x - xlo <= xw;
-            if (COUNT) {
-                cnt[x] += in ? 1u : 0u;
-            } else if (in) {
-                sub_refs[cnt[x]] = t;
-                ++cnt[x];
-            }
-        }
-    }
-}
-
-template <bool COUNT>
-__device__ __forceinline__ void g2_stream(const G2Build& b, uint32_t kb, uint32_t ke,
-                                         const float sb[3], const float si[3], const uint32_t r[3],
-                                         bool owner, uint32_t y, uint32_t z,
-                                         uint32_t (&cnt)[kSubMax], uint32_t* s_rng,
-                                         uint32_t* s_id)
-{
-    for (uint32_t k0 = kb; k0 < ke; k0 += kG2Chunk) {
-        const uint32_t nk = ke - k0 < kG2Chunk ? ke - k0 : kG2Chunk;
-        for (uint32_t i = threadIdx.x; i < nk; i += blockDim.x) {
-            const uint32_t t = b.refs[k0 + i];
-            const float* v = b.verts + 9ull * t;
-            uint32_t p = 0;
-            for (int a = 0; a < 3; ++a) {
-                const float x = v[a], yv = v[3 + a], zv = v[6 + a];
-                float mn = x, mx = x;
-                mn = yv < mn ? yv : mn;
-                mx = yv > mx ? yv : mx;
-                mn = zv < mn ? zv : mn;
-                mx = zv > mx ? zv : mx;
-                const uint32_t lo = g2_vox(mn, sb[a], si[a], r[a]);
-                const uint32_t hi = g2_vox(mx, sb[a], si[a], r[a]);
-                p |= (lo | (hi << 4)) << (8 * a);
-            }
-            s_rng[i] = p;
-            s_id[i] = t;
-        }
-        __syncthreads();
-        if (owner) {
-            const uint4* r4 = reinterpret_cast<const uint4*>(s_rng);
-            const uint4* t4 = reinterpret_cast<const uint4*>(s_id);
-            const uint32_t n4 = nk / 4u;
-            for (uint32_t i = 0; i < n4; ++i) {
-                const uint4 p = r4[i];
-                const uint4 t = t4[i];
-                g2_ref<COUNT>(p.x, t.x, y, z, cnt, b.sub_refs);
-                g2_ref<COUNT>(p.y, t.y, y, z, cnt, b.sub_refs);
-                g2_ref<COUNT>(p.z, t.z, y, z, cnt, b.sub_refs);
-                g2_ref<COUNT>(p.w, t.w, y, z, cnt, b.sub_refs);
-            }
-            for (uint32_t i = 4u * n4; i < nk; ++i)
-                g2_ref<COUNT>(s_rng[i], s_id[i], y, z, cnt, b.sub_refs);
-        }
-        __syncthreads();
-    }
-}
-
 // Status bits of the second level in g2->status.
 __device__ __forceinline__ bool top_ok(const DevFrame* fr)
 {
@@ -148,12 +72,17 @@ __device__ __forceinline__ bool top_ok(const DevFrame* fr)
 }
 
 // 1 (LIST = false): per 4096-cell tile, the sub-cells and dense cells it holds.
-// 3 (LIST = true): per tile, every cell's record {refs begin, end, first
-// sub-cell, plan} and the tile's dense cells into the global dense list, in
-// cell order.  16 consecutive cells per thread.
+// 3 (LIST = true): per tile, every cell's 8-byte record -- {0x80000000 | first
+// sub-cell, plan} for a cell with a sub-grid, else {refs begin, end} -- and
+// the tile's dense cells into the global dense list, in cell order.  Slot k of
+// thread j is cell base + 256 k + j, so loads and record stores are coalesced;
+// in-tile positions come from a wave scan per slot (sub-cells in bits 0..23,
+// the dense flag in bits 24..31: a wave's sums stay below 2^18 and 64) and a
+// scan of the 64 (slot, wave) totals in slot-major order.
 template <bool LIST>
 __global__ __launch_bounds__(256) void k_g2_plan(G2Build b)
 {
+    __shared__ uint32_t s_tot[2][64];
     __shared__ uint32_t s_scan[8];
     const DevFrame* fr = b.fr;
     if (!top_ok(fr))
@@ -168,40 +97,61 @@ __global__ __launch_bounds__(256) void k_g2_plan(G2Build b)
         return;
     }
     const bool sub_ok = !LIST || b.g2->status == 0;
-    const uint32_t c0 = base + 16u * threadIdx.x;
-    uint32_t off[17];
-#pragma unroll
-    for (uint32_t k = 0; k < 17; ++k)
-        off[k] = c0 + k <= m_top ? b.offsets[c0 + k] : 0u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t packed[16], ms[16], nd = 0, msum = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) {
+        const uint32_t c = base + 256u * k + threadIdx.x;
         ms[k] = 0;
-        packed[k] = (c0 + k < m_top && sub_ok)
-                        ? g2_plan(fr, off[k + 1] - off[k], b.lambda2, b.min_refs, ms[k])
+        packed[k] = (c < m_top && sub_ok)
+                        ? g2_plan(fr, b.offsets[c + 1] - b.offsets[c], b.lambda2, b.min_refs, ms[k])
                         : 0u;
         nd += packed[k] ? 1u : 0u;
         msum += ms[k];
     }
-    uint32_t nd_total, m_total;
-    uint32_t dpos = block_excl_scan<256>(nd, s_scan, nd_total);
-    uint32_t mpos = block_excl_scan<256>(msum, s_scan, m_total);
     if (!LIST) {
+        uint32_t nd_total, m_total;
+        block_excl_scan<256>(nd, s_scan, nd_total);
+        block_excl_scan<256>(msum, s_scan, m_total);
         if (threadIdx.x == 0) {
             b.tile_m[blockIdx.x] = m_total;
             b.tile_d[blockIdx.x] = nd_total;
         }
         return;
     }
-    mpos += b.tile_m[blockIdx.x];
-    dpos += b.tile_d[blockIdx.x];
+    // ms[k] becomes the exclusive prefix within (slot k, this wave)
+    uint32_t ed[16];
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) {
-        if (c0 + k < m_top) {
-            b.cellrec[c0 + k] = make_uint4(off[k], off[k + 1], packed[k] ? mpos : 0u, packed[k]);
-            if (packed[k])
-                b.dense[dpos++] = c0 + k;
-            mpos += ms[k];
+        const uint32_t v = ms[k] | ((packed[k] ? 1u : 0u) << 24);
+        const uint32_t inc = wave_incl_scan(v);
+        const uint32_t ex = inc - v;
+        ms[k] = ex & 0xffffffu;
+        ed[k] = ex >> 24;
+        if (lane == 63u) {
+            s_tot[0][4u * k + wave] = inc & 0xffffffu;
+            s_tot[1][4u * k + wave] = inc >> 24;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t x0 = s_tot[0][lane], x1 = s_tot[1][lane];
+        s_tot[0][lane] = wave_incl_scan(x0) - x0;
+        s_tot[1][lane] = wave_incl_scan(x1) - x1;
+    }
+    __syncthreads();
+    const uint32_t tm = b.tile_m[blockIdx.x], td = b.tile_d[blockIdx.x];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) {
+        const uint32_t c = base + 256u * k + threadIdx.x;
+        if (c < m_top) {
+            if (packed[k]) {
+                const uint32_t mpos = tm + s_tot[0][4u * k + wave] + ms[k];
+                b.cellrec[c] = make_uint2(0x80000000u | mpos, packed[k]);
+                b.dense[td + s_tot[1][4u * k + wave] + ed[k]] = c;
+            } else {
+                b.cellrec[c] = make_uint2(b.offsets[c], b.offsets[c + 1]);
+            }
         }
     }
 }
@@ -219,6 +169,7 @@ __global__ __launch_bounds__(1024) void k_g2_scan_tiles(G2Build b, uint32_t n_ti
             g2->m = 0;
             g2->r = 0;
             g2->n[0] = 0;
+            g2->n[1] = 0;
             g2->status = kStatusFrameInvalid;
         }
         return;
@@ -246,18 +197,17 @@ __global__ __launch_bounds__(1024) void k_g2_scan_tiles(G2Build b, uint32_t n_ti
             status |= kStatusCellsOver;
         g2->m = (uint32_t)(cm > 0xffffffffull ? 0xffffffffull : cm);
         g2->n[0] = (int32_t)(cd > 0x7fffffffull ? 0x7fffffffull : cd);
+        g2->n[1] = 0;
         g2->r = 0;
         g2->status = status;
     }
 }
 
-// Frame of dense cell c's sub-grid and this thread's row.
+// Frame of dense cell c's sub-grid.
 struct G2Cell {
     uint32_t kb, ke, pk, m;
     uint32_t r[3];
     float sb[3], si[3];
-    bool owner;
-    uint32_t y, z;
 };
 
 __device__ __forceinline__ void g2_cell(const G2Build& b, uint32_t c, G2Cell& g)
@@ -276,70 +226,183 @@ __device__ __forceinline__ void g2_cell(const G2Build& b, uint32_t c, G2Cell& g)
         const float sc = fr->cell[a] / (float)g.r[a];
         g.si[a] = 1.0f / sc;
     }
-    const uint32_t row = threadIdx.x;
-    g.owner = row < g.r[1] * g.r[2];
-    g.y = g.r[1] ? row % g.r[1] : 0u;
-    g.z = g.r[1] ? row / g.r[1] : 0u;
 }
 
-// 4 (FILL = false): per dense cell (one block each, grid-strided), its
-// sub-references into dense_refs.  6 (FILL = true): per dense cell, the
-// sub-cell offsets and the references, ascending per sub-cell.
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_g2_cells(G2Build b)
+// Sub-cell box of triangle t in cell g's frame: lo | hi << 4 per axis, axis a
+// in bits 8a..8a+7.
+__device__ __forceinline__ uint32_t g2_box(const float* __restrict__ verts, uint32_t t,
+                                           const G2Cell& g)
 {
-    __shared__ __align__(16) uint32_t s_rng[kG2Chunk];
-    __shared__ __align__(16) uint32_t s_id[kG2Chunk];
-    __shared__ uint32_t s_scan[8];
-    if (!top_ok(b.fr) || (b.g2->status & ~(FILL ? 0u : kStatusRefsOver)) != 0)
+    const float* v = verts + 9ull * t;
+    uint32_t p = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float x = v[a], yv = v[3 + a], zv = v[6 + a];
+        float mn = x, mx = x;
+        mn = yv < mn ? yv : mn;
+        mx = yv > mx ? yv : mx;
+        mn = zv < mn ? zv : mn;
+        mx = zv > mx ? zv : mx;
+        const uint32_t lo = g2_vox(mn, g.sb[a], g.si[a], g.r[a]);
+        const uint32_t hi = g2_vox(mx, g.sb[a], g.si[a], g.r[a]);
+        p |= (lo | (hi << 4)) << (8 * a);
+    }
+    return p;
+}
+
+// 4: per dense cell (one block each, grid-strided), every reference's box
+// into b.box (at its top-reference position), the sub-cell counts into the
+// cell's sub_offsets slots (when present) and the cell's sub-reference total
+// into dense_refs.  References go one per thread with LDS atomic counters.
+__global__ __launch_bounds__(1024) void k_g2_count(G2Build b)
+{
+    __shared__ uint32_t s_cnt[kSubMax * kSubMax * kSubMax];
+    __shared__ uint32_t s_scan[17];
+    if (!top_ok(b.fr) || (b.g2->status & ~kStatusRefsOver) != 0)
         return;
     const uint32_t nd = (uint32_t)b.g2->n[0];
     for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
         const uint32_t c = b.dense[d];
         G2Cell g;
         g2_cell(b, c, g);
-        const uint32_t sub_base = b.cellrec[c].z;
-        uint32_t* row_off = b.sub_offsets ? b.sub_offsets + sub_base + g.r[0] * threadIdx.x
-                                          : nullptr;
-        uint32_t cnt[kSubMax];
-        if (!FILL) {
-            // The counts go into the row's offset slots for the fill pass.
-#pragma unroll
-            for (uint32_t x = 0; x < kSubMax; ++x)
-                cnt[x] = 0;
-            g2_stream<true>(b, g.kb, g.ke, g.sb, g.si, g.r, g.owner, g.y, g.z, cnt, s_rng, s_id);
-        } else {
-#pragma unroll
-            for (uint32_t x = 0; x < kSubMax; ++x)
-                cnt[x] = (g.owner && x < g.r[0]) ? row_off[x] : 0u;
-        }
+        for (uint32_t s = threadIdx.x; s < g.m; s += blockDim.x)
+            s_cnt[s] = 0u;
+        __syncthreads();
+        const uint32_t r0 = g.r[0], r01 = g.r[0] * g.r[1];
         uint32_t tot = 0;
-#pragma unroll
-        for (uint32_t x = 0; x < kSubMax; ++x)
-            tot += cnt[x];
+        for (uint32_t k = g.kb + threadIdx.x; k < g.ke; k += blockDim.x) {
+            const uint32_t p = g2_box(b.verts, b.refs[k], g);
+            b.box[k] = p;
+            const uint32_t xlo = p & 15u, xhi = (p >> 4) & 15u;
+            const uint32_t ylo = (p >> 8) & 15u, yhi = (p >> 12) & 15u;
+            const uint32_t zlo = (p >> 16) & 15u, zhi = (p >> 20) & 15u;
+            tot += (xhi - xlo + 1u) * (yhi - ylo + 1u) * (zhi - zlo + 1u);
+            for (uint32_t z = zlo; z <= zhi; ++z)
+                for (uint32_t y = ylo; y <= yhi; ++y)
+                    for (uint32_t x = xlo; x <= xhi; ++x)
+                        atomicAdd(s_cnt + z * r01 + y * r0 + x, 1u);
+        }
         uint32_t cell_total;
-        const uint32_t row_pre = block_excl_scan<256>(tot, s_scan, cell_total);
-        if (!FILL) {
-            if (threadIdx.x == 0)
-                b.dense_refs[d] = cell_total;
-            if (row_off && g.owner) {
+        block_excl_scan<1024>(tot, s_scan, cell_total);   // barrier: counts complete
+        if (threadIdx.x == 0)
+            b.dense_refs[d] = cell_total;
+        if (b.sub_offsets) {
+            uint32_t* out = b.sub_offsets + (b.cellrec[c].x & 0x7fffffffu);
+            for (uint32_t s = threadIdx.x; s < g.m; s += blockDim.x)
+                out[s] = s_cnt[s];
+        }
+        __syncthreads();
+    }
+}
+
+// 6: per dense cell, the sub-cell offsets (scan of the counts left in
+// sub_offsets), the references scattered through LDS cursors, then every
+// sub-cell's ids sorted ascending: 2..16 by one thread, 17..1024 by one
+// wave in registers; longer lists go to b.big for k_g2_sort_big.
+constexpr uint32_t kG2WaveSortMax = 1024;
+
+__global__ __launch_bounds__(1024) void k_g2_fill(G2Build b)
+{
+    constexpr uint32_t kM = kSubMax * kSubMax * kSubMax;
+    __shared__ uint32_t s_off[kM + 1];
+    __shared__ uint32_t s_cur[kM];
+    __shared__ uint32_t s_long[1024];
+    __shared__ uint32_t s_nlong;
+    __shared__ uint32_t s_scan[17];
+    if (!top_ok(b.fr) || b.g2->status != 0)
+        return;
+    const uint32_t nd = (uint32_t)b.g2->n[0];
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
+        const uint32_t c = b.dense[d];
+        G2Cell g;
+        g2_cell(b, c, g);
+        const uint32_t sub_base = b.cellrec[c].x & 0x7fffffffu;
+        uint32_t* out = b.sub_offsets + sub_base;
+        // offsets: 4 consecutive sub-cells per thread (m <= 4096)
+        const uint32_t s0 = 4u * threadIdx.x;
+        uint32_t v[4], sum = 0;
 #pragma unroll
-                for (uint32_t x = 0; x < kSubMax; ++x)
-                    if (x < g.r[0])
-                        row_off[x] = cnt[x];
+        for (uint32_t i = 0; i < 4; ++i) {
+            v[i] = s0 + i < g.m ? out[s0 + i] : 0u;
+            sum += v[i];
+        }
+        uint32_t cell_total;
+        uint32_t o = b.dense_refs[d] + block_excl_scan<1024>(sum, s_scan, cell_total);
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            if (s0 + i < g.m) {
+                s_off[s0 + i] = o;
+                s_cur[s0 + i] = o;
+                out[s0 + i] = o;
             }
-            continue;
+            o += v[i];
         }
-        uint32_t p = b.dense_refs[d] + row_pre;
-#pragma unroll
-        for (uint32_t x = 0; x < kSubMax; ++x) {
-            const uint32_t n = cnt[x];
-            cnt[x] = p;
-            p += n;
-            if (g.owner && x < g.r[0])
-                row_off[x] = cnt[x];
+        if (threadIdx.x == 0)
+            s_off[g.m] = b.dense_refs[d] + cell_total;
+        __syncthreads();
+        const uint32_t r0 = g.r[0], r01 = g.r[0] * g.r[1];
+        for (uint32_t k = g.kb + threadIdx.x; k < g.ke; k += blockDim.x) {
+            const uint32_t p = b.box[k];
+            const uint32_t t = b.refs[k];
+            const uint32_t xlo = p & 15u, xhi = (p >> 4) & 15u;
+            const uint32_t ylo = (p >> 8) & 15u, yhi = (p >> 12) & 15u;
+            const uint32_t zlo = (p >> 16) & 15u, zhi = (p >> 20) & 15u;
+            for (uint32_t z = zlo; z <= zhi; ++z)
+                for (uint32_t y = ylo; y <= yhi; ++y)
+                    for (uint32_t x = xlo; x <= xhi; ++x)
+                        b.sub_refs[atomicAdd(s_cur + z * r01 + y * r0 + x, 1u)] = t;
         }
-        g2_stream<false>(b, g.kb, g.ke, g.sb, g.si, g.r, g.owner, g.y, g.z, cnt, s_rng, s_id);
+        __syncthreads();
+        // ascending ids per sub-cell, in groups of blockDim sub-cells so the
+        // LDS list of longer ones never overflows
+        for (uint32_t g0 = 0; g0 < g.m; g0 += blockDim.x) {
+            if (threadIdx.x == 0)
+                s_nlong = 0u;
+            __syncthreads();
+            const uint32_t s = g0 + threadIdx.x;
+            if (s < g.m) {
+                const uint32_t n = s_off[s + 1] - s_off[s];
+                uint32_t* seg = b.sub_refs + s_off[s];
+                if (n >= 2u && n <= 8u)
+                    sort_cell_thread<8>(seg, n);
+                else if (n > 8u && n <= 16u)
+                    sort_cell_thread<16>(seg, n);
+                else if (n > 16u && n <= kG2WaveSortMax)
+                    s_long[atomicAdd(&s_nlong, 1u)] = s;
+                else if (n > kG2WaveSortMax)
+                    b.big[atomicAdd(reinterpret_cast<uint32_t*>(&b.g2->n[1]), 1u)] = sub_base + s;
+            }
+            __syncthreads();
+            const uint32_t nl = s_nlong;
+            for (uint32_t li = wave; li < nl; li += blockDim.x >> 6) {
+                const uint32_t sl = s_long[li];
+                const uint32_t n = s_off[sl + 1] - s_off[sl];
+                uint32_t* seg = b.sub_refs + s_off[sl];
+                if (n <= 64u)
+                    wave_sort<1>(seg, n);
+                else if (n <= 256u)
+                    wave_sort<4>(seg, n);
+                else
+                    wave_sort<16>(seg, n);
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+}
+
+// 7: sub-cells of more than kG2WaveSortMax references (b.big), one block each.
+__global__ __launch_bounds__(256) void k_g2_sort_big(G2Build b)
+{
+    __shared__ uint32_t lds[kSortTile];
+    if (!top_ok(b.fr) || b.g2->status != 0)
+        return;
+    const uint32_t nb = (uint32_t)b.g2->n[1];
+    for (uint32_t li = blockIdx.x; li < nb; li += gridDim.x) {
+        const uint32_t s = b.big[li];
+        const uint32_t o = b.sub_offsets[s];
+        block_sort_cell(b.sub_refs + o, b.sub_offsets[s + 1] - o, lds);
     }
 }
 
@@ -379,8 +442,8 @@ hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t 
     hipLaunchKernelGGL(k_g2_scan_tiles, dim3(1), dim3(1024), 0, s, b, nt, sub_cells_cap,
                        dense_cap);
     hipLaunchKernelGGL(k_g2_plan<true>, dim3(nt), dim3(256), 0, s, b);
-    const uint32_t nb = dense_cap < 4096u ? (dense_cap ? dense_cap : 1u) : 4096u;
-    hipLaunchKernelGGL(k_g2_cells<false>, dim3(nb), dim3(256), 0, s, b);
+    const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
+    hipLaunchKernelGGL(k_g2_count, dim3(nb), dim3(1024), 0, s, b);
     hipLaunchKernelGGL(k_g2_scan_dense, dim3(1), dim3(1024), 0, s, b, sub_refs_cap);
     return hipGetLastError();
 }
@@ -392,8 +455,10 @@ hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t
     hipError_t e0 = launch_grid2_plan(b, top_cells_cap, sub_cells_cap, sub_refs_cap, dense_cap, s);
     if (e0 != hipSuccess)
         return e0;
-    const uint32_t nb = dense_cap < 4096u ? (dense_cap ? dense_cap : 1u) : 4096u;
-    hipLaunchKernelGGL(k_g2_cells<true>, dim3(nb), dim3(256), 0, s, b);
+    const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
+    hipLaunchKernelGGL(k_g2_fill, dim3(nb), dim3(1024), 0, s, b);
+    const uint32_t nbig = b.big_cap < 256u ? (b.big_cap ? b.big_cap : 1u) : 256u;
+    hipLaunchKernelGGL(k_g2_sort_big, dim3(nbig), dim3(256), 0, s, b);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

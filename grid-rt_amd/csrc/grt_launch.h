@@ -95,11 +95,15 @@ struct G2Build {
     uint32_t min_refs;
     uint32_t* tile_m;           // per 4096-cell tile: sub-cells, scanned in place
     uint32_t* tile_d;           // per tile: dense cells, scanned in place
-    uint4* cellrec;             // per top cell {first ref, end ref, first sub-cell, r0|r1<<8|r2<<16}
+    uint2* cellrec;             // per top cell {0x80000000 | first sub-cell, r0|r1<<8|r2<<16}
+                                // with a sub-grid, else {first ref, end ref}
     uint32_t* dense;            // dense top cells in cell order [dense_cap]
     uint32_t* dense_refs;       // per dense cell: sub-references, then their prefix [dense_cap]
     uint32_t* sub_offsets;      // [sub-cells + 1]
     uint32_t* sub_refs;
+    uint32_t* box;              // per top reference: its sub-cell box (dense cells only)
+    uint32_t* big;              // sub-cells of more than 1024 references [big_cap]
+    uint32_t big_cap;
 };
 // Plan, dense list and per-cell counts: g2->m, g2->n[0], g2->r receive the
 // sub-cell, dense-cell and sub-reference totals, g2->status the capacities
@@ -133,7 +137,7 @@ struct TraceArgs {
     // Two-level grid (all set, or cellrec NULL for the top grid alone).  The
     // sub-grids are used only when g2->status is 0.
     const DevFrame* g2 = nullptr;
-    const uint4* cellrec = nullptr;        // per top cell {refs begin, end, first sub-cell, r}
+    const uint2* cellrec = nullptr;        // per top cell, as G2Build::cellrec
     const uint32_t* sub_offsets = nullptr;
     const float4* sub_trir = nullptr;
 };

@@ -1365,11 +1365,13 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
             ++cells;
             const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
             const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            const uint4 rec = a.cellrec[ci];
+            const uint2 rec = a.cellrec[ci];
+            const bool has_sub = (rec.x >> 31) != 0u;
             bool done = false;
-            if (sub_ok && rec.w != 0) {
-                const int32_t r0 = (int32_t)(rec.w & 255u), r1 = (int32_t)((rec.w >> 8) & 255u),
-                              r2 = (int32_t)((rec.w >> 16) & 255u);
+            if (sub_ok && has_sub) {
+                const uint32_t sub_base = rec.x & 0x7fffffffu;
+                const int32_t r0 = (int32_t)(rec.y & 255u), r1 = (int32_t)((rec.y >> 8) & 255u),
+                              r2 = (int32_t)((rec.y >> 16) & 255u);
                 const float sb0 = (float)s.cx * fr->cell[0] + fr->bmin[0];
                 const float sb1 = (float)s.cy * fr->cell[1] + fr->bmin[1];
                 const float sb2 = (float)s.cz * fr->cell[2] + fr->bmin[2];
@@ -1388,7 +1390,7 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
                 for (;;) {
                     ++cells;
                     const float sexit = tmin_f(tmin_f(tx, ty), tz);
-                    const uint32_t si = rec.z + (uint32_t)qx +
+                    const uint32_t si = sub_base + (uint32_t)qx +
                                         (uint32_t)r0 * ((uint32_t)qy + (uint32_t)r1 * (uint32_t)qz);
                     const uint32_t kb = a.sub_offsets[si], ke = a.sub_offsets[si + 1];
                     if (trace2_cell<ANY>(a.sub_trir, kb, ke, o0, o1, o2, d0, d1, d2, tmin, tmax,
@@ -1418,10 +1420,16 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
                         tz += dtz;
                     }
                 }
-            } else if (trace2_cell<ANY>(a.trir, rec.x, rec.y, o0, o1, o2, d0, d1, d2, tmin, tmax,
-                                        exit_t * kExitScale, id, best, bu, bv, tests)) {
-                done = true;
-                occ = ANY;
+            } else {
+                // the top cell's own list (also for a sub-grid cell when the
+                // second level is not valid)
+                const uint32_t kb = has_sub ? a.offsets[ci] : rec.x;
+                const uint32_t ke = has_sub ? a.offsets[ci + 1] : rec.y;
+                if (trace2_cell<ANY>(a.trir, kb, ke, o0, o1, o2, d0, d1, d2, tmin, tmax,
+                                     exit_t * kExitScale, id, best, bu, bv, tests)) {
+                    done = true;
+                    occ = ANY;
+                }
             }
             if (done || exit_t > tmax)
                 break;
