@@ -5,8 +5,9 @@ Workload ("teapot in a stadium", 2^20 binary32 triangles): 256 Gaussian
 clusters of small triangles + a 2-triangle ground plane 100x the cluster
 extent.  One step = one frame:
   1. per-cluster rigid transform of the rest pose (device),
-  2. uniform-grid rebuild on the device (lambda chosen by the autotuner among
-     8..512 before the timed region),
+  2. uniform-grid rebuild on the device -- one level (lambda 8..512) or two
+     (a coarse top grid with uniform sub-grids in its dense cells), the
+     configuration chosen by the autotuner before the timed region,
   3. 1920x1080 primary rays, closest hit,
   4. shadow rays toward a point light, any hit,
   5. one mirror bounce, closest hit,
@@ -40,6 +41,9 @@ N_CLUSTER_TRIS = (1 << 20) - 2
 WIDTH, HEIGHT = 1920, 1080
 LAMBDA = 5.0
 LAMBDAS = (8.0, 32.0, 64.0, 128.0, 256.0, 512.0)
+# Two-level grids: (top lambda, sub-grid lambda, minimum references of a cell
+# that gets a sub-grid).
+TWO_LEVEL = ((8.0, 4.0, 8), (8.0, 8.0, 8), (16.0, 4.0, 8), (4.0, 4.0, 8))
 LIGHT = (1000.0, 1000.0, 1000.0)
 OFFSET = 1e-4
 EYE, CENTER, UP = (0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0.0, 1.0, 0.0)
@@ -72,6 +76,9 @@ def parse():
                         "references left (0 = off)")
     p.add_argument("--lam", type=float, default=None,
                    help="grid resolution factor; default: tuned over %s" % (LAMBDAS,))
+    p.add_argument("--sub", default=None,
+                   help="second grid level 'LAMBDA2,MIN_REFS' or 'off' (with --lam); "
+                        "default: tuned over %s" % (TWO_LEVEL,))
     p.add_argument("--mode", default="frame", choices=["frame", "serve", "batch", "ao", "kdtree"],
                    help="frame: the stadium frame (default, headline line, with a short "
                         "serve-vs-plain latency section); serve / batch / ao: the serving, "
@@ -95,8 +102,16 @@ def intervals(ev):
 STAGES = (("primary", True), ("shadow", False), ("bounce", True), ("bounce_shadow", False))
 
 
-def kernel_name(closest, sched_char):
-    """The traversal kernel a stage launches under a schedule character."""
+def grid_tag(lam, sub):
+    """'LAMBDA' for a one-level grid, 'LAMBDA/LAMBDA2/MIN_REFS' for two levels."""
+    return "%g" % lam if sub is None else "%g/%g/%d" % (lam, sub[0], sub[1])
+
+
+def kernel_name(closest, sched_char, two_level=False):
+    """The traversal kernel a stage launches under a schedule character (a
+    two-level grid is walked by its own kernel under every schedule)."""
+    if two_level:
+        return "k_trace2<%s>" % ("false" if closest else "true")
     if sched_char == "p":
         return "k_trace_persistent<%s, false>" % ("false" if closest else "true")
     if sched_char in "248x":
@@ -108,9 +123,10 @@ def kernel_name(closest, sched_char):
 class Frame:
     """Device buffers and the per-frame pipeline of the stadium workload."""
 
-    def __init__(self, grt, torch, lam=LAMBDA, sched="spsp", refill=16):
+    def __init__(self, grt, torch, lam=LAMBDA, sched="spsp", refill=16, sub=None):
         self.grt, self.torch = grt, torch
         self.lam = lam
+        self.sub = sub
         self.sched = sched
         self.refill = refill
         self.ctx = grt.Context(0)
@@ -137,15 +153,20 @@ class Frame:
         self.phong = grt.phong({"light": LIGHT})
         # Grid buffers sized with headroom by one synchronous build of frame 0.
         self.xf = {}
-        self.set_lambda(lam)
+        self.set_grid(lam, sub)
         names = ["transform", "build", "primary", "shadow", "bounce", "bounce_shadow"]
         self.stage_names = names
 
-    def set_lambda(self, lam):
-        """Synchronous build of frame 0 at this resolution; reserves headroom
+    def set_grid(self, lam, sub=None):
+        """Synchronous build of frame 0 at this resolution (sub: the second
+        level's (lambda2, min_refs), or None for one level); reserves headroom
         so the per-frame rebuilds never reallocate."""
-        self.lam = lam
+        self.lam, self.sub = lam, sub
         self.scene.transform_clusters(self.transforms(0), TRIS_PER_CLUSTER, N_CLUSTER_TRIS)
+        if sub is None:
+            self.scene.set_subgrid(0.0, 8)
+        else:
+            self.scene.set_subgrid(float(sub[0]), int(sub[1]))
         self.scene.build_grid(lam)
         info = self.scene.grid_info()
         self.scene.reserve_grid(int(info["m"] * 1.25) + 4096, int(info["r"] * 1.5) + 4096)
@@ -246,27 +267,31 @@ class Frame:
         return int((f["tmin"] <= f["tmax"]).sum().item())
 
 
-def autotune(frame, lams, sched, lists):
-    """Pick lambda, the per-stage traversal schedule and the secondary stages'
-    ray lists on the frame itself (outside the timed region): for each lambda,
-    time every stage under each schedule, with and without ray lists (each
-    stage's time includes its ray generation and list compaction), and keep the
-    fastest per stage."""
+def autotune(frame, grids, sched, lists):
+    """Pick the grid, the per-stage traversal schedule and the secondary stages'
+    ray lists on the frame itself (outside the timed region).  grids lists
+    (lambda, sub) pairs, sub = None for one level or (lambda2, min_refs).  For
+    a one-level grid, every stage is timed under each schedule, with and
+    without ray lists (each stage's time includes its ray generation and list
+    compaction), and the fastest per stage is kept; a two-level grid has one
+    traversal kernel, so only its ray lists are chosen."""
     from grt.autotune import Autotuner
     tuner = Autotuner()
     table = {}
 
     def measure(c):
-        frame.set_lambda(c["lam"])
+        sub = tuple(c["sub"]) if c["sub"] is not None else None
+        tag = grid_tag(c["lam"], sub)
+        frame.set_grid(c["lam"], sub)
         if sched:
             frame.sched = sched
             frame.lists_kind = lists or "---"
             ms = frame.time_stages()
-            table[c["lam"]] = {"sched": sched, "lists": frame.lists_kind,
-                               "frame_ms": round(sum(ms), 3)}
+            table[tag] = {"sched": sched, "lists": frame.lists_kind,
+                          "frame_ms": round(sum(ms), 3), "lam": c["lam"], "sub": c["sub"]}
             return sum(ms)
         by = {}
-        for ch in Frame.SCHEDULES:
+        for ch in (Frame.SCHEDULES if sub is None else "s"):
             for kind in (("-",) if lists else ("-", "c")):
                 frame.sched = ch * 4
                 frame.lists_kind = lists or kind * 3
@@ -283,31 +308,39 @@ def autotune(frame, lams, sched, lists):
         best_lists = lists or "".join(kind for _, kind in picks[1:])
         total = sum(sum(ms[:2]) for ms in by.values()) / len(by) + \
             sum(by[k][2 + i] for i, k in enumerate(picks))
-        table[c["lam"]] = {"sched": best, "lists": best_lists, "frame_ms": round(total, 3)}
+        table[tag] = {"sched": best, "lists": best_lists, "frame_ms": round(total, 3),
+                      "lam": c["lam"], "sub": c["sub"]}
         return total
 
-    best = tuner.tune("stadium-1M", [{"lam": lam} for lam in lams], measure)
-    if not sched and len(lams) > 1:
-        # The two best resolutions are often within 1%: time each whole frame
-        # with its tuned schedule over more frames and keep the faster.
+    cands = [{"lam": lam, "sub": (list(sub) if sub is not None else None)} for lam, sub in grids]
+    best = tuner.tune("stadium-1M", cands, measure)
+    best_tag = grid_tag(best["lam"], tuple(best["sub"]) if best["sub"] is not None else None)
+    if not sched and len(grids) > 1:
+        # The two best grids are often within 1%: time each whole frame with
+        # its tuned schedule over more frames and keep the faster.
         top = sorted(table, key=lambda k: table[k]["frame_ms"])[:2]
         final = {}
-        for lam in top:
-            frame.set_lambda(lam)
-            frame.sched, frame.lists_kind = table[lam]["sched"], table[lam]["lists"]
-            final[lam] = sum(frame.time_stages(frames=7))
-            table[lam]["frame_ms_final"] = round(final[lam], 3)
-        best = {"lam": min(final, key=final.get)}
-    entry = table[best["lam"]]
-    return best["lam"], entry["sched"], entry["lists"], table
+        for tag in top:
+            e = table[tag]
+            frame.set_grid(e["lam"], tuple(e["sub"]) if e["sub"] is not None else None)
+            frame.sched, frame.lists_kind = e["sched"], e["lists"]
+            final[tag] = sum(frame.time_stages(frames=7))
+            e["frame_ms_final"] = round(final[tag], 3)
+        best_tag = min(final, key=final.get)
+        best = {"lam": table[best_tag]["lam"], "sub": table[best_tag]["sub"]}
+    entry = table[best_tag]
+    sub = tuple(best["sub"]) if best["sub"] is not None else None
+    return best["lam"], sub, entry["sched"], entry["lists"], table
 
 
 def stage_counters(frame, torch):
     """Per traversal stage, from device counters (outside the timed region):
     active rays, mean/max cells visited and triangle tests, and the stage's
     algorithmic bytes -- per active ray 32 B ray in + 16 B hit out (closest) or
-    4 B flag out (any) + 8 B per cell visited (offsets[c], offsets[c+1]) +
-    52 B per triangle test (4 B reference + 48 B triangle record)."""
+    4 B flag out (any) + 8 B per cell visited (offsets[c], offsets[c+1]; on a
+    two-level grid the 8-byte cell record of a top cell and the offset pair of
+    a sub-cell, both counted as cells) + 52 B per triangle test (4 B reference
+    + 48 B triangle record)."""
     out = {}
     cnt = torch.empty(2 * frame.n, dtype=torch.int32, device="cuda:0")
     rays_of = {"primary": frame.prim, "shadow": frame.shadow, "bounce": frame.refl,
@@ -354,7 +387,7 @@ def host_description():
     return {"cpu_model": model, "nproc": os.cpu_count() or 1, "allowed_cpus": allowed}
 
 
-def cpu_baseline(frame, torch, budget_s):
+def cpu_baseline(frame, torch, budget_s, lam):
     """The oracle (C, OpenMP over rows, schedule(dynamic, 16 rows), mirroring
     MultiThreadRenderer.cpp:23-27) on the frame's primary + shadow rays with
     every CPU this process may use (OMP_NUM_THREADS when the host sets it):
@@ -362,7 +395,8 @@ def cpu_baseline(frame, torch, budget_s):
     the whole frame repeated until about budget_s has been spent.  Also the
     same trace on one thread (SingleThreadRenderer.cpp:8-33) on a row sample
     of about budget_s / 3, and the single-thread grid build of the frame
-    (UniformGridAccStructBuilder::buildGeometry)."""
+    (UniformGridAccStructBuilder::buildGeometry).  The grid is the reference's
+    one-level grid at `lam`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     host = host_description()
@@ -370,7 +404,7 @@ def cpu_baseline(frame, torch, budget_s):
     threads = max(1, env_threads or host["allowed_cpus"])
     v, nrm = frame.scene.download_geometry()
     t0 = time.perf_counter()
-    fr = orc.grid_frame(v, frame.lam)
+    fr = orc.grid_frame(v, lam)
     _, offs, refs = orc.grid_build(v, fr)
     build_s = time.perf_counter() - t0
     rays = orc.primary_rays(frame.grt.camera_dict(frame.cam))
@@ -420,7 +454,7 @@ def cpu_baseline(frame, torch, budget_s):
                        f"{n_rays} rays in {spent:.1f} s on {threads} threads (every CPU "
                        f"the process may use, or OMP_NUM_THREADS when set), OpenMP "
                        f"schedule(dynamic) over 16-row chunks, same grid (lambda "
-                       f"{frame.lam:g}); single-thread grid build of the frame "
+                       f"{lam:g}, one level); single-thread grid build of the frame "
                        f"{build_s * 1e3:.0f} ms")}
 
 
@@ -451,7 +485,11 @@ def main():
             res = perf.ao_config()
         print(json.dumps({"mode": args.mode, **res}))
         return
-    frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill)
+    sub_arg = None
+    if args.sub is not None and args.sub != "off":
+        l2, mr = args.sub.split(",")
+        sub_arg = (float(l2), int(mr))
+    frame = Frame(grt, torch, args.lam or LAMBDA, args.sched or "spsp", args.refill, sub_arg)
     if args.ray_lists is not None and (len(args.ray_lists) != 3 or
                                        set(args.ray_lists) - set("-c")):
         raise SystemExit("--ray-lists takes three letters of - and c (shadow, bounce, "
@@ -461,11 +499,15 @@ def main():
     frame.ctx.set_coop(args.coop)
     if args.copy_calibration:
         copy_calibration(torch)
-    lams = [args.lam] if args.lam else list(LAMBDAS)
-    lam, sched, lists, tune_table = autotune(frame, lams, args.sched, args.ray_lists)
+    if args.lam:
+        grids = [(args.lam, sub_arg)]
+    else:
+        grids = [(lam, None) for lam in LAMBDAS] + [(l1, (l2, mr)) for l1, l2, mr in TWO_LEVEL]
+    lam, sub, sched, lists, tune_table = autotune(frame, grids, args.sched, args.ray_lists)
     frame.sched = sched
     frame.lists_kind = lists
-    frame.set_lambda(lam)
+    frame.set_grid(lam, sub)
+    tag = grid_tag(lam, sub)
     for w in range(args.warmup):
         frame.run(1000 + w)
     frame.scene.check_grid()
@@ -488,7 +530,7 @@ def main():
     stage_ms = {"transform": d[0], "build": d[1], "raygen": d[2] + d[4] + d[6] + d[8],
                 **trace_ms, "shade": d[10]}
     if args.profile:
-        print(json.dumps({"mode": "profile", "lambda": lam, "schedule": sched, "coop": args.coop,
+        print(json.dumps({"mode": "profile", "grid": tag, "schedule": sched, "coop": args.coop,
                           "lists": frame.lists_kind,
                           "ms_per_step": round(ms_per_step, 3),
                           "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}}))
@@ -498,6 +540,7 @@ def main():
             "bounce": frame.active(frame.refl), "bounce_shadow": frame.active(frame.shadow2)}
     value = sum(rays.values()) / (ms_per_step * 1e-3) / 1e6
     info = frame.scene.grid_info()
+    sub_info = frame.scene.subgrid_info() if sub is not None else None
     counters = stage_counters(frame, torch)
 
     # Roofline of the dominant traversal launch: the stage whose traversal
@@ -505,13 +548,13 @@ def main():
     # achieved = that stage's algorithmic bytes over its launch duration.
     dom_stage = max(trace_ms, key=trace_ms.get)
     si = [name for name, _ in STAGES].index(dom_stage)
-    dom = kernel_name(STAGES[si][1], sched[si])
+    dom = kernel_name(STAGES[si][1], sched[si], sub is not None)
     dom_bytes = counters[dom_stage]["algorithmic_bytes"]
     achieved = dom_bytes / (trace_ms[dom_stage] * 1e-3) / 1e9
     # DRAM bytes of that launch at this lambda, from the committed
     # FETCH_SIZE / WRITE_SIZE profile (null if it has no such line).
-    traffic_of = {name: profile_traffic(lam, sched, frame.lists_kind, name,
-                                        kernel_name(closest, sched[i]))
+    traffic_of = {name: profile_traffic(tag, sched, frame.lists_kind, name,
+                                        kernel_name(closest, sched[i], sub is not None))
                   for i, (name, closest) in enumerate(STAGES)}
     traffic = traffic_of[dom_stage]
     per_stage_roof = {}
@@ -548,9 +591,15 @@ def main():
             "workload": "stadium-1M: 2^20 tris (256 Gaussian clusters + 2-tri ground plane), "
                         "1920x1080 primary + shadow + 1 mirror bounce + bounce shadow, "
                         "Phong shading of both levels into an RGBA8 frame, "
-                        f"per-frame transform + device grid rebuild (lambda={lam:g}, autotuned)",
+                        f"per-frame transform + device grid rebuild ({grid_desc(lam, sub)}, "
+                        f"autotuned)",
             "rays_per_frame": rays,
-            "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"]},
+            "grid": {"n": [int(x) for x in info["n"]], "cells": info["m"], "refs": info["r"],
+                     "lambda": lam,
+                     "subgrid": (None if sub_info is None else
+                                 {"lambda2": sub[0], "min_refs": sub[1],
+                                  "sub_cells": sub_info["cells"],
+                                  "sub_refs": sub_info["refs"]})},
             "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
             "value_definition": "every ray the frame traces -- closest-hit rays (primary, "
                                 "mirror bounce) and shadow rays of both levels, inactive "
@@ -565,9 +614,9 @@ def main():
             "roofline_by_stage": per_stage_roof,
             "stage_counters": counters,
             "traversal": {"schedule_per_stage": sched, "refill": args.refill,
-                          "block": args.block or 128, "lambda": lam, "coop": args.coop,
+                          "block": args.block or 128, "grid": tag, "coop": args.coop,
                           "secondary_ray_lists": frame.lists_kind},
-            "autotune_frame_ms_by_lambda": tune_table,
+            "autotune_frame_ms_by_grid": tune_table,
             "parallelism": "single GPU",
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -590,19 +639,32 @@ def main():
         lat["triangles"] = tris
         out["latency"] = lat
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(frame, torch, args.cpu_seconds)
+        # the reference's one-level grid: at the tuned lambda, or for a
+        # two-level pick at the fastest one-level lambda of the tuning table
+        one = {k: e for k, e in tune_table.items() if e.get("sub") is None and "/" not in k}
+        cpu_lam = lam if sub is None else (
+            min(one.values(), key=lambda e: e.get("frame_ms_final", e["frame_ms"]))["lam"]
+            if one and all("lam" in e for e in one.values()) else 256.0)
+        out["cpu_baseline"] = cpu_baseline(frame, torch, args.cpu_seconds, cpu_lam)
     print(json.dumps(out))
 
 
-def profile_traffic(lam, sched, lists, stage, kernel):
-    """HBM bytes per launch of a stage's traversal kernel at this lambda,
-    schedule and ray-list choice, from the committed FETCH_SIZE / WRITE_SIZE
-    profile (None when the profile has no such line)."""
+def grid_desc(lam, sub):
+    if sub is None:
+        return f"one level, lambda={lam:g}"
+    return (f"two levels: top lambda={lam:g}, sub-grids at lambda2={sub[0]:g} in cells of "
+            f">= {sub[1]} references")
+
+
+def profile_traffic(tag, sched, lists, stage, kernel):
+    """HBM bytes per launch of a stage's traversal kernel on this grid (its
+    grid_tag), schedule and ray-list choice, from the committed FETCH_SIZE /
+    WRITE_SIZE profile (None when the profile has no such line)."""
     tpath = os.path.join(ROOT, "profiles", "trace_stadium_traffic.md")
     if not os.path.exists(tpath):
         return None
-    key = "hbm_bytes_per_launch[lambda=%g][sched=%s][lists=%s][%s][%s]:" % (
-        lam, sched, lists, stage, kernel)
+    key = "hbm_bytes_per_launch[grid=%s][sched=%s][lists=%s][%s][%s]:" % (
+        tag, sched, lists, stage, kernel)
     for line in open(tpath):
         if line.startswith(key):
             return float(line.split(":", 1)[1])

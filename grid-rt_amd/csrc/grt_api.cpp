@@ -255,7 +255,7 @@ static int g2_reserve_top(grt_scene_t* sc)
 // The dense-cell list (and its reference counts) for `dense` cells.
 static int g2_reserve_dense(grt_scene_t* sc, uint32_t dense)
 {
-    const uint64_t dc = (uint64_t)dense + dense / 8 + 1024;
+    const uint64_t dc = (uint64_t)dense + dense / 4 + 1024;
     if (dc > 0x7ffffff0ull)
         return set_error(GRT_CAPACITY, "%u dense cells are too many", dense);
     if (dc <= sc->g2_dense_cap && sc->g2dense.ptr)
@@ -269,7 +269,8 @@ static int g2_reserve_dense(grt_scene_t* sc, uint32_t dense)
 
 static int g2_reserve_sub(grt_scene_t* sc, uint32_t cells, uint32_t refs)
 {
-    const uint64_t cc = (uint64_t)cells + cells / 8 + 4096, rc64 = (uint64_t)refs + refs / 8 + 4096;
+    // headroom for the rebuilds of later frames (clusters move)
+    const uint64_t cc = (uint64_t)cells + cells / 4 + 4096, rc64 = (uint64_t)refs + refs / 2 + 4096;
     if (cc > 0x7ffffff0ull || rc64 > 0xfffffff0ull)
         return set_error(GRT_CAPACITY, "sub-grids of %u cells / %u references are too large",
                          cells, refs);

@@ -240,8 +240,10 @@ int grt_grid_info_get(grt_scene scene, grt_grid_info* out);
  * counters count top cells plus sub-cells.  The top grid stays complete, so
  * batches, serving and instance traces (which use it alone) are unchanged.
  * lambda2 = 0 turns the level off.  Its buffers are sized by grt_grid_build
- * (and grt_grid_reserve); an asynchronous rebuild that outgrows them traces
- * the top grid alone and grt_grid_check reports GRT_CAPACITY. */
+ * (sub-cells + 1/4, references + 1/2, for the rebuilds of later frames) and
+ * grt_grid_reserve (the top grid's share); an asynchronous rebuild that
+ * outgrows them traces the top grid alone and grt_grid_check reports
+ * GRT_CAPACITY. */
 int grt_grid_set_subgrid(grt_scene scene, float lambda2, uint32_t min_refs);
 typedef struct {
     float lambda2;

@@ -17,6 +17,11 @@ over compacted ray lists, and the Phong frame is shaded from them.
 Bar: bit-exact against the oracle (its OpenMP path) -- transformed vertices,
 grid frame, offsets, references, generated rays, hit id / t / u / v,
 occlusion flags, per-ray cell and test counters, RGB and RGBA8 pixels.
+
+The two-level grids the autotuner also chooses among (a coarse top grid at
+lambda 4..16 with uniform sub-grids in its dense cells) are checked the same
+way against the oracle's two-level restatement, and against the one-level
+oracle's hits and occlusion.
 """
 import numpy as np
 import pytest
@@ -239,5 +244,122 @@ def test_bench_frame_four_stages(gpu_ctx, stadium, lam):
                                         rgb=o_rgb)
         assert_same(rgb.cpu().numpy().ravel(), o_rgb.ravel(), f"Phong rgb {tag}")
         assert_same(rgba.cpu().numpy().view(np.uint32), o_rgba, f"Phong rgba8 {tag}")
+    finally:
+        sc.close()
+
+
+# Two-level grids the frame autotuner chooses among: (top lambda, lambda2,
+# minimum references of a subdivided cell).
+TWO_LEVEL = [(8.0, 4.0, 8), (8.0, 8.0, 8), (16.0, 4.0, 8), (4.0, 4.0, 8)]
+
+
+def _two_level_stage(ctx, sc, rays, fr, offs, refs, g2, ov, closest, tag, lst, count):
+    """One stage through the two-level grid: whole image under two trace
+    modes (both walk the same two-level kernel) and as a compacted list, all
+    equal to the oracle's two-level traversal; returns the oracle's result."""
+    n = rays.n
+    o = rays.numpy()
+    active = o["tmin"] <= o["tmax"]
+    fn = orc.grid2_trace_closest if closest else orc.grid2_trace_any
+    ores = fn(fr, offs, refs, g2, ov, o, counters=True)
+    for mode, listed in (("simple", False), ("persistent", False), ("simple", True)):
+        ctx.set_trace_mode(mode, 16)
+        cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
+        t = f"{tag} [{mode}{' listed' if listed else ''}]"
+        if closest:
+            hb = grt.HitBatch.empty(n)
+            if listed:
+                ctx.compact_rays(rays, WIDTH, lst, count, miss_hits=hb)
+                sc.trace_closest_list(rays, hb, lst, count, counters=cnt)
+            else:
+                sc.trace_closest(rays, hb, image_width=WIDTH, counters=cnt)
+            g = hb.numpy()
+            for k in ("id", "t", "u", "v"):
+                assert_same(g[k], ores[k], f"hit.{k} {t}")
+        else:
+            occ = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
+            if listed:
+                ctx.compact_rays(rays, WIDTH, lst, count, occluded=occ)
+                sc.trace_any_list(rays, occ, lst, count, counters=cnt)
+            else:
+                sc.trace_any(rays, occ, image_width=WIDTH, counters=cnt)
+            assert_same(occ.cpu().numpy(), ores["occluded"], f"occluded {t}")
+        gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+        sel = active if listed else slice(None)
+        assert_same(gc[sel, 0], ores["cells"][sel], f"cells {t}")
+        assert_same(gc[sel, 1], ores["tests"][sel], f"tests {t}")
+    ctx.set_trace_mode("simple", 16)
+    return ores
+
+
+@pytest.mark.parametrize("lam,lam2,min_refs", TWO_LEVEL)
+def test_bench_two_level_frame(gpu_ctx, stadium, lam, lam2, min_refs):
+    """The frame on each two-level grid the autotuner may pick, set up as the
+    bench does (frame 0 built synchronously with headroom, then a per-frame
+    transform + asynchronous rebuild of both levels): both levels equal the
+    oracle's, the four stages equal the oracle's two-level traversal bit for
+    bit (hits, flags, top cells + sub-cells, tests), and the closest hits and
+    occlusion equal the one-level oracle's (the reference semantics) except on
+    rays whose two nearest hits lie within the exit bound's 2^-20 slack."""
+    ctx = gpu_ctx
+    v, nrm, _ = stadium
+    sc = grt.Scene(ctx)
+    try:
+        sc.upload(v, nrm)
+        xf0, _, _ = _oracle_pose(stadium, 0)
+        sc.transform_clusters(xf0, PER_CLUSTER, N_CLUSTER_TRIS)
+        sc.set_subgrid(lam2, min_refs)
+        sc.build_grid(lam)
+        info = sc.grid_info()
+        sc.reserve_grid(int(info["m"] * 1.25) + 4096, int(info["r"] * 1.5) + 4096)
+        xf, ov, on = _oracle_pose(stadium, 1)
+        sc.transform_clusters(xf, PER_CLUSTER, N_CLUSTER_TRIS)
+        sc.rebuild_grid_async(lam)
+        sc.check_grid()
+        tag = f"[two-level {lam:g}/{lam2:g}/{min_refs}]"
+        info, offs_g, refs_g = sc.download_grid()
+        fr = orc.grid_frame(ov, lam)
+        _, offs, refs = orc.grid_build(ov, fr)
+        assert_same(offs_g, offs, f"top offsets {tag}")
+        assert_same(refs_g, refs, f"top refs {tag}")
+        si = sc.subgrid_info()
+        assert si["built"] and si["status"] == 0, si
+        g2 = orc.grid2_build(ov, fr, offs, refs, min_refs, lam2, 16)
+        cs, so, sr = sc.download_subgrid()
+        assert_same(cs.ravel(), g2[0].ravel(), f"cell plans {tag}")
+        assert_same(so, g2[1], f"sub-cell offsets {tag}")
+        assert_same(sr, g2[2], f"sub-cell refs {tag}")
+
+        n = WIDTH * HEIGHT
+        lst = torch.empty(n, dtype=torch.int32, device="cuda:0")
+        count = torch.zeros(1, dtype=torch.int32, device="cuda:0")
+        cam = grt.camera_lookat(EYE, CENTER, UP, FOVY, WIDTH, HEIGHT)
+        prim = grt.RayBatch.empty(n)
+        ctx.gen_primary(cam, prim)
+        oh = _two_level_stage(ctx, sc, prim, fr, offs, refs, g2, ov, True, f"primary {tag}",
+                              lst, count)
+        assert (oh["id"] >= 0).sum() > n // 4
+        hits = grt.HitBatch.from_numpy(oh)
+        orays = prim.numpy()
+        o1 = orc.trace_closest(fr, offs, refs, ov, orays)
+        same = (o1["id"] == oh["id"]) & (o1["t"].view(np.uint32) == oh["t"].view(np.uint32))
+        assert same.mean() > 0.9999, (int((~same).sum()), tag)
+
+        shadow = grt.RayBatch.empty(n)
+        sc.gen_shadow(prim, hits, LIGHT, OFFSET, shadow)
+        oocc = _two_level_stage(ctx, sc, shadow, fr, offs, refs, g2, ov, False,
+                                f"shadow {tag}", lst, count)
+        assert oocc["occluded"].sum() > 1000
+        a1 = orc.trace_any(fr, offs, refs, ov, shadow.numpy())["occluded"]
+        assert_same(oocc["occluded"], a1, f"two-level vs one-level occlusion {tag}")
+
+        refl = grt.RayBatch.empty(n)
+        sc.gen_reflection(prim, hits, OFFSET, refl)
+        orh = _two_level_stage(ctx, sc, refl, fr, offs, refs, g2, ov, True, f"bounce {tag}",
+                               lst, count)
+        shadow2 = grt.RayBatch.empty(n)
+        sc.gen_shadow(refl, grt.HitBatch.from_numpy(orh), LIGHT, OFFSET, shadow2)
+        _two_level_stage(ctx, sc, shadow2, fr, offs, refs, g2, ov, False,
+                         f"bounce shadow {tag}", lst, count)
     finally:
         sc.close()
