@@ -111,7 +111,8 @@ def kernel_name(closest, sched_char, two_level=False):
     """The traversal kernel a stage launches under a schedule character (a
     two-level grid is walked by its own kernel under every schedule)."""
     if two_level:
-        return "k_trace2<%s>" % ("false" if closest else "true")
+        kern = "k_trace2_split" if sched_char in "248x" else "k_trace2"
+        return "%s<%s>" % (kern, "false" if closest else "true")
     if sched_char == "p":
         return "k_trace_persistent<%s, false>" % ("false" if closest else "true")
     if sched_char in "248x":
@@ -273,8 +274,9 @@ def autotune(frame, grids, sched, lists):
     (lambda, sub) pairs, sub = None for one level or (lambda2, min_refs).  For
     a one-level grid, every stage is timed under each schedule, with and
     without ray lists (each stage's time includes its ray generation and list
-    compaction), and the fastest per stage is kept; a two-level grid has one
-    traversal kernel, so only its ray lists are chosen."""
+    compaction), and the fastest per stage is kept.  A two-level grid has two
+    traversal kernels, one ray per lane ('s') and 2..16 lanes per ray ('2',
+    '4', '8', 'x'), chosen per stage the same way."""
     from grt.autotune import Autotuner
     tuner = Autotuner()
     table = {}
@@ -291,7 +293,9 @@ def autotune(frame, grids, sched, lists):
                           "frame_ms": round(sum(ms), 3), "lam": c["lam"], "sub": c["sub"]}
             return sum(ms)
         by = {}
-        for ch in (Frame.SCHEDULES if sub is None else "s"):
+        # on a two-level grid 's', 'w' and 'p' launch the same one-ray-per-lane
+        # kernel; '2'..'x' its lanes-per-ray variant
+        for ch in (Frame.SCHEDULES if sub is None else "s248x"):
             for kind in (("-",) if lists else ("-", "c")):
                 frame.sched = ch * 4
                 frame.lists_kind = lists or kind * 3

@@ -229,6 +229,7 @@ static G2Build g2_args(grt_scene_t* sc, bool with_outputs)
     b.box = sc->g2box.as<uint32_t>();
     b.big = sc->g2big.as<uint32_t>();
     b.big_cap = sc->g2_big_cap;
+    b.sub_tab = sc->g2tab.as<float>();
     return b;
 }
 
@@ -242,6 +243,8 @@ static int g2_reserve_top(grt_scene_t* sc)
         sc->g2_box_cap = sc->refs_cap;
     }
     if ((rc = sc->g2frame.reserve(sc->ctx, sizeof(DevFrame), true)) != GRT_OK)
+        return rc;
+    if ((rc = sc->g2tab.reserve(sc->ctx, 96 * sizeof(float), true)) != GRT_OK)
         return rc;
     if ((rc = sc->cellrec.reserve(sc->ctx, (size_t)sc->cells_cap * 8, false)) != GRT_OK)
         return rc;
@@ -1068,8 +1071,15 @@ static TraceArgs trace_args(grt_scene sc, uint32_t n, uint32_t image_width, uint
         a.cellrec = sc->cellrec.as<uint2>();
         a.sub_offsets = sc->sub_offsets.as<uint32_t>();
         a.sub_trir = sc->sub_trir.as<float4>();
+        a.sub_tab = sc->g2tab.as<float>();
     }
     return a;
+}
+
+// Lanes per ray of a two-level trace: the split schedule's, else one.
+static uint32_t trace2_lanes(const grt_context_t* ctx)
+{
+    return ctx->trace_mode == GRT_TRACE_SPLIT ? ctx->split_lanes : 1u;
 }
 
 // One closest-hit launch of the context's schedule.
@@ -1079,7 +1089,7 @@ static int dispatch_closest(grt_scene sc, const TraceArgs& a, const grt_rays* ra
     const HitSoA h{hits->id, hits->t, hits->u, hits->v};
     grt_context_t* ctx = sc->ctx;
     if (a.cellrec)
-        return hip_check(launch_trace2(a, to_soa(rays), h, nullptr, ctx->block_closest,
+        return hip_check(launch_trace2(a, to_soa(rays), h, nullptr, trace2_lanes(ctx),
                                        ctx->stream),
                          "trace_closest");
     if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
@@ -1101,7 +1111,7 @@ static int dispatch_any(grt_scene sc, const TraceArgs& a, const grt_rays* rays,
 {
     grt_context_t* ctx = sc->ctx;
     if (a.cellrec)
-        return hip_check(launch_trace2(a, to_soa(rays), HitSoA{}, occluded_dev, ctx->block_any,
+        return hip_check(launch_trace2(a, to_soa(rays), HitSoA{}, occluded_dev, trace2_lanes(ctx),
                                        ctx->stream),
                          "trace_any");
     if (ctx->trace_mode == GRT_TRACE_PERSISTENT)

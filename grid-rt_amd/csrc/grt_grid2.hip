@@ -174,6 +174,13 @@ __global__ __launch_bounds__(1024) void k_g2_scan_tiles(G2Build b, uint32_t n_ti
         }
         return;
     }
+    if (threadIdx.x < 48u) {
+        // sub-cell size and reciprocal per axis and resolution (traversal table)
+        const uint32_t ax = threadIdx.x >> 4, r = (threadIdx.x & 15u) + 1u;
+        const float sc = fr->cell[ax] / (float)r;
+        b.sub_tab[threadIdx.x] = sc;
+        b.sub_tab[48u + threadIdx.x] = 1.0f / sc;
+    }
     const uint32_t used = (fr->m + kScanTile - 1) / kScanTile;
     const uint32_t nt = used < n_tiles ? used : n_tiles;
     uint64_t cm = 0, cd = 0;

@@ -104,6 +104,7 @@ struct G2Build {
     uint32_t* box;              // per top reference: its sub-cell box (dense cells only)
     uint32_t* big;              // sub-cells of more than 1024 references [big_cap]
     uint32_t big_cap;
+    float* sub_tab;             // [96]: cell_a / r at 16 a + r - 1, 1 / that at 48 + 16 a + r - 1
 };
 // Plan, dense list and per-cell counts: g2->m, g2->n[0], g2->r receive the
 // sub-cell, dense-cell and sub-reference totals, g2->status the capacities
@@ -140,13 +141,15 @@ struct TraceArgs {
     const uint2* cellrec = nullptr;        // per top cell, as G2Build::cellrec
     const uint32_t* sub_offsets = nullptr;
     const float4* sub_trir = nullptr;
+    const float* sub_tab = nullptr;        // 96 floats: sub-cell sizes, reciprocals (G2Build)
 };
 hipError_t launch_trace_closest(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
                                 uint32_t block, hipStream_t s);
-// Two-level traversal (a.cellrec set), one ray per lane: closest hit into
-// hits, or any hit into occluded when it is set.
+// Two-level traversal (a.cellrec set): closest hit into hits, or any hit into
+// occluded when it is set; one ray per lane (lanes <= 1) or `lanes` (2..16)
+// lanes per ray walking its top cells in turn.
 hipError_t launch_trace2(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
-                         uint8_t* occluded, uint32_t block, hipStream_t s);
+                         uint8_t* occluded, uint32_t lanes, hipStream_t s);
 hipError_t launch_trace_any(const TraceArgs& a, const RaySoA& rays, uint8_t* occluded,
                             uint32_t block, hipStream_t s);
 // Persistent traversal (closest hit, or any hit when `any`): lanes refill

@@ -1330,108 +1330,158 @@ __device__ __forceinline__ int32_t sub_vox(float val, float bmin, float inv, int
     return (int32_t)x;
 }
 
+// Per-ray constants of the two-level walk.
+struct Ray2 {
+    float o0, o1, o2, d0, d1, d2, i0, i1, i2, tmin, tmax;
+    bool n0, n1, n2;
+};
+
+// Sub-cell sizes and their reciprocals by resolution, staged in LDS by every
+// block of a two-level kernel: sc = tab[16 a + r - 1], 1/sc = tab[48 + 16 a +
+// r - 1] (the build evaluates cell_a / r and 1 / that, the frame's own
+// operations, once per frame).
+__device__ __forceinline__ void stage_sub_tab(const TraceArgs& a, float* tab)
+{
+    if (threadIdx.x < 96u)
+        tab[threadIdx.x] = a.sub_tab[threadIdx.x];
+    __syncthreads();
+}
+
+// One top cell (index ci, coordinates of s) of the two-level walk, entered at
+// t_enter and left at exit_t: its sub-grid from t_enter when it has one and
+// the second level is valid, else its own references bounded by exit_t.
+// Counts the sub-cells visited and the tests.  True when the walk ends in this
+// cell: a hit (closest), an occluder (any, *occ set) or a sub-cell whose exit
+// lies beyond tmax.
+template <bool ANY>
+__device__ __forceinline__ bool top_cell2(const TraceArgs& a, const DevFrame* __restrict__ fr,
+                                          const float* tab, bool sub_ok, const Ray2& r,
+                                          const Dda& s, uint32_t ci, float t_enter, float exit_t,
+                                          bool& occ, int32_t& id, float& best, float& bu,
+                                          float& bv, uint32_t& cells, uint32_t& tests)
+{
+    const uint2 rec = a.cellrec[ci];
+    const bool has_sub = (rec.x >> 31) != 0u;
+    if (!(sub_ok && has_sub)) {
+        // the top cell's own list (also for a sub-grid cell when the second
+        // level is not valid)
+        const uint32_t kb = has_sub ? a.offsets[ci] : rec.x;
+        const uint32_t ke = has_sub ? a.offsets[ci + 1] : rec.y;
+        if (trace2_cell<ANY>(a.trir, kb, ke, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax,
+                             exit_t * kExitScale, id, best, bu, bv, tests)) {
+            occ = ANY;
+            return true;
+        }
+        return false;
+    }
+    const uint32_t sub_base = rec.x & 0x7fffffffu;
+    const int32_t r0 = (int32_t)(rec.y & 255u), r1 = (int32_t)((rec.y >> 8) & 255u),
+                  r2 = (int32_t)((rec.y >> 16) & 255u);
+    const float sb0 = (float)s.cx * fr->cell[0] + fr->bmin[0];
+    const float sb1 = (float)s.cy * fr->cell[1] + fr->bmin[1];
+    const float sb2 = (float)s.cz * fr->cell[2] + fr->bmin[2];
+    const float sc0 = tab[r0 - 1], sc1 = tab[16 + r1 - 1], sc2 = tab[32 + r2 - 1];
+    const float si0 = tab[48 + r0 - 1], si1 = tab[64 + r1 - 1], si2 = tab[80 + r2 - 1];
+    int32_t qx = sub_vox(r.o0 + r.d0 * t_enter, sb0, si0, r0);
+    int32_t qy = sub_vox(r.o1 + r.d1 * t_enter, sb1, si1, r1);
+    int32_t qz = sub_vox(r.o2 + r.d2 * t_enter, sb2, si2, r2);
+    const int32_t sx = r.n0 ? -1 : 1, sy = r.n1 ? -1 : 1, sz = r.n2 ? -1 : 1;
+    const int32_t ox = r.n0 ? -1 : r0, oy = r.n1 ? -1 : r1, oz = r.n2 ? -1 : r2;
+    float tx = ((float)(qx + (r.n0 ? 0 : 1)) * sc0 + sb0 - r.o0) * r.i0;
+    float ty = ((float)(qy + (r.n1 ? 0 : 1)) * sc1 + sb1 - r.o1) * r.i1;
+    float tz = ((float)(qz + (r.n2 ? 0 : 1)) * sc2 + sb2 - r.o2) * r.i2;
+    const float dtx = fabsf(sc0 * r.i0), dty = fabsf(sc1 * r.i1), dtz = fabsf(sc2 * r.i2);
+    for (;;) {
+        ++cells;
+        const float sexit = tmin_f(tmin_f(tx, ty), tz);
+        const uint32_t si =
+            sub_base + (uint32_t)qx + (uint32_t)r0 * ((uint32_t)qy + (uint32_t)r1 * (uint32_t)qz);
+        const uint32_t kb = a.sub_offsets[si], ke = a.sub_offsets[si + 1];
+        if (trace2_cell<ANY>(a.sub_trir, kb, ke, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                             r.tmax, sexit * kExitScale, id, best, bu, bv, tests)) {
+            occ = ANY;
+            return true;
+        }
+        if (sexit > r.tmax)
+            return true;
+        if (tx < ty && tx < tz) {
+            qx += sx;
+            if (qx == ox)
+                return false;
+            tx += dtx;
+        } else if (ty < tz) {
+            qy += sy;
+            if (qy == oy)
+                return false;
+            ty += dty;
+        } else {
+            qz += sz;
+            if (qz == oz)
+                return false;
+            tz += dtz;
+        }
+    }
+}
+
+// Set-up of a two-level ray: constants, the top DDA and the clipped entry.
+__device__ __forceinline__ bool ray2_setup(const DevFrame* __restrict__ fr, const RaySoA& rays,
+                                           uint32_t i, Ray2& r, Dda& s, float& t_enter)
+{
+    r.o0 = rays.ox[i];
+    r.o1 = rays.oy[i];
+    r.o2 = rays.oz[i];
+    r.d0 = rays.dx[i];
+    r.d1 = rays.dy[i];
+    r.d2 = rays.dz[i];
+    r.tmin = rays.tmin[i];
+    r.tmax = rays.tmax[i];
+    if (!(grid_ok(fr) && dda_setup(fr, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, s)))
+        return false;
+    r.i0 = 1.0f / r.d0;
+    r.i1 = 1.0f / r.d1;
+    r.i2 = 1.0f / r.d2;
+    // the clipped entry, as dda_setup computes it
+    t_enter = r.tmin;
+    t_enter = tmax_f(t_enter, tmin_f((fr->bmin[0] - r.o0) * r.i0, (fr->bmax[0] - r.o0) * r.i0));
+    t_enter = tmax_f(t_enter, tmin_f((fr->bmin[1] - r.o1) * r.i1, (fr->bmax[1] - r.o1) * r.i1));
+    t_enter = tmax_f(t_enter, tmin_f((fr->bmin[2] - r.o2) * r.i2, (fr->bmax[2] - r.o2) * r.i2));
+    r.n0 = __builtin_signbit(r.d0);
+    r.n1 = __builtin_signbit(r.d1);
+    r.n2 = __builtin_signbit(r.d2);
+    return true;
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA hits,
                                                 uint8_t* __restrict__ occluded)
 {
+    __shared__ float tab[96];
+    stage_sub_tab(a, tab);
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid;
     const uint32_t i = trace_ray(a, slot, valid);
     if (!valid)
         return;
     const DevFrame* __restrict__ fr = a.fr;
-    const float o0 = rays.ox[i], o1 = rays.oy[i], o2 = rays.oz[i];
-    const float d0 = rays.dx[i], d1 = rays.dy[i], d2 = rays.dz[i];
-    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
     int32_t id = -1;
     float best = INFINITY, bu = 0.0f, bv = 0.0f;
     bool occ = false;
     uint32_t cells = 0, tests = 0;
-    Dda s;
     // Over its capacity the second level's records are not complete: the top
     // grid alone then.
     const bool sub_ok = a.g2->status == 0;
-    if (grid_ok(fr) && dda_setup(fr, o0, o1, o2, d0, d1, d2, tmin, tmax, s)) {
-        const float i0 = 1.0f / d0, i1 = 1.0f / d1, i2 = 1.0f / d2;
-        // the clipped entry, as dda_setup computes it
-        float t_enter = tmin;
-        t_enter = tmax_f(t_enter, tmin_f((fr->bmin[0] - o0) * i0, (fr->bmax[0] - o0) * i0));
-        t_enter = tmax_f(t_enter, tmin_f((fr->bmin[1] - o1) * i1, (fr->bmax[1] - o1) * i1));
-        t_enter = tmax_f(t_enter, tmin_f((fr->bmin[2] - o2) * i2, (fr->bmax[2] - o2) * i2));
-        const bool n0 = __builtin_signbit(d0), n1 = __builtin_signbit(d1),
-                   n2 = __builtin_signbit(d2);
+    Ray2 r;
+    Dda s;
+    float t_enter;
+    if (ray2_setup(fr, rays, i, r, s, t_enter)) {
         const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
         for (;;) {
             ++cells;
             const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
             const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            const uint2 rec = a.cellrec[ci];
-            const bool has_sub = (rec.x >> 31) != 0u;
-            bool done = false;
-            if (sub_ok && has_sub) {
-                const uint32_t sub_base = rec.x & 0x7fffffffu;
-                const int32_t r0 = (int32_t)(rec.y & 255u), r1 = (int32_t)((rec.y >> 8) & 255u),
-                              r2 = (int32_t)((rec.y >> 16) & 255u);
-                const float sb0 = (float)s.cx * fr->cell[0] + fr->bmin[0];
-                const float sb1 = (float)s.cy * fr->cell[1] + fr->bmin[1];
-                const float sb2 = (float)s.cz * fr->cell[2] + fr->bmin[2];
-                const float sc0 = fr->cell[0] / (float)r0, sc1 = fr->cell[1] / (float)r1,
-                            sc2 = fr->cell[2] / (float)r2;
-                const float si0 = 1.0f / sc0, si1 = 1.0f / sc1, si2 = 1.0f / sc2;
-                int32_t qx = sub_vox(o0 + d0 * t_enter, sb0, si0, r0);
-                int32_t qy = sub_vox(o1 + d1 * t_enter, sb1, si1, r1);
-                int32_t qz = sub_vox(o2 + d2 * t_enter, sb2, si2, r2);
-                const int32_t sx = n0 ? -1 : 1, sy = n1 ? -1 : 1, sz = n2 ? -1 : 1;
-                const int32_t ox = n0 ? -1 : r0, oy = n1 ? -1 : r1, oz = n2 ? -1 : r2;
-                float tx = ((float)(qx + (n0 ? 0 : 1)) * sc0 + sb0 - o0) * i0;
-                float ty = ((float)(qy + (n1 ? 0 : 1)) * sc1 + sb1 - o1) * i1;
-                float tz = ((float)(qz + (n2 ? 0 : 1)) * sc2 + sb2 - o2) * i2;
-                const float dtx = fabsf(sc0 * i0), dty = fabsf(sc1 * i1), dtz = fabsf(sc2 * i2);
-                for (;;) {
-                    ++cells;
-                    const float sexit = tmin_f(tmin_f(tx, ty), tz);
-                    const uint32_t si = sub_base + (uint32_t)qx +
-                                        (uint32_t)r0 * ((uint32_t)qy + (uint32_t)r1 * (uint32_t)qz);
-                    const uint32_t kb = a.sub_offsets[si], ke = a.sub_offsets[si + 1];
-                    if (trace2_cell<ANY>(a.sub_trir, kb, ke, o0, o1, o2, d0, d1, d2, tmin, tmax,
-                                         sexit * kExitScale, id, best, bu, bv, tests)) {
-                        done = true;
-                        occ = ANY;
-                        break;
-                    }
-                    if (sexit > tmax) {
-                        done = true;
-                        break;
-                    }
-                    if (tx < ty && tx < tz) {
-                        qx += sx;
-                        if (qx == ox)
-                            break;
-                        tx += dtx;
-                    } else if (ty < tz) {
-                        qy += sy;
-                        if (qy == oy)
-                            break;
-                        ty += dty;
-                    } else {
-                        qz += sz;
-                        if (qz == oz)
-                            break;
-                        tz += dtz;
-                    }
-                }
-            } else {
-                // the top cell's own list (also for a sub-grid cell when the
-                // second level is not valid)
-                const uint32_t kb = has_sub ? a.offsets[ci] : rec.x;
-                const uint32_t ke = has_sub ? a.offsets[ci + 1] : rec.y;
-                if (trace2_cell<ANY>(a.trir, kb, ke, o0, o1, o2, d0, d1, d2, tmin, tmax,
-                                     exit_t * kExitScale, id, best, bu, bv, tests)) {
-                    done = true;
-                    occ = ANY;
-                }
-            }
-            if (done || exit_t > tmax)
+            if (top_cell2<ANY>(a, fr, tab, sub_ok, r, s, ci, t_enter, exit_t, occ, id, best, bu,
+                               bv, cells, tests) ||
+                exit_t > r.tmax)
                 break;
             t_enter = exit_t;
             if (!dda_step(s))
@@ -1452,17 +1502,129 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
     }
 }
 
-hipError_t launch_trace2(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
-                         uint8_t* occluded, uint32_t block, hipStream_t s)
+// `count` DDA steps, each after checking the cell's exit against tmax (the
+// serial walk's stop); t_enter receives the exit of the last cell left.
+__device__ __forceinline__ bool dda_skip2(Dda& s, uint32_t count, float tmax, float& t_enter)
 {
-    (void)block;
+    for (uint32_t q = 0; q < count; ++q) {
+        const float e = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+        if (e > tmax || !dda_step(s))
+            return false;
+        t_enter = e;
+    }
+    return true;
+}
+
+// Several lanes per ray (2..16, `lanes`): lane j of a ray's group walks the
+// ray's top cells j, j + lanes, ... of the serial order, each one whole (its
+// sub-grid included).  After every round the group's earliest cell that ends
+// the walk (a hit, an occluder, a sub-cell exit beyond tmax) decides: its
+// result is the serial result, and the cells after it in that round are not
+// counted.  Latency of a long ray drops about `lanes`-fold; the sparse
+// secondary stages are bound by it.
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_trace2_split(TraceArgs a, RaySoA rays, HitSoA hits,
+                                                      uint8_t* __restrict__ occluded,
+                                                      uint32_t lanes)
+{
+    __shared__ float tab[96];
+    stage_sub_tab(a, tab);
+    const uint32_t shift = (uint32_t)__builtin_ctz(lanes);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid;
+    const uint32_t i = trace_ray(a, t >> shift, valid);
+    const DevFrame* __restrict__ fr = a.fr;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t j = lane & (lanes - 1u);
+    const uint32_t g0 = lane - j;
+    const uint64_t gmask = (lanes >= 64u ? ~0ull : ((1ull << lanes) - 1ull)) << g0;
+    const bool sub_ok = a.g2->status == 0;
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+    int32_t id = -1;
+    float best = INFINITY, bu = 0.0f, bv = 0.0f;
+    bool occ = false;
+    uint32_t cells = 0, tests = 0;
+    Ray2 r = {};
+    Dda s = {};
+    float t_enter = 0.0f;
+    bool alive = valid && ray2_setup(fr, rays, i, r, s, t_enter) && dda_skip2(s, j, r.tmax, t_enter);
+    bool winner = false;
+    while (__ballot(alive) != 0) {
+        const uint32_t c0 = cells, t0 = tests;
+        bool stop = false;
+        float exit_t = 0.0f;
+        if (alive) {
+            ++cells;
+            exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
+            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+            stop = top_cell2<ANY>(a, fr, tab, sub_ok, r, s, ci, t_enter, exit_t, occ, id, best,
+                                  bu, bv, cells, tests);
+        }
+        // The group's earliest cell that ends the walk ends the ray; cells
+        // after it (higher lanes of this round) are not part of the serial walk.
+        const uint64_t sm = __ballot(alive && stop) & gmask;
+        if (sm != 0) {
+            const uint32_t jw = (uint32_t)__builtin_ctzll(sm) - g0;
+            if (alive && j > jw) {
+                cells = c0;
+                tests = t0;
+            }
+            winner = alive && j == jw && (ANY ? occ : id >= 0);
+            alive = false;
+        } else if (alive) {
+            t_enter = exit_t;
+            alive = dda_skip2(s, lanes, r.tmax, t_enter);
+        }
+    }
+    const bool group_hit = (__ballot(winner) & gmask) != 0;
+    uint32_t sc = 0, st = 0;
+    if (a.counters) {
+        sc = group_sum(cells, lanes);
+        st = group_sum(tests, lanes);
+    }
+    if (!valid)
+        return;
+    if (ANY) {
+        if (j == 0)
+            occluded[i] = group_hit ? 1 : 0;
+    } else if (winner || (j == 0 && !group_hit)) {
+        hits.id[i] = winner ? id : -1;
+        hits.t[i] = winner ? best : INFINITY;
+        hits.u[i] = winner ? bu : 0.0f;
+        hits.v[i] = winner ? bv : 0.0f;
+    }
+    if (a.counters && j == 0) {
+        a.counters[2ull * i] = sc;
+        a.counters[2ull * i + 1] = st;
+    }
+}
+
+hipError_t launch_trace2(const TraceArgs& a, const RaySoA& rays, const HitSoA& hits,
+                         uint8_t* occluded, uint32_t lanes, hipStream_t s)
+{
     if (a.n == 0)
         return hipSuccess;
-    const uint32_t g = ray_grid(a.n, a.image_width, 128);
+    if (lanes <= 1) {
+        const uint32_t g = ray_grid(a.n, a.image_width, 128);
+        if (occluded)
+            hipLaunchKernelGGL(k_trace2<true>, dim3(g), dim3(128), 0, s, a, rays, hits, occluded);
+        else
+            hipLaunchKernelGGL(k_trace2<false>, dim3(g), dim3(128), 0, s, a, rays, hits,
+                               occluded);
+        return hipGetLastError();
+    }
+    if (lanes > 16 || (lanes & (lanes - 1)) != 0)
+        return hipErrorInvalidValue;
+    const uint64_t threads = slot_count(a.n, a.image_width) * lanes;
+    if (threads > 0xffffffffull - 256)
+        return hipErrorInvalidValue;
+    const dim3 g((uint32_t)((threads + 255) / 256));
     if (occluded)
-        hipLaunchKernelGGL(k_trace2<true>, dim3(g), dim3(128), 0, s, a, rays, hits, occluded);
+        hipLaunchKernelGGL(k_trace2_split<true>, g, dim3(256), 0, s, a, rays, hits, occluded,
+                           lanes);
     else
-        hipLaunchKernelGGL(k_trace2<false>, dim3(g), dim3(128), 0, s, a, rays, hits, occluded);
+        hipLaunchKernelGGL(k_trace2_split<false>, g, dim3(256), 0, s, a, rays, hits, occluded,
+                           lanes);
     return hipGetLastError();
 }
 

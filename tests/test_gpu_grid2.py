@@ -47,27 +47,39 @@ def check_levels(sc, v, lam, lam2, min_refs, tag=""):
     return fr, ooffs, orefs, (ocs, oso, osr)
 
 
-def check_trace2(sc, fr, offs, refs, g2, v, rays, image_width=0, one_level=True):
+# Trace modes: one ray per lane, and 2, 8 and 16 lanes per ray walking the
+# ray's top cells in turn (the split schedule).
+MODES = [("simple", 16), ("split", 2), ("split", 8), ("split", 16)]
+
+
+def check_trace2(sc, fr, offs, refs, g2, v, rays, image_width=0, one_level=True, modes=MODES):
     n = rays["ox"].shape[0]
     rb = grt.RayBatch.from_numpy(rays)
-    hb = grt.HitBatch.empty(n)
-    cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
-    sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
     o = orc.grid2_trace_closest(fr, offs, refs, g2, v, rays, counters=True)
-    g = hb.numpy()
-    gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-    for k in ("id", "t", "u", "v"):
-        assert_same(g[k], o[k], f"two-level hit.{k}")
-    assert_same(gc[:, 0], o["cells"], "two-level cells visited")
-    assert_same(gc[:, 1], o["tests"], "two-level triangle tests")
-    occ = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
-    cnt.fill_(-7)
-    sc.trace_any(rb, occ, image_width=image_width, counters=cnt)
     oa = orc.grid2_trace_any(fr, offs, refs, g2, v, rays, counters=True)
-    gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
-    assert_same(occ.cpu().numpy(), oa["occluded"], "two-level occluded")
-    assert_same(gc[:, 0], oa["cells"], "two-level any cells")
-    assert_same(gc[:, 1], oa["tests"], "two-level any tests")
+    ctx = sc.ctx
+    g = None
+    for mode, param in modes:
+        ctx.set_trace_mode(mode, param)
+        tag = f"[{mode} {param}]"
+        hb = grt.HitBatch.empty(n)
+        cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
+        sc.trace_closest(rb, hb, image_width=image_width, counters=cnt)
+        gm = hb.numpy()
+        gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+        for k in ("id", "t", "u", "v"):
+            assert_same(gm[k], o[k], f"two-level hit.{k} {tag}")
+        assert_same(gc[:, 0], o["cells"], f"two-level cells visited {tag}")
+        assert_same(gc[:, 1], o["tests"], f"two-level triangle tests {tag}")
+        occ = torch.full((n,), 9, dtype=torch.uint8, device="cuda:0")
+        cnt.fill_(-7)
+        sc.trace_any(rb, occ, image_width=image_width, counters=cnt)
+        gc = cnt.cpu().numpy().view(np.uint32).reshape(n, 2)
+        assert_same(occ.cpu().numpy(), oa["occluded"], f"two-level occluded {tag}")
+        assert_same(gc[:, 0], oa["cells"], f"two-level any cells {tag}")
+        assert_same(gc[:, 1], oa["tests"], f"two-level any tests {tag}")
+        g = g if g is not None else gm
+    ctx.set_trace_mode("simple", 16)
     if one_level:
         # the reference semantics: the one-level grid's hits and flags
         o1 = orc.trace_closest(fr, offs, refs, v, rays)
