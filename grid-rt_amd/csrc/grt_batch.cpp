@@ -31,6 +31,15 @@ struct grt_graph_t {
     hipGraphExec_t exec = nullptr;
 };
 
+int grt::batch_reserve_slots(grt_batch b, uint64_t slots)
+{
+    if (!b)
+        return set_error(GRT_INVALID_PARAM, "batch_reserve_slots: NULL batch");
+    if (slots > 0xffffffc0ull)
+        slots = 0xffffffc0ull;
+    return b->map.reserve(b->ctx, (slots / 64 + 1) * 4, false);
+}
+
 extern "C" {
 
 int grt_batch_create(grt_context ctx, uint32_t max_items, grt_batch* out)

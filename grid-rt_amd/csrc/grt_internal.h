@@ -20,6 +20,10 @@ int hip_check(hipError_t e, const char* what);
 // Allocation-free compaction (grt_noalloc.hip): take the current device's
 // address of the fixed scratch array, once per device (called by grt_init).
 int noalloc_setup(int device);
+// Size a batch's slot map for `slots` ray slots up front, so later
+// grt_batch_set calls of at most that many slots never reallocate it (a
+// pipelined caller keeps earlier launches reading it).
+int batch_reserve_slots(grt_batch b, uint64_t slots);
 
 // One device allocation, counted against its context.
 struct DevMem {

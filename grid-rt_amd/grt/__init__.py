@@ -243,6 +243,13 @@ def check(rc: int):
         raise GrtError(rc, lib().grt_last_error().decode(errors="replace"))
 
 
+def _call(name: str, *args) -> int:
+    """Call a libgrt entry point with its declared signature."""
+    fn = getattr(lib(), name)
+    fn.restype, fn.argtypes = _SIGS[name]
+    return fn(*args)
+
+
 def _ptr(a) -> int:
     """Data pointer of a numpy array or torch tensor (0 for None)."""
     if a is None:
@@ -939,3 +946,90 @@ def serve_run(ctx: Context, requests, batched: bool, memory_budget: int, max_act
     finish = np.array([res[i].finish_s for i in range(n)], np.float64)
     rays = np.array([res[i].rays for i in range(n)], np.uint64)
     return arrival, start, finish, rays, wall.value
+
+
+_SIGS.update({
+    "grt_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServeConfig), ctypes.POINTER(_P)]),
+    "grt_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(Request),
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "grt_server_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(RequestResult)]),
+    "grt_server_query": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(RequestResult)]),
+    "grt_server_destroy": (ctypes.c_int, [_P]),
+})
+
+
+class Server:
+    """Requests submitted as they come, from any thread (grt_server_*): a
+    thread of the library owns the device loop on the context's stream, which
+    is reserved to the server until close()."""
+
+    def __init__(self, ctx: Context, memory_budget: int, max_active: int = 64,
+                 batched: bool = True, light=(1000.0, 1000.0, 1000.0), offset: float = 1e-4):
+        self.ctx = ctx
+        cfg = ServeConfig()
+        cfg.memory_budget = memory_budget
+        cfg.max_active = max_active
+        cfg.batched = 1 if batched else 0
+        cfg.light[:] = list(light)
+        cfg.offset = offset
+        h = _P()
+        check(_call("grt_server_create", ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}   # ticket -> output buffers, alive until the result is taken
+
+    def submit(self, scene, camera, depth: int, out_hits=None, out_occluded=None) -> int:
+        """Queue a request; returns its ticket at once.  out_hits (HitBatch) and
+        out_occluded (uint8 tensor) receive the primary hits and shadow flags."""
+        r = Request()
+        r.scene = scene.h
+        r.camera = camera
+        r.depth = depth
+        r.arrival_s = 0.0
+        if out_hits is not None:
+            r.out_hits = out_hits.c()
+        if out_occluded is not None:
+            r.out_occluded = _ptr(out_occluded)
+        t = ctypes.c_uint64()
+        check(_call("grt_server_submit", self.h, ctypes.byref(r), ctypes.byref(t)))
+        self._keep[t.value] = (scene, out_hits, out_occluded)
+        return t.value
+
+    @staticmethod
+    def _res(r):
+        return {"start_s": r.start_s, "finish_s": r.finish_s, "rays": int(r.rays)}
+
+    def wait(self, ticket: int) -> dict:
+        r = RequestResult()
+        try:
+            check(_call("grt_server_wait", self.h, ticket, ctypes.byref(r)))
+        finally:
+            self._keep.pop(ticket, None)
+        return self._res(r)
+
+    def query(self, ticket: int):
+        """The result, or None while the request runs."""
+        r = RequestResult()
+        rc = _call("grt_server_query", self.h, ticket, ctypes.byref(r))
+        if rc == GRT_NOT_READY:
+            return None
+        self._keep.pop(ticket, None)
+        check(rc)
+        return self._res(r)
+
+    def close(self):
+        if getattr(self, "h", None):
+            h, self.h = self.h, None
+            check(_call("grt_server_destroy", h))
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

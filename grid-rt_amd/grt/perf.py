@@ -105,7 +105,30 @@ def serve_vs_plain(ctx: Context, scenes, n_requests: int = 120,
 
     top = next((p for p in sweep if p["load_vs_plain_capacity"] == 1.0), sweep[-1])
     slo = {"serve": max_load("serve"), "plain": max_load("plain")}
+    # The floor under the p99 at that load: the p99 of the same request
+    # stream with every request served alone on an idle GPU.
+    ctx.set_trace_mode(mode_of[True], modes[mode_of[True]])
+    stream = _requests(scenes, W.serve_requests(n_open, top["arrival_rate_hz"], len(scenes)))
+    alone = []
+    for req in stream:
+        arr, st, fin, _, _ = serve_run(ctx, [req[:3] + (0.0,)], True, 1 << 30, max_active)
+        alone.append(float((fin[0] - arr[0]) * 1e3))
+    floor = {"p99_ms": round(float(np.percentile(alone, 99)), 3),
+             "p50_ms": round(float(np.median(alone)), 3),
+             "max_ms": round(max(alone), 3)}
+    need = top["plain"]["p99_ms"] / 10.0
+    reason = None
+    if top["p99_ratio"] < 10.0:
+        reason = (f"at load {top['load_vs_plain_capacity']:g} the plain path's p99 is "
+                  f"{top['plain']['p99_ms']:.2f} ms, so 10x needs a serving p99 of "
+                  f"{need:.2f} ms; the serving p99 there is {top['serve']['p99_ms']:.2f} ms, "
+                  f"and the same {n_open} requests served one at a time on an idle GPU "
+                  f"already have a p99 of {floor['p99_ms']:.2f} ms (max "
+                  f"{floor['max_ms']:.2f} ms): the largest requests' own service time, "
+                  f"not queueing, sets the serving p99")
     return {
+        "same_requests_alone": floor,
+        "p99_headroom_reason": reason,
         "plain_capacity_req_s": round(capacity, 2),
         "schedule": {"serve": mode_of[True], "plain": mode_of[False]},
         "saturation": {"serve": serve_sat, "plain": plain_sat},

@@ -482,12 +482,36 @@ typedef struct {
     float offset;            /* secondary-ray origin offset along the normal */
 } grt_serve_config;
 /* Runs the requests in real time (arrivals at their arrival_s), blocking until
- * all complete.  Batched: each round traces the current step of every admitted
- * request in one batched launch; arrivals are admitted at round boundaries,
- * FIFO, while a slot and budget memory are free.  Buffers come from one slab of
- * memory_budget bytes, so nothing is allocated inside the loop. */
+ * all complete.  Batched: each round traces the next step of every admitted
+ * request in one batched launch; up to two rounds are queued on the stream at
+ * once (a request's next step waits for its previous one by stream order, not
+ * by a host barrier); arrivals are admitted FIFO, while a slot and budget
+ * memory are free, into the next round issued.  Plain (batched = 0): one
+ * request at a time, one launch per ray batch, the host waiting for every step.
+ * Buffers come from one slab of memory_budget bytes, so nothing is allocated
+ * inside the loop. */
 int grt_serve_run(grt_context ctx, const grt_request* requests, uint32_t n,
                   const grt_serve_config* cfg, grt_request_result* out, double* wall_s);
+/* The same loop behind a thread-safe interface, for callers that submit
+ * requests as they come (no counterpart in the reference, which rendered one
+ * frame per call).  grt_server_create starts a thread that owns the device
+ * loop on ctx's stream: the context is reserved to the server until
+ * grt_server_destroy, and the requests' scenes and output buffers must stay
+ * valid (and their grids unchanged) until their results are taken.
+ * grt_server_submit (any thread) checks and queues a request (arrival_s is
+ * ignored) and returns its ticket at once; the request joins the next round
+ * once a slot and budget memory are free.  grt_server_wait blocks until the
+ * ticket's request is complete on the device and returns its result (times in
+ * seconds after grt_server_create); grt_server_query returns GRT_NOT_READY
+ * while it runs.  A result is taken once.  grt_server_destroy completes every
+ * submitted request, then stops the thread.  A device error stops the loop and
+ * fails every open ticket with its code. */
+typedef struct grt_server_t* grt_server;
+int grt_server_create(grt_context ctx, const grt_serve_config* cfg, grt_server* out);
+int grt_server_submit(grt_server server, const grt_request* request, uint64_t* ticket);
+int grt_server_wait(grt_server server, uint64_t ticket, grt_request_result* out);
+int grt_server_query(grt_server server, uint64_t ticket, grt_request_result* out);
+int grt_server_destroy(grt_server server);
 
 /* ---- launch graphs ---------------------------------------------------- */
 /* Capture every launch enqueued on the context's stream between begin and end

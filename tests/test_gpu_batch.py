@@ -236,3 +236,77 @@ def test_serve_budget_admission(gpu_ctx, three_scenes):
     with pytest.raises(grt.GrtError):   # a request larger than the whole budget
         grt.serve_run(gpu_ctx, [(three_scenes[0][0], W.camera(1024, 1024), 1, 0.0)],
                       batched=True, memory_budget=1 << 20)
+
+
+def test_server_concurrent_submit_same_as_alone(gpu_ctx, three_scenes):
+    """grt_server: four host threads submit 16 requests of mixed size and depth
+    at once to one server (6 request slots, so admission waits for slots) and
+    each waits on its own tickets.  Every request's primary hits, shadow flags
+    and active-ray count equal the same request served alone on the plain
+    path; a polled ticket reports "running" (None) until done; an unknown
+    ticket is an error; after close() the context serves normally again."""
+    import threading
+    import torch
+    specs = [(k % 3, 64 + 16 * (k % 5), 48 + 16 * (k % 4), 1 + k % 3) for k in range(16)]
+
+    def cam_of(si, w, h):
+        if si != 2:
+            return W.camera(w, h)
+        return grt.camera_lookat((0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0, 1, 0), 60.0, w, h)
+
+    bufs = []
+    for si, w, h, d in specs:
+        bufs.append((grt.HitBatch.empty(w * h),
+                     torch.full((w * h,), 5, dtype=torch.uint8, device="cuda:0"),
+                     cam_of(si, w, h)))
+    torch.cuda.synchronize()
+    results, errors = [None] * len(specs), []
+    srv = grt.Server(gpu_ctx, memory_budget=64 << 20, max_active=6)
+    try:
+        def worker(ks):
+            try:
+                mine = [(k, srv.submit(three_scenes[specs[k][0]][0], bufs[k][2], specs[k][3],
+                                       bufs[k][0], bufs[k][1])) for k in ks]
+                for k, t in mine:
+                    results[k] = srv.wait(t)
+            except Exception as e:   # reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(range(i, len(specs), 4),))
+                   for i in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=60)
+        assert not any(th.is_alive() for th in threads), "a submitting thread hung"
+        assert not errors, errors
+        # polling: one more request, queried until complete
+        hb, occ = grt.HitBatch.empty(64 * 64), torch.empty(64 * 64, dtype=torch.uint8,
+                                                           device="cuda:0")
+        t = srv.submit(three_scenes[0][0], W.camera(64, 64), 2, hb, occ)
+        polled = None
+        for _ in range(200000):
+            polled = srv.query(t)
+            if polled is not None:
+                break
+        assert polled is not None and polled["rays"] >= 64 * 64
+        with pytest.raises(grt.GrtError):
+            srv.query(t)           # taken once
+        with pytest.raises(grt.GrtError):
+            srv.query(10 ** 12)    # never issued
+    finally:
+        srv.close()
+    for k, (si, w, h, d) in enumerate(specs):
+        r = results[k]
+        assert r is not None and r["finish_s"] >= r["start_s"] >= 0.0, (k, r)
+        hb, occ = grt.HitBatch.empty(w * h), torch.full((w * h,), 5, dtype=torch.uint8,
+                                                        device="cuda:0")
+        torch.cuda.synchronize()
+        res = grt.serve_run(gpu_ctx, [(three_scenes[si][0], bufs[k][2], d, 0.0, (hb, occ))],
+                            batched=False, memory_budget=64 << 20)
+        torch.cuda.synchronize()
+        assert int(res[3][0]) == r["rays"], (k, int(res[3][0]), r["rays"])
+        a, b = bufs[k][0].numpy(), hb.numpy()
+        for f in ("id", "t", "u", "v"):
+            assert_same(a[f], b[f], f"request {k} primary hit.{f} (server vs alone)")
+        assert_same(bufs[k][1].cpu().numpy(), occ.cpu().numpy(), f"request {k} shadow")
