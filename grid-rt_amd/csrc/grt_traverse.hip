@@ -1347,6 +1347,68 @@ __device__ __forceinline__ void stage_sub_tab(const TraceArgs& a, float* tab)
     __syncthreads();
 }
 
+// DDA state inside a top cell's sub-grid: current sub-cell (and its index si
+// in the sub-cell arrays), next boundary crossings and their increments.
+struct Sub2 {
+    uint32_t base, si;
+    int32_t r0, r1, qx, qy, qz, ox, oy, oz;
+    float tx, ty, tz, dtx, dty, dtz;
+};
+
+// Entry into the sub-grid of the top cell at s (record rec) at t_enter.
+__device__ __forceinline__ void sub_setup(const DevFrame* __restrict__ fr, const float* tab,
+                                          const Ray2& r, const Dda& s, uint2 rec, float t_enter,
+                                          Sub2& q)
+{
+    q.base = rec.x & 0x7fffffffu;
+    q.r0 = (int32_t)(rec.y & 255u);
+    q.r1 = (int32_t)((rec.y >> 8) & 255u);
+    const int32_t r2 = (int32_t)((rec.y >> 16) & 255u);
+    const float sb0 = (float)s.cx * fr->cell[0] + fr->bmin[0];
+    const float sb1 = (float)s.cy * fr->cell[1] + fr->bmin[1];
+    const float sb2 = (float)s.cz * fr->cell[2] + fr->bmin[2];
+    const float sc0 = tab[q.r0 - 1], sc1 = tab[16 + q.r1 - 1], sc2 = tab[32 + r2 - 1];
+    const float si0 = tab[48 + q.r0 - 1], si1 = tab[64 + q.r1 - 1], si2 = tab[80 + r2 - 1];
+    q.qx = sub_vox(r.o0 + r.d0 * t_enter, sb0, si0, q.r0);
+    q.qy = sub_vox(r.o1 + r.d1 * t_enter, sb1, si1, q.r1);
+    q.qz = sub_vox(r.o2 + r.d2 * t_enter, sb2, si2, r2);
+    q.ox = r.n0 ? -1 : q.r0;
+    q.oy = r.n1 ? -1 : q.r1;
+    q.oz = r.n2 ? -1 : r2;
+    q.tx = ((float)(q.qx + (r.n0 ? 0 : 1)) * sc0 + sb0 - r.o0) * r.i0;
+    q.ty = ((float)(q.qy + (r.n1 ? 0 : 1)) * sc1 + sb1 - r.o1) * r.i1;
+    q.tz = ((float)(q.qz + (r.n2 ? 0 : 1)) * sc2 + sb2 - r.o2) * r.i2;
+    q.dtx = fabsf(sc0 * r.i0);
+    q.dty = fabsf(sc1 * r.i1);
+    q.dtz = fabsf(sc2 * r.i2);
+    q.si = q.base + (uint32_t)q.qx +
+           (uint32_t)q.r0 * ((uint32_t)q.qy + (uint32_t)q.r1 * (uint32_t)q.qz);
+}
+
+// One sub-grid step (the top DDA's tie order); false when it leaves the grid.
+__device__ __forceinline__ bool sub_step(const Ray2& r, Sub2& q)
+{
+    if (q.tx < q.ty && q.tx < q.tz) {
+        q.qx += r.n0 ? -1 : 1;
+        if (q.qx == q.ox)
+            return false;
+        q.tx += q.dtx;
+    } else if (q.ty < q.tz) {
+        q.qy += r.n1 ? -1 : 1;
+        if (q.qy == q.oy)
+            return false;
+        q.ty += q.dty;
+    } else {
+        q.qz += r.n2 ? -1 : 1;
+        if (q.qz == q.oz)
+            return false;
+        q.tz += q.dtz;
+    }
+    q.si = q.base + (uint32_t)q.qx +
+           (uint32_t)q.r0 * ((uint32_t)q.qy + (uint32_t)q.r1 * (uint32_t)q.qz);
+    return true;
+}
+
 // One top cell (index ci, coordinates of s) of the two-level walk, entered at
 // t_enter and left at exit_t: its sub-grid from t_enter when it has one and
 // the second level is valid, else its own references bounded by exit_t.
@@ -1374,29 +1436,12 @@ __device__ __forceinline__ bool top_cell2(const TraceArgs& a, const DevFrame* __
         }
         return false;
     }
-    const uint32_t sub_base = rec.x & 0x7fffffffu;
-    const int32_t r0 = (int32_t)(rec.y & 255u), r1 = (int32_t)((rec.y >> 8) & 255u),
-                  r2 = (int32_t)((rec.y >> 16) & 255u);
-    const float sb0 = (float)s.cx * fr->cell[0] + fr->bmin[0];
-    const float sb1 = (float)s.cy * fr->cell[1] + fr->bmin[1];
-    const float sb2 = (float)s.cz * fr->cell[2] + fr->bmin[2];
-    const float sc0 = tab[r0 - 1], sc1 = tab[16 + r1 - 1], sc2 = tab[32 + r2 - 1];
-    const float si0 = tab[48 + r0 - 1], si1 = tab[64 + r1 - 1], si2 = tab[80 + r2 - 1];
-    int32_t qx = sub_vox(r.o0 + r.d0 * t_enter, sb0, si0, r0);
-    int32_t qy = sub_vox(r.o1 + r.d1 * t_enter, sb1, si1, r1);
-    int32_t qz = sub_vox(r.o2 + r.d2 * t_enter, sb2, si2, r2);
-    const int32_t sx = r.n0 ? -1 : 1, sy = r.n1 ? -1 : 1, sz = r.n2 ? -1 : 1;
-    const int32_t ox = r.n0 ? -1 : r0, oy = r.n1 ? -1 : r1, oz = r.n2 ? -1 : r2;
-    float tx = ((float)(qx + (r.n0 ? 0 : 1)) * sc0 + sb0 - r.o0) * r.i0;
-    float ty = ((float)(qy + (r.n1 ? 0 : 1)) * sc1 + sb1 - r.o1) * r.i1;
-    float tz = ((float)(qz + (r.n2 ? 0 : 1)) * sc2 + sb2 - r.o2) * r.i2;
-    const float dtx = fabsf(sc0 * r.i0), dty = fabsf(sc1 * r.i1), dtz = fabsf(sc2 * r.i2);
+    Sub2 q;
+    sub_setup(fr, tab, r, s, rec, t_enter, q);
     for (;;) {
         ++cells;
-        const float sexit = tmin_f(tmin_f(tx, ty), tz);
-        const uint32_t si =
-            sub_base + (uint32_t)qx + (uint32_t)r0 * ((uint32_t)qy + (uint32_t)r1 * (uint32_t)qz);
-        const uint32_t kb = a.sub_offsets[si], ke = a.sub_offsets[si + 1];
+        const float sexit = tmin_f(tmin_f(q.tx, q.ty), q.tz);
+        const uint32_t kb = a.sub_offsets[q.si], ke = a.sub_offsets[q.si + 1];
         if (trace2_cell<ANY>(a.sub_trir, kb, ke, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
                              r.tmax, sexit * kExitScale, id, best, bu, bv, tests)) {
             occ = ANY;
@@ -1404,22 +1449,8 @@ __device__ __forceinline__ bool top_cell2(const TraceArgs& a, const DevFrame* __
         }
         if (sexit > r.tmax)
             return true;
-        if (tx < ty && tx < tz) {
-            qx += sx;
-            if (qx == ox)
-                return false;
-            tx += dtx;
-        } else if (ty < tz) {
-            qy += sy;
-            if (qy == oy)
-                return false;
-            ty += dty;
-        } else {
-            qz += sz;
-            if (qz == oz)
-                return false;
-            tz += dtz;
-        }
+        if (!sub_step(r, q))
+            return false;
     }
 }
 

@@ -255,17 +255,17 @@ TWO_LEVEL = [(8.0, 4.0, 8), (8.0, 8.0, 8), (16.0, 4.0, 8), (4.0, 4.0, 8)]
 
 def _two_level_stage(ctx, sc, rays, fr, offs, refs, g2, ov, closest, tag, lst, count):
     """One stage through the two-level grid: whole image one ray per lane
-    (simple, persistent: the same kernel) and 4 and 16 lanes per ray (split),
-    and as compacted lists, all equal to the oracle's two-level traversal;
-    returns the oracle's result."""
+    (simple, pooled, persistent: the same kernel), 4 and 16 lanes per ray
+    (split), and as compacted lists, all equal to the oracle's two-level
+    traversal; returns the oracle's result."""
     n = rays.n
     o = rays.numpy()
     active = o["tmin"] <= o["tmax"]
     fn = orc.grid2_trace_closest if closest else orc.grid2_trace_any
     ores = fn(fr, offs, refs, g2, ov, o, counters=True)
     for mode, param, listed in (("simple", 16, False), ("persistent", 16, False),
-                                ("split", 4, False), ("split", 16, False), ("simple", 16, True),
-                                ("split", 8, True)):
+                                ("pooled", 16, False), ("split", 4, False), ("split", 16, False),
+                                ("simple", 16, True), ("pooled", 16, True), ("split", 8, True)):
         ctx.set_trace_mode(mode, param)
         cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
         t = f"{tag} [{mode} {param}{' listed' if listed else ''}]"
