@@ -229,6 +229,8 @@ static G2Build g2_args(grt_scene_t* sc, bool with_outputs)
     b.box = sc->g2box.as<uint32_t>();
     b.big = sc->g2big.as<uint32_t>();
     b.big_cap = sc->g2_big_cap;
+    b.longl = sc->g2long.as<uint32_t>();
+    b.long_cap = sc->g2_long_cap;
     b.sub_tab = sc->g2tab.as<float>();
     return b;
 }
@@ -293,6 +295,11 @@ static int g2_reserve_sub(grt_scene_t* sc, uint32_t cells, uint32_t refs)
         if ((rc = sc->g2big.reserve(sc->ctx, bc * 4, false)) != GRT_OK)
             return rc;
         sc->g2_big_cap = (uint32_t)bc;
+        // sub-cells of 17..1024 references: at most refs / 17 of them
+        const uint64_t lc = rc64 / 17 + 1;
+        if ((rc = sc->g2long.reserve(sc->ctx, lc * 4, false)) != GRT_OK)
+            return rc;
+        sc->g2_long_cap = (uint32_t)lc;
         sc->g2_refs_cap = (uint32_t)rc64;
     }
     return GRT_OK;
@@ -306,7 +313,7 @@ static int g2_enqueue(grt_scene_t* sc)
     if (!(sc->g2_lambda > 0.0f))
         return GRT_OK;
     if (sc->g2_top_cap < sc->cells_cap || sc->g2_box_cap < sc->refs_cap || !sc->sub_offsets.ptr ||
-        !sc->sub_refs.ptr || !sc->g2dense.ptr || !sc->g2big.ptr)
+        !sc->sub_refs.ptr || !sc->g2dense.ptr || !sc->g2big.ptr || !sc->g2long.ptr)
         return GRT_OK;
     const G2Build b = g2_args(sc, true);
     const int rc = hip_check(launch_grid2_build(b, sc->cells_cap, sc->g2_cells_cap,

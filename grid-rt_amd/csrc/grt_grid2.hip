@@ -18,13 +18,16 @@
 // thread: the count pass adds to LDS counters over the reference's sub-cell
 // box (and keeps the box, 4 bits per bound, at the reference's position), the
 // fill pass scatters through LDS cursors, and every sub-cell's ids are then
-// sorted ascending -- by a thread, a wave or, past 1024 ids, a block.
+// sorted ascending over the whole level -- by a thread, a wave or, past 1024
+// ids, a block.
 //   k_g2_plan<false>  per tile: sub-cells and dense cells (tile sums)
 //   k_g2_scan_tiles   exclusive scan of both; totals, capacity
 //   k_g2_plan<true>   per tile: cell records, the ordered dense-cell list
 //   k_g2_count        per dense cell: boxes, sub-cell counts, sub-references
 //   k_g2_scan_dense   exclusive scan of those; total, capacity
-//   k_g2_fill         per dense cell: sub-cell offsets, references, sorts
+//   k_g2_fill         per dense cell: sub-cell offsets, references
+//   k_g2_sort_thread  per sub-cell: ids ascending (2..16), longer ones listed
+//   k_g2_sort_wave    sub-cells of 17..1024 references, a wave each
 //   k_g2_sort_big     sub-cells of more than 1024 references
 //   k_gather_refs     48-byte record of every sub-reference
 #include "grt_device.h"
@@ -170,6 +173,7 @@ __global__ __launch_bounds__(1024) void k_g2_scan_tiles(G2Build b, uint32_t n_ti
             g2->r = 0;
             g2->n[0] = 0;
             g2->n[1] = 0;
+            g2->n[2] = 0;
             g2->status = kStatusFrameInvalid;
         }
         return;
@@ -205,6 +209,7 @@ __global__ __launch_bounds__(1024) void k_g2_scan_tiles(G2Build b, uint32_t n_ti
         g2->m = (uint32_t)(cm > 0xffffffffull ? 0xffffffffull : cm);
         g2->n[0] = (int32_t)(cd > 0x7fffffffull ? 0x7fffffffull : cd);
         g2->n[1] = 0;
+        g2->n[2] = 0;
         g2->r = 0;
         g2->status = status;
     }
@@ -303,23 +308,17 @@ __global__ __launch_bounds__(1024) void k_g2_count(G2Build b)
 }
 
 // 6: per dense cell, the sub-cell offsets (scan of the counts left in
-// sub_offsets), the references scattered through LDS cursors, then every
-// sub-cell's ids sorted ascending: 2..16 by one thread, 17..1024 by one
-// wave in registers; longer lists go to b.big for k_g2_sort_big.
+// sub_offsets) and the references scattered through LDS cursors.
 constexpr uint32_t kG2WaveSortMax = 1024;
 
 __global__ __launch_bounds__(1024) void k_g2_fill(G2Build b)
 {
     constexpr uint32_t kM = kSubMax * kSubMax * kSubMax;
-    __shared__ uint32_t s_off[kM + 1];
     __shared__ uint32_t s_cur[kM];
-    __shared__ uint32_t s_long[1024];
-    __shared__ uint32_t s_nlong;
     __shared__ uint32_t s_scan[17];
     if (!top_ok(b.fr) || b.g2->status != 0)
         return;
     const uint32_t nd = (uint32_t)b.g2->n[0];
-    const uint32_t wave = threadIdx.x >> 6;
     for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
         const uint32_t c = b.dense[d];
         G2Cell g;
@@ -339,14 +338,12 @@ __global__ __launch_bounds__(1024) void k_g2_fill(G2Build b)
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
             if (s0 + i < g.m) {
-                s_off[s0 + i] = o;
                 s_cur[s0 + i] = o;
                 out[s0 + i] = o;
             }
             o += v[i];
         }
-        if (threadIdx.x == 0)
-            s_off[g.m] = b.dense_refs[d] + cell_total;
+        (void)cell_total;
         __syncthreads();
         const uint32_t r0 = g.r[0], r01 = g.r[0] * g.r[1];
         for (uint32_t k = g.kb + threadIdx.x; k < g.ke; k += blockDim.x) {
@@ -361,45 +358,79 @@ __global__ __launch_bounds__(1024) void k_g2_fill(G2Build b)
                         b.sub_refs[atomicAdd(s_cur + z * r01 + y * r0 + x, 1u)] = t;
         }
         __syncthreads();
-        // ascending ids per sub-cell, in groups of blockDim sub-cells so the
-        // LDS list of longer ones never overflows
-        for (uint32_t g0 = 0; g0 < g.m; g0 += blockDim.x) {
-            if (threadIdx.x == 0)
-                s_nlong = 0u;
-            __syncthreads();
-            const uint32_t s = g0 + threadIdx.x;
-            if (s < g.m) {
-                const uint32_t n = s_off[s + 1] - s_off[s];
-                uint32_t* seg = b.sub_refs + s_off[s];
-                if (n >= 2u && n <= 8u)
-                    sort_cell_thread<8>(seg, n);
-                else if (n > 8u && n <= 16u)
-                    sort_cell_thread<16>(seg, n);
-                else if (n > 16u && n <= kG2WaveSortMax)
-                    s_long[atomicAdd(&s_nlong, 1u)] = s;
-                else if (n > kG2WaveSortMax)
-                    b.big[atomicAdd(reinterpret_cast<uint32_t*>(&b.g2->n[1]), 1u)] = sub_base + s;
-            }
-            __syncthreads();
-            const uint32_t nl = s_nlong;
-            for (uint32_t li = wave; li < nl; li += blockDim.x >> 6) {
-                const uint32_t sl = s_long[li];
-                const uint32_t n = s_off[sl + 1] - s_off[sl];
-                uint32_t* seg = b.sub_refs + s_off[sl];
-                if (n <= 64u)
-                    wave_sort<1>(seg, n);
-                else if (n <= 256u)
-                    wave_sort<4>(seg, n);
-                else
-                    wave_sort<16>(seg, n);
-            }
-            __syncthreads();
-        }
-        __syncthreads();
     }
 }
 
-// 7: sub-cells of more than kG2WaveSortMax references (b.big), one block each.
+// 7: every sub-cell's ids ascending (the scatter's atomics leave them in any
+// order), one thread per sub-cell over the whole second level: 2..16 sorted in
+// registers; 17..kG2WaveSortMax queued on b.longl for k_g2_sort_wave, longer
+// ones on b.big for k_g2_sort_big.  The loop runs over whole blocks so every
+// lane takes part in the ballots.
+__global__ __launch_bounds__(256) void k_g2_sort_thread(G2Build b)
+{
+    if (!top_ok(b.fr) || b.g2->status != 0)
+        return;
+    const uint32_t m = b.g2->m;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t s0 = blockIdx.x * blockDim.x; s0 < m; s0 += stride) {
+        const uint32_t s = s0 + threadIdx.x;
+        uint32_t n = 0, o = 0;
+        if (s < m) {
+            o = b.sub_offsets[s];
+            n = b.sub_offsets[s + 1] - o;
+        }
+        if (n >= 2u && n <= 8u)
+            sort_cell_thread<8>(b.sub_refs + o, n);
+        else if (n > 8u && n <= 16u)
+            sort_cell_thread<16>(b.sub_refs + o, n);
+        const bool is_long = n > 16u && n <= kG2WaveSortMax;
+        const bool is_big = n > kG2WaveSortMax;
+        const uint64_t ml = __ballot(is_long), mb = __ballot(is_big);
+        if (ml) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(ml);
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(reinterpret_cast<uint32_t*>(&b.g2->n[2]), (uint32_t)__popcll(ml));
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
+            if (is_long)
+                b.longl[base + lanes_below(ml)] = s;
+        }
+        if (mb) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(mb);
+            uint32_t base = 0;
+            if (lane == leader)
+                base = atomicAdd(reinterpret_cast<uint32_t*>(&b.g2->n[1]), (uint32_t)__popcll(mb));
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
+            if (is_big)
+                b.big[base + lanes_below(mb)] = s;
+        }
+    }
+}
+
+// 8: sub-cells of 17..kG2WaveSortMax references (b.longl), one wave each, in
+// registers.
+__global__ __launch_bounds__(256) void k_g2_sort_wave(G2Build b)
+{
+    if (!top_ok(b.fr) || b.g2->status != 0)
+        return;
+    const uint32_t nl = (uint32_t)b.g2->n[2];
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t li = wave; li < nl; li += n_waves) {
+        const uint32_t sc = b.longl[li];
+        const uint32_t o = b.sub_offsets[sc];
+        const uint32_t n = b.sub_offsets[sc + 1] - o;
+        if (n <= 64u)
+            wave_sort<1>(b.sub_refs + o, n);
+        else if (n <= 256u)
+            wave_sort<4>(b.sub_refs + o, n);
+        else
+            wave_sort<16>(b.sub_refs + o, n);
+    }
+}
+
+// 9: sub-cells of more than kG2WaveSortMax references (b.big), one block each.
 __global__ __launch_bounds__(256) void k_g2_sort_big(G2Build b)
 {
     __shared__ uint32_t lds[kSortTile];
@@ -464,6 +495,10 @@ hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t
         return e0;
     const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
     hipLaunchKernelGGL(k_g2_fill, dim3(nb), dim3(1024), 0, s, b);
+    const uint32_t nt = sub_cells_cap / 256u + 1u;
+    hipLaunchKernelGGL(k_g2_sort_thread, dim3(nt < 4096u ? nt : 4096u), dim3(256), 0, s, b);
+    const uint32_t nl = b.long_cap / 4u + 1u;
+    hipLaunchKernelGGL(k_g2_sort_wave, dim3(nl < 2048u ? nl : 2048u), dim3(256), 0, s, b);
     const uint32_t nbig = b.big_cap < 256u ? (b.big_cap ? b.big_cap : 1u) : 256u;
     hipLaunchKernelGGL(k_g2_sort_big, dim3(nbig), dim3(256), 0, s, b);
     hipError_t e = hipGetLastError();

@@ -104,6 +104,8 @@ struct G2Build {
     uint32_t* box;              // per top reference: its sub-cell box (dense cells only)
     uint32_t* big;              // sub-cells of more than 1024 references [big_cap]
     uint32_t big_cap;
+    uint32_t* longl;            // sub-cells of 17..1024 references [long_cap]
+    uint32_t long_cap;
     float* sub_tab;             // [96]: cell_a / r at 16 a + r - 1, 1 / that at 48 + 16 a + r - 1
 };
 // Plan, dense list and per-cell counts: g2->m, g2->n[0], g2->r receive the
