@@ -43,7 +43,7 @@ LAMBDA = 5.0
 LAMBDAS = (8.0, 32.0, 64.0, 128.0, 256.0, 512.0)
 # Two-level grids: (top lambda, sub-grid lambda, minimum references of a cell
 # that gets a sub-grid).
-TWO_LEVEL = ((8.0, 4.0, 8), (8.0, 8.0, 8), (16.0, 4.0, 8), (4.0, 4.0, 8))
+TWO_LEVEL = ((8.0, 4.0, 8), (16.0, 4.0, 8))
 LIGHT = (1000.0, 1000.0, 1000.0)
 OFFSET = 1e-4
 EYE, CENTER, UP = (0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0.0, 1.0, 0.0)
@@ -557,8 +557,9 @@ def main():
     achieved = dom_bytes / (trace_ms[dom_stage] * 1e-3) / 1e9
     # DRAM bytes of that launch at this lambda, from the committed
     # FETCH_SIZE / WRITE_SIZE profile (null if it has no such line).
-    traffic_of = {name: profile_traffic(tag, sched, frame.lists_kind, name,
-                                        kernel_name(closest, sched[i], sub is not None))
+    traffic_of = {name: profile_traffic(tag, name,
+                                        kernel_name(closest, sched[i], sub is not None),
+                                        "-" if i == 0 else frame.lists_kind[i - 1])
                   for i, (name, closest) in enumerate(STAGES)}
     traffic = traffic_of[dom_stage]
     per_stage_roof = {}
@@ -633,6 +634,33 @@ def main():
                      "avg_launch_ms": round(trace_ms[dom_stage], 3),
                      "algorithmic_bytes_per_launch": dom_bytes},
     }
+    if sub is not None:
+        # The same roofline on the fastest one-level grid of the tuning table:
+        # the reference's structure, more bytes per ray, a longer frame.
+        one = {k: e for k, e in tune_table.items() if e.get("sub") is None}
+        if one:
+            tag1 = min(one, key=lambda k: one[k].get("frame_ms_final", one[k]["frame_ms"]))
+            e1 = one[tag1]
+            frame.set_grid(e1["lam"], None)
+            frame.sched, frame.lists_kind = e1["sched"], e1["lists"]
+            ms1 = frame.time_stages(frames=6)
+            c1 = stage_counters(frame, torch)
+            st1 = {"primary": ms1[2], "shadow": ms1[3]}
+            b1 = c1["primary"]["algorithmic_bytes"] + c1["shadow"]["algorithmic_bytes"]
+            gbs1 = b1 / ((st1["primary"] + st1["shadow"]) * 1e-3) / 1e9
+            out["config"]["one_level_comparison"] = {
+                "grid": tag1, "schedule_per_stage": e1["sched"], "lists": e1["lists"],
+                "frame_ms": round(sum(ms1), 3),
+                "stage_ms": {n: round(v, 3) for n, v in zip(frame.stage_names, ms1)},
+                "primary_shadow_algorithmic_bytes": b1,
+                "roofline_primary_shadow": {"gbs": round(gbs1, 1),
+                                            "frac": round(gbs1 / HBM_PEAK_GBS, 4)},
+                "mean_cells_tests": {n: [c1[n]["mean_cells"], c1[n]["mean_tests"]]
+                                     for n in c1},
+                "note": "stage times include ray generation and list compaction, as in "
+                        "the tuning table"}
+            frame.set_grid(lam, sub)
+            frame.sched, frame.lists_kind = sched, lists
     if not args.no_serve:
         # p50/p99 request latency, continuous batching vs the plain path, on
         # the serving workload (16 resident sphere scenes, mixed requests).
@@ -660,15 +688,14 @@ def grid_desc(lam, sub):
             f">= {sub[1]} references")
 
 
-def profile_traffic(tag, sched, lists, stage, kernel):
+def profile_traffic(tag, stage, kernel, listed):
     """HBM bytes per launch of a stage's traversal kernel on this grid (its
-    grid_tag), schedule and ray-list choice, from the committed FETCH_SIZE /
-    WRITE_SIZE profile (None when the profile has no such line)."""
+    grid_tag), whole image ('-') or compacted list ('c'), from the committed
+    FETCH_SIZE / WRITE_SIZE profile (None when the profile has no such line)."""
     tpath = os.path.join(ROOT, "profiles", "trace_stadium_traffic.md")
     if not os.path.exists(tpath):
         return None
-    key = "hbm_bytes_per_launch[grid=%s][sched=%s][lists=%s][%s][%s]:" % (
-        tag, sched, lists, stage, kernel)
+    key = "hbm_bytes_per_launch[grid=%s][%s][%s][list=%s]:" % (tag, stage, kernel, listed)
     for line in open(tpath):
         if line.startswith(key):
             return float(line.split(":", 1)[1])

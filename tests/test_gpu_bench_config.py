@@ -248,9 +248,9 @@ def test_bench_frame_four_stages(gpu_ctx, stadium, lam):
         sc.close()
 
 
-# Two-level grids the frame autotuner chooses among: (top lambda, lambda2,
-# minimum references of a subdivided cell).
-TWO_LEVEL = [(8.0, 4.0, 8), (8.0, 8.0, 8), (16.0, 4.0, 8), (4.0, 4.0, 8)]
+# Two-level grids (top lambda, lambda2, minimum references of a subdivided
+# cell): the two the frame autotuner chooses among and two neighbours.
+TWO_LEVEL = [(8.0, 4.0, 8), (16.0, 4.0, 8), (8.0, 8.0, 8), (4.0, 4.0, 8)]
 
 
 def _two_level_stage(ctx, sc, rays, fr, offs, refs, g2, ov, closest, tag, lst, count):
@@ -297,13 +297,14 @@ def _two_level_stage(ctx, sc, rays, fr, offs, refs, g2, ov, closest, tag, lst, c
 
 @pytest.mark.parametrize("lam,lam2,min_refs", TWO_LEVEL)
 def test_bench_two_level_frame(gpu_ctx, stadium, lam, lam2, min_refs):
-    """The frame on each two-level grid the autotuner may pick, set up as the
-    bench does (frame 0 built synchronously with headroom, then a per-frame
-    transform + asynchronous rebuild of both levels): both levels equal the
-    oracle's, the four stages equal the oracle's two-level traversal bit for
-    bit (hits, flags, top cells + sub-cells, tests), and the closest hits and
-    occlusion equal the one-level oracle's (the reference semantics) except on
-    rays whose two nearest hits lie within the exit bound's 2^-20 slack."""
+    """The frame on the two-level grids the autotuner picks from (and two
+    neighbours), set up as the bench does (frame 0 built synchronously with
+    headroom, then a per-frame transform + asynchronous rebuild of both
+    levels): both levels equal the oracle's, the four stages equal the
+    oracle's two-level traversal bit for bit (hits, flags, top cells +
+    sub-cells, tests), and the closest hits and occlusion equal the one-level
+    oracle's (the reference semantics) except on rays whose two nearest hits
+    lie within the exit bound's 2^-20 slack."""
     ctx = gpu_ctx
     v, nrm, _ = stadium
     sc = grt.Scene(ctx)
