@@ -786,14 +786,21 @@ int grt_grid_reserve(grt_scene sc, uint32_t cells, uint32_t refs)
 {
     if (!sc)
         return set_error(GRT_INVALID_PARAM, "grt_grid_reserve: NULL scene");
+    const void* before[4] = {sc->offsets.ptr, sc->refs.ptr, sc->trir.ptr, sc->cellrec.ptr};
     int rc;
-    if ((rc = grid_reserve_cells(sc, cells)) != GRT_OK)
-        return rc;
-    if ((rc = grid_reserve_refs(sc, refs)) != GRT_OK)
-        return rc;
-    if (sc->g2_lambda > 0.0f && (sc->g2_top_cap < sc->cells_cap || sc->g2_box_cap < sc->refs_cap))
-        return g2_reserve_top(sc);
-    return GRT_OK;
+    if ((rc = grid_reserve_cells(sc, cells)) == GRT_OK &&
+        (rc = grid_reserve_refs(sc, refs)) == GRT_OK && sc->g2_lambda > 0.0f &&
+        (sc->g2_top_cap < sc->cells_cap || sc->g2_box_cap < sc->refs_cap))
+        rc = g2_reserve_top(sc);
+    // A grown buffer is a new allocation without the old contents: the grid
+    // is gone until the next build (traces then report GRT_NOT_READY instead
+    // of reading uninitialised offsets).
+    const void* after[4] = {sc->offsets.ptr, sc->refs.ptr, sc->trir.ptr, sc->cellrec.ptr};
+    if (!std::equal(before, before + 4, after)) {
+        sc->grid_valid = false;
+        sc->g2_built = false;
+    }
+    return rc;
 }
 
 int grt_grid_rebuild_async(grt_scene sc, float lambda)

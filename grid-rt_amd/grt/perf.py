@@ -438,15 +438,17 @@ def ao_config(lams=(4.0, 8.0, 16.0), reps: int = 3, width: int = 1024, height: i
 # ---------------------------------------------------------------------------
 # kd-tree vs grid on the stadium frame
 # ---------------------------------------------------------------------------
-def kdtree_config(lam: float = 256.0, reps: int = 5, width: int = 1920, height: int = 1080):
+def kdtree_config(lam: float = 256.0, reps: int = 5, width: int = 1920, height: int = 1080,
+                  two_level=(8.0, 4.0, 8)):
     """The kd-tree comparison accelerator against the uniform grid on the
     stadium frame (2^20 triangles in 256 clusters + the ground plane, frame 0's
     rigid cluster transforms): the same four traversal stages -- primary,
     shadow, mirror bounce, bounce shadow, rays generated from the grid's hits --
-    timed per launch (HIP events, median of `reps`) through the grid (best of
-    one ray per lane, pooled tests and 2 / 16 lanes per ray) and through the
-    kd-tree, with the per-ray work of each (cells or leaves, triangle tests) and
-    both build times (the grid's device rebuild, the kd-tree's host build)."""
+    timed per launch (HIP events, median of `reps`) through the one-level grid
+    (best of one ray per lane, pooled tests and 2 / 16 lanes per ray), through
+    the kd-tree and through the two-level grid, with the per-ray work of each
+    (cells or leaves, triangle tests) and the build times (the grids' device
+    rebuilds, the kd-tree's host build)."""
     torch = _torch()
     ctx = Context(0)
     seed = W.SEED_STADIUM
@@ -457,6 +459,15 @@ def kdtree_config(lam: float = 256.0, reps: int = 5, width: int = 1920, height: 
     sc.build_grid(lam)
     info = sc.grid_info()
     sc.reserve_grid(int(info["m"] * 1.25) + 4096, int(info["r"] * 1.5) + 4096)
+    # the device rebuild into the reserved buffers (a grown reserve drops the
+    # grid until the next build)
+    ev_b = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    sc.rebuild_grid_async(lam)
+    ev_b[0].record()
+    sc.rebuild_grid_async(lam)
+    ev_b[1].record()
+    sc.check_grid()
+    rebuild_ms = round(ev_b[0].elapsed_time(ev_b[1]), 3)
     kd = KdTree(sc)
     kinfo = kd.info()
     n = width * height
@@ -493,12 +504,7 @@ def kdtree_config(lam: float = 256.0, reps: int = 5, width: int = 1920, height: 
            "kdtree": {k: kinfo[k] for k in ("nodes", "leaves", "empty_leaves", "elements",
                                             "depth")},
            "kdtree_host_build_ms": round(kinfo["build_ms"], 1), "stages": {}}
-    ev_b = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ev_b[0].record()
-    sc.rebuild_grid_async(lam)
-    ev_b[1].record()
-    torch.cuda.synchronize()
-    out["grid_device_rebuild_ms"] = round(ev_b[0].elapsed_time(ev_b[1]), 3)
+    out["grid_device_rebuild_ms"] = rebuild_ms
     tot = {"grid": 0.0, "kd": 0.0}
     n_rays = 0
     for name, (rays, closest) in stages.items():
@@ -538,6 +544,44 @@ def kdtree_config(lam: float = 256.0, reps: int = 5, width: int = 1920, height: 
             "kd_mrays_s": round(n_act / (kd_ms * 1e-3) / 1e6, 1),
             "grid_cells_tests_per_ray": [round(float(gc[0]), 2), round(float(gc[1]), 2)],
             "kd_leaves_tests_per_ray": [round(float(kc[0]), 2), round(float(kc[1]), 2)]}
+    # The same rays through the two-level grid (top lambda 8, sub-grids at
+    # lambda2 4 in cells of >= 8 references), one ray per lane or 8 / 16 lanes
+    # per ray, per stage the faster.
+    lam1, lam2, min_refs = two_level
+    sc.set_subgrid(lam2, min_refs)
+    sc.build_grid(lam1)
+    ev_b[0].record()
+    sc.rebuild_grid_async(lam1)
+    ev_b[1].record()
+    sc.check_grid()
+    out["two_level"] = {"grid": "%g/%g/%d" % two_level, "device_rebuild_ms":
+                        round(ev_b[0].elapsed_time(ev_b[1]), 3)}
+    tot["two_level"] = 0.0
+    for name, (rays, closest) in stages.items():
+        active = rays.f["tmin"] <= rays.f["tmax"]
+        g2_ms = {}
+        for mode, param in (("simple", 16), ("split", 8), ("split", 16)):
+            ctx.set_trace_mode(mode, param)
+            if closest:
+                g2_ms[f"{mode}{param if mode == 'split' else ''}"] = timed(
+                    lambda: sc.trace_closest(rays, out_h, image_width=width))
+            else:
+                g2_ms[f"{mode}{param if mode == 'split' else ''}"] = timed(
+                    lambda: sc.trace_any(rays, occ, image_width=width))
+        ctx.set_trace_mode("simple", 16)
+        if closest:
+            sc.trace_closest(rays, out_h, image_width=width, counters=cnt)
+        else:
+            sc.trace_any(rays, occ, image_width=width, counters=cnt)
+        c2 = cnt.view(-1, 2)[active].double().mean(0)
+        best = min(g2_ms, key=g2_ms.get)
+        tot["two_level"] += g2_ms[best]
+        e = out["stages"][name]
+        e["two_level_ms"] = round(g2_ms[best], 3)
+        e["two_level_schedule"] = best
+        e["two_level_mrays_s"] = round(e["rays"] / (g2_ms[best] * 1e-3) / 1e6, 1)
+        e["two_level_cells_tests_per_ray"] = [round(float(c2[0]), 2), round(float(c2[1]), 2)]
+    sc.set_subgrid(0.0, 8)
     out["traversal_ms"] = {k: round(x, 3) for k, x in tot.items()}
     out["traversal_mrays_s"] = {k: round(n_rays / (x * 1e-3) / 1e6, 1) for k, x in tot.items()}
     kd.close()

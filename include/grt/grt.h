@@ -216,6 +216,11 @@ int grt_grid_build(grt_scene scene, float lambda);
  * grt_grid_build / grt_grid_reserve.  Overflow is flagged on the device and
  * reported by grt_grid_check. */
 int grt_grid_rebuild_async(grt_scene scene, float lambda);
+/* Capacity for later asynchronous rebuilds: `cells` cells and `refs`
+ * references.  A buffer that has to grow is reallocated without its contents,
+ * so the grid is then gone (traces report GRT_NOT_READY) until the next
+ * grt_grid_build or grt_grid_rebuild_async, and batches holding the scene must
+ * be set again. */
 int grt_grid_reserve(grt_scene scene, uint32_t cells, uint32_t refs);
 /* Synchronise and report the device status of the last (re)build. */
 int grt_grid_check(grt_scene scene);

@@ -282,6 +282,49 @@ def test_rebuild_over_refs_then_build_without_check(gpu_ctx, c1_scene, then):
     assert_same(refs, orefs, f"refs after the overflow ({then})")
 
 
+@pytest.mark.parametrize("lam2", [0.0, 4.0])
+def test_growing_reserve_drops_the_grid_until_rebuilt(gpu_ctx, c1_scene, lam2):
+    """grt_grid_reserve that grows the buffers reallocates them without their
+    contents: a trace before the next build is refused with GRT_NOT_READY
+    (not run over uninitialised offsets), a reserve that fits leaves the grid
+    usable, and the rebuild into the grown buffers traces like the oracle --
+    one level, and two levels (lambda2 4)."""
+    v, n = c1_scene
+    sc = grt.Scene(gpu_ctx)
+    try:
+        sc.upload(v, n)
+        sc.set_subgrid(lam2, 8)
+        sc.build_grid(5.0)
+        cam = c1_camera(96, 64)
+        rays = orc.primary_rays(grt.camera_dict(cam))
+        rb, hb = grt.RayBatch.from_numpy(rays), grt.HitBatch.empty(cam.width * cam.height)
+        info = sc.grid_info()
+        sc.reserve_grid(info["m"], info["r"])        # fits: the grid stays
+        sc.trace_closest(rb, hb, image_width=cam.width)
+        sc.reserve_grid(4 * info["m"] + 65536, 4 * info["r"] + 65536)   # grows
+        for trace in (lambda: sc.trace_closest(rb, hb, image_width=cam.width),
+                      lambda: sc.trace_any(rb, hb.f["id"], image_width=cam.width)):
+            with pytest.raises(grt.GrtError) as e:
+                trace()
+            assert e.value.code == grt.GRT_NOT_READY
+        sc.rebuild_grid_async(5.0)
+        sc.check_grid()
+        sc.trace_closest(rb, hb, image_width=cam.width)
+        fr = orc.grid_frame(v, 5.0)
+        _, offs, refs = orc.grid_build(v, fr)
+        if lam2 > 0:
+            assert sc.subgrid_info()["built"]
+            g2 = orc.grid2_build(v, fr, offs, refs, 8, lam2, 16)
+            o = orc.grid2_trace_closest(fr, offs, refs, g2, v, rays)
+        else:
+            o = orc.trace_closest(fr, offs, refs, v, rays)
+        g = hb.numpy()
+        for k in ("id", "t", "u", "v"):
+            assert_same(g[k], o[k], f"hit.{k} after the grown reserve and rebuild")
+    finally:
+        sc.close()
+
+
 def test_grid_build_stadium_small(gpu_ctx):
     v, n, _ = grt.gen_stadium(SEED_C2, 16, 2048, 16 * 2048 - 2)
     check_grid(gpu_ctx, v, n, 5.0)
