@@ -232,6 +232,8 @@ static G2Build g2_args(grt_scene_t* sc, bool with_outputs)
     b.longl = sc->g2long.as<uint32_t>();
     b.long_cap = sc->g2_long_cap;
     b.sub_tab = sc->g2tab.as<float>();
+    b.skip = sc->g2skip.as<uint8_t>();
+    b.skip_stride = sc->g2_top_cap;
     return b;
 }
 
@@ -249,6 +251,8 @@ static int g2_reserve_top(grt_scene_t* sc)
     if ((rc = sc->g2tab.reserve(sc->ctx, 96 * sizeof(float), true)) != GRT_OK)
         return rc;
     if ((rc = sc->cellrec.reserve(sc->ctx, (size_t)sc->cells_cap * 8, false)) != GRT_OK)
+        return rc;
+    if ((rc = sc->g2skip.reserve(sc->ctx, (size_t)sc->cells_cap * 2, false)) != GRT_OK)
         return rc;
     if ((rc = sc->g2tiles.reserve(sc->ctx, 2 * ((size_t)sc->cells_cap / kScanTile + 1) * 4,
                                   false)) != GRT_OK)
@@ -991,8 +995,9 @@ int grt_subgrid_download(grt_scene sc, uint32_t* cellsub_host, uint32_t* sub_off
             e = hipStreamSynchronize(s);
         for (size_t c = 0; e == hipSuccess && c < gi.m; ++c) {
             const bool sub = (rec[2 * c] >> 31) != 0;
+            const bool empty = rec[2 * c] == rec[2 * c + 1];
             cellsub_host[2 * c] = sub ? rec[2 * c] & 0x7fffffffu : 0u;
-            cellsub_host[2 * c + 1] = sub ? rec[2 * c + 1] : 0u;
+            cellsub_host[2 * c + 1] = sub ? rec[2 * c + 1] : empty ? 0x80000000u | rec[2 * c] : 0u;
         }
     }
     if (e == hipSuccess && sub_offsets_host)

@@ -30,6 +30,7 @@
 //   k_g2_sort_wave    sub-cells of 17..1024 references, a wave each
 //   k_g2_sort_big     sub-cells of more than 1024 references
 //   k_gather_refs     48-byte record of every sub-reference
+//   k_g2_skip_x, k_g2_skip_line<1, 2>  skip radius of every empty top cell into its record
 #include "grt_device.h"
 #include "grt_launch.h"
 
@@ -486,6 +487,137 @@ hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t 
     return hipGetLastError();
 }
 
+// Skip radii: the Chebyshev distance D from each top cell to the nearest cell
+// holding a reference (cells outside the grid hold none), capped at C =
+// kSkipMax + 1, axis by axis -- D_x along the cell's x row, then D_xy =
+// min_k max(|k|, D_x(y + k)), then D = min_k max(|k|, D_xy(z + k)) -- so an
+// empty cell's radius is D - 1.  Each pass stages its tile plus a halo of
+// kSkipMax cells per side in LDS; a cell's scan stops once |k| reaches the
+// best distance found.  The frame's dimensions are device values, so the
+// launches are fixed grids that loop over the tiles.
+constexpr uint32_t kSkipTileX = 1024;   // cells per x-pass tile
+constexpr uint32_t kSkipHalo = (uint32_t)kSkipMax;
+constexpr uint32_t kSkipRows = 64;      // rows per line-pass tile (64 lines wide)
+
+// x pass: occupancy bits of a tile of consecutive cells and 64 cells either
+// side (one ballot per wave), then per cell the nearest occupied cell of its
+// row within kSkipMax from the 15-bit window around it (rows may change
+// inside the tile: the window is masked to the row).
+__global__ __launch_bounds__(256) void k_g2_skip_x(G2Build b)
+{
+    constexpr uint32_t W = kSkipTileX / 64u + 2u;   // words: the tile and one either side
+    __shared__ uint64_t s_occ[W + 1];
+    const DevFrame* fr = b.fr;
+    if (!top_ok(fr))
+        return;
+    const uint32_t m = fr->m, n0 = (uint32_t)fr->n[0];
+    constexpr uint32_t C = (uint32_t)kSkipMax + 1u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t tiles = (m + kSkipTileX - 1u) / kSkipTileX;
+    for (uint32_t w = blockIdx.x; w < tiles; w += gridDim.x) {
+        const int64_t base = (int64_t)w * kSkipTileX - 64;
+        for (uint32_t q = wave; q < W; q += 4u) {
+            const int64_t c = base + 64 * (int64_t)q + lane;
+            const bool occ = c >= 0 && c < (int64_t)m && b.offsets[c + 1] != b.offsets[c];
+            const uint64_t bits = __ballot(occ);
+            if (lane == 0u)
+                s_occ[q] = bits;
+        }
+        if (threadIdx.x == 0u)
+            s_occ[W] = 0ull;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kSkipTileX; i += 256u) {
+            const uint32_t c = w * kSkipTileX + i;
+            if (c >= m)
+                break;
+            // bits p - 7 .. p + 7 of the tile's bit array, p = the cell's bit
+            const uint32_t lo = i + 64u - kSkipHalo;
+            const uint32_t sh = lo & 63u;
+            const uint64_t w0 = s_occ[lo >> 6], w1 = s_occ[(lo >> 6) + 1u];
+            const uint32_t win = (uint32_t)((sh ? (w0 >> sh) | (w1 << (64u - sh)) : w0) & 0x7fffu);
+            const uint32_t x = c % n0, right = n0 - 1u - x;
+            uint32_t d = C;
+            if (win & (1u << kSkipHalo)) {
+                d = 0u;
+            } else {
+                // below: bit j is the cell k = 7 - j to the left, within the row
+                const uint32_t below =
+                    win & (x >= kSkipHalo ? 0x7fu : (0x7fu << (kSkipHalo - x)) & 0x7fu);
+                if (below)
+                    d = kSkipHalo - (31u - (uint32_t)__clz(below));
+                // above: bit i is the cell k = i + 1 to the right, within the row
+                const uint32_t above =
+                    (win >> (kSkipHalo + 1u)) & (right >= kSkipHalo ? 0x7fu : (1u << right) - 1u);
+                if (above) {
+                    const uint32_t ka = (uint32_t)__ffs(above);
+                    d = ka < d ? ka : d;
+                }
+            }
+            b.skip[c] = (uint8_t)d;
+        }
+        __syncthreads();
+    }
+}
+
+// y (AXIS 1) or z (AXIS 2) pass over tiles of 64 lines x kSkipRows rows: the
+// lines of a tile start at 64 consecutive cells of coordinate 0 on the axis
+// (x of one z plane for y; x + n0 y for z).  The z pass writes the radius of
+// every empty cell into its record as {D - 1, D - 1}.
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
+{
+    __shared__ uint8_t s_in[kSkipRows + 2 * kSkipHalo][64];
+    const DevFrame* fr = b.fr;
+    if (!top_ok(fr))
+        return;
+    constexpr uint32_t C = (uint32_t)kSkipMax + 1u;
+    const uint32_t n0 = (uint32_t)fr->n[0], n1 = (uint32_t)fr->n[1], n2 = (uint32_t)fr->n[2];
+    const uint32_t step = AXIS == 1 ? n0 : n0 * n1;        // cell stride along the axis
+    const uint32_t len = AXIS == 1 ? n1 : n2;
+    const uint32_t span = AXIS == 1 ? n0 : n0 * n1;        // line starts per plane
+    const uint32_t planes = AXIS == 1 ? n2 : 1u;
+    // rows per tile: the axis when it is short, else kSkipRows
+    const uint32_t rows = len < kSkipRows ? ((len + 3u) & ~3u) : kSkipRows;
+    const uint32_t gx = (span + 63u) / 64u, segs = (len + rows - 1u) / rows;
+    const uint64_t work = (uint64_t)gx * planes * segs;
+    const uint8_t* in = AXIS == 1 ? b.skip : b.skip + b.skip_stride;
+    const uint32_t lane = threadIdx.x & 63u, row0 = threadIdx.x >> 6;
+    for (uint64_t w = blockIdx.x; w < work; w += gridDim.x) {
+        const uint32_t seg = (uint32_t)(w % segs);
+        const uint64_t g = w / segs;
+        const uint32_t a = (uint32_t)(g % gx) * 64u + lane;
+        const bool lane_ok = a < span;
+        const uint64_t start = (uint64_t)a + (uint64_t)(g / gx) * n0 * n1;
+        const int64_t q0 = (int64_t)seg * rows - (int64_t)kSkipHalo;
+        for (uint32_t r = row0; r < rows + 2u * kSkipHalo; r += 4u) {
+            const int64_t q = q0 + (int64_t)r;
+            s_in[r][lane] = lane_ok && q >= 0 && q < (int64_t)len
+                                ? in[start + (uint64_t)q * step]
+                                : (uint8_t)C;
+        }
+        __syncthreads();
+        for (uint32_t r = row0; r < rows; r += 4u) {
+            const uint32_t q = seg * rows + r;
+            if (!lane_ok || q >= len)
+                continue;
+            const uint32_t j = r + kSkipHalo;
+            uint32_t d = s_in[j][lane];
+            for (uint32_t k = 1; k < d; ++k) {
+                const uint32_t lo = s_in[j - k][lane], hi = s_in[j + k][lane];
+                const uint32_t v = lo < hi ? lo : hi;
+                const uint32_t cand = v > k ? v : k;
+                d = cand < d ? cand : d;
+            }
+            const uint64_t c = start + (uint64_t)q * step;
+            if (AXIS == 1)
+                b.skip[b.skip_stride + c] = (uint8_t)d;
+            else if (d > 0u)
+                b.cellrec[c] = make_uint2(d - 1u, d - 1u);
+        }
+        __syncthreads();
+    }
+}
+
 hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
                               uint32_t sub_refs_cap, uint32_t dense_cap, const float4* tri,
                               float4* sub_trir, hipStream_t s)
@@ -493,6 +625,13 @@ hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t
     hipError_t e0 = launch_grid2_plan(b, top_cells_cap, sub_cells_cap, sub_refs_cap, dense_cap, s);
     if (e0 != hipSuccess)
         return e0;
+    if (b.skip_stride < top_cells_cap)
+        return hipErrorInvalidValue;
+    const uint32_t nsk = top_cells_cap / kSkipTileX + 1u;
+    const dim3 gsk(nsk < 4096u ? nsk : 4096u);
+    hipLaunchKernelGGL(k_g2_skip_x, gsk, dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_g2_skip_line<1>, gsk, dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_g2_skip_line<2>, gsk, dim3(256), 0, s, b);
     const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
     hipLaunchKernelGGL(k_g2_fill, dim3(nb), dim3(1024), 0, s, b);
     const uint32_t nt = sub_cells_cap / 256u + 1u;

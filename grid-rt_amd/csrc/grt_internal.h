@@ -85,8 +85,8 @@ struct grt_scene_t {
     float g2_lambda = 0.0f;
     uint32_t g2_min_refs = 8;
     bool g2_built = false;   // the last build produced the second level's records
-    grt::DevMem g2frame, g2tab, cellrec, g2tiles, g2dense, g2box, g2big, g2long, sub_offsets,
-        sub_refs, sub_trir;
+    grt::DevMem g2frame, g2tab, cellrec, g2skip, g2tiles, g2dense, g2box, g2big, g2long,
+        sub_offsets, sub_refs, sub_trir;
     uint32_t g2_top_cap = 0, g2_box_cap = 0, g2_cells_cap = 0, g2_refs_cap = 0, g2_dense_cap = 0,
              g2_big_cap = 0, g2_long_cap = 0;
     bool grid_valid = false;
@@ -99,7 +99,7 @@ struct grt_scene_t {
         return {&verts,  &normals,    &rest_verts, &rest_normals, &tri,       &xf,
                 &frame,  &partial,    &counts,     &offsets,      &refs,      &block_sums,
                 &big_list, &sort_list, &sort_list_small, &sort_classes, &small_counters,
-                &trir, &g2frame, &g2tab, &cellrec, &g2tiles, &g2dense, &g2box, &g2big, &g2long,
+                &trir, &g2frame, &g2tab, &cellrec, &g2skip, &g2tiles, &g2dense, &g2box, &g2big, &g2long,
                 &sub_offsets, &sub_refs, &sub_trir};
     }
     ~grt_scene_t()

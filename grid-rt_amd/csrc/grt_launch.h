@@ -85,6 +85,10 @@ hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32
 
 // ---- second grid level (grt_grid2.hip) ----------------------------------------
 constexpr uint32_t kSubMax = 16;   // sub-cells per axis of a dense top cell, at most
+// Skip radius of an empty top cell, at most: the largest r <= kSkipMax such
+// that every cell within r cells of it on every axis (Chebyshev distance)
+// holds no reference; cells outside the grid hold none.
+constexpr int32_t kSkipMax = 7;
 struct G2Build {
     const DevFrame* fr;         // top frame (m cells, status)
     DevFrame* g2;               // level 2: m = sub-cells, r = sub-references, status
@@ -96,7 +100,8 @@ struct G2Build {
     uint32_t* tile_m;           // per 4096-cell tile: sub-cells, scanned in place
     uint32_t* tile_d;           // per tile: dense cells, scanned in place
     uint2* cellrec;             // per top cell {0x80000000 | first sub-cell, r0|r1<<8|r2<<16}
-                                // with a sub-grid, else {first ref, end ref}
+                                // with a sub-grid, {r, r} (skip radius r) when empty,
+                                // else {first ref, end ref}
     uint32_t* dense;            // dense top cells in cell order [dense_cap]
     uint32_t* dense_refs;       // per dense cell: sub-references, then their prefix [dense_cap]
     uint32_t* sub_offsets;      // [sub-cells + 1]
@@ -107,6 +112,8 @@ struct G2Build {
     uint32_t* longl;            // sub-cells of 17..1024 references [long_cap]
     uint32_t long_cap;
     float* sub_tab;             // [96]: cell_a / r at 16 a + r - 1, 1 / that at 48 + 16 a + r - 1
+    uint8_t* skip;              // two planes of skip_stride bytes: per-axis distances
+    uint32_t skip_stride;       // >= top cells
 };
 // Plan, dense list and per-cell counts: g2->m, g2->n[0], g2->r receive the
 // sub-cell, dense-cell and sub-reference totals, g2->status the capacities
@@ -115,7 +122,8 @@ struct G2Build {
 hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
                              uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s);
 // The whole second level after the top build on the same stream: the plan,
-// then sub-cell offsets, references and their 48-byte records.  Over a
+// the skip radii of the empty top cells, then sub-cell offsets, references
+// and their 48-byte records.  Over a
 // capacity, g2->status is set and every cell record says "no sub-grid".
 hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
                               uint32_t sub_refs_cap, uint32_t dense_cap, const float4* tri,

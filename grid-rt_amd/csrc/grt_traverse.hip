@@ -1409,8 +1409,8 @@ __device__ __forceinline__ bool sub_step(const Ray2& r, Sub2& q)
     return true;
 }
 
-// One top cell (index ci, coordinates of s) of the two-level walk, entered at
-// t_enter and left at exit_t: its sub-grid from t_enter when it has one and
+// One non-empty top cell (index ci, coordinates of s, record rec) of the
+// two-level walk, entered at t_enter and left at exit_t: its sub-grid from t_enter when it has one and
 // the second level is valid, else its own references bounded by exit_t.
 // Counts the sub-cells visited and the tests.  True when the walk ends in this
 // cell: a hit (closest), an occluder (any, *occ set) or a sub-cell whose exit
@@ -1418,11 +1418,11 @@ __device__ __forceinline__ bool sub_step(const Ray2& r, Sub2& q)
 template <bool ANY>
 __device__ __forceinline__ bool top_cell2(const TraceArgs& a, const DevFrame* __restrict__ fr,
                                           const float* tab, bool sub_ok, const Ray2& r,
-                                          const Dda& s, uint32_t ci, float t_enter, float exit_t,
-                                          bool& occ, int32_t& id, float& best, float& bu,
-                                          float& bv, uint32_t& cells, uint32_t& tests)
+                                          const Dda& s, uint2 rec, uint32_t ci, float t_enter,
+                                          float exit_t, bool& occ, int32_t& id, float& best,
+                                          float& bu, float& bv, uint32_t& cells,
+                                          uint32_t& tests)
 {
-    const uint2 rec = a.cellrec[ci];
     const bool has_sub = (rec.x >> 31) != 0u;
     if (!(sub_ok && has_sub)) {
         // the top cell's own list (also for a sub-grid cell when the second
@@ -1482,6 +1482,90 @@ __device__ __forceinline__ bool ray2_setup(const DevFrame* __restrict__ fr, cons
     return true;
 }
 
+// The walk over the top grid goes item by item.  An item is a cell of the
+// grid; for an empty cell with skip radius rr > 0 it is the cells ahead of it
+// within rr on every axis (all empty), left in one step.  Each item's exit is
+// its first boundary plane crossed, from the plane formula of the DDA set-up
+// (t = ((float)k cell_a + bmin_a - o_a) / d_a), in the DDA's tie order.
+
+// Skip radius of a cell record: {r, r} marks an empty cell.
+__device__ __forceinline__ int32_t skip_radius(uint2 rec, bool& empty)
+{
+    empty = rec.x == rec.y;
+    return empty && rec.x <= (uint32_t)kSkipMax ? (int32_t)rec.x : 0;
+}
+
+// Exit t of the item at the walk's cell s with radius rr; axis: the axis
+// crossed.
+__device__ __forceinline__ float item_exit(const DevFrame* __restrict__ fr, const Ray2& r,
+                                           const Dda& s, int32_t rr, int& axis)
+{
+    const float t0 =
+        ((float)(r.n0 ? s.cx - rr : s.cx + rr + 1) * fr->cell[0] + fr->bmin[0] - r.o0) * r.i0;
+    const float t1 =
+        ((float)(r.n1 ? s.cy - rr : s.cy + rr + 1) * fr->cell[1] + fr->bmin[1] - r.o1) * r.i1;
+    const float t2 =
+        ((float)(r.n2 ? s.cz - rr : s.cz + rr + 1) * fr->cell[2] + fr->bmin[2] - r.o2) * r.i2;
+    if (t0 < t1 && t0 < t2) {
+        axis = 0;
+        return t0;
+    }
+    if (t1 < t2) {
+        axis = 1;
+        return t1;
+    }
+    axis = 2;
+    return t2;
+}
+
+// v kept within the rr cells ahead of c (behind, for a negative direction).
+__device__ __forceinline__ int32_t skip_cell(int32_t v, int32_t c, int32_t rr, bool neg)
+{
+    if (neg) {
+        v = v < c ? v : c;
+        return v > c - rr ? v : c - rr;
+    }
+    v = v > c ? v : c;
+    return v < c + rr ? v : c + rr;
+}
+
+// Step to the next item across the current one's exit (axis, at t = e);
+// false when that leaves the grid.  After a skip (rr > 0) the other two
+// axes take the cells of the exit point, kept within the skipped cells: no
+// axis ever moves backwards, and the crossed one moves by rr + 1.
+__device__ __forceinline__ bool item_advance(const DevFrame* __restrict__ fr, const Ray2& r,
+                                             Dda& s, int32_t rr, int axis, float e)
+{
+    if (axis == 0) {
+        const int32_t c = r.n0 ? s.cx - rr - 1 : s.cx + rr + 1;
+        if (c < 0 || c >= fr->n[0])
+            return false;
+        s.cx = c;
+    } else if (axis == 1) {
+        const int32_t c = r.n1 ? s.cy - rr - 1 : s.cy + rr + 1;
+        if (c < 0 || c >= fr->n[1])
+            return false;
+        s.cy = c;
+    } else {
+        const int32_t c = r.n2 ? s.cz - rr - 1 : s.cz + rr + 1;
+        if (c < 0 || c >= fr->n[2])
+            return false;
+        s.cz = c;
+    }
+    if (rr > 0) {
+        if (axis != 0)
+            s.cx = skip_cell(vox(r.o0 + r.d0 * e, fr->bmin[0], fr->inv[0], fr->n[0]), s.cx, rr,
+                             r.n0);
+        if (axis != 1)
+            s.cy = skip_cell(vox(r.o1 + r.d1 * e, fr->bmin[1], fr->inv[1], fr->n[1]), s.cy, rr,
+                             r.n1);
+        if (axis != 2)
+            s.cz = skip_cell(vox(r.o2 + r.d2 * e, fr->bmin[2], fr->inv[2], fr->n[2]), s.cz, rr,
+                             r.n2);
+    }
+    return true;
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA hits,
                                                 uint8_t* __restrict__ occluded)
@@ -1508,14 +1592,18 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
         const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
         for (;;) {
             ++cells;
-            const float exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
             const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            if (top_cell2<ANY>(a, fr, tab, sub_ok, r, s, ci, t_enter, exit_t, occ, id, best, bu,
-                               bv, cells, tests) ||
-                exit_t > r.tmax)
+            const uint2 rec = a.cellrec[ci];
+            bool empty;
+            const int32_t rr = skip_radius(rec, empty);
+            int axis;
+            const float e = item_exit(fr, r, s, rr, axis);
+            if ((!empty && top_cell2<ANY>(a, fr, tab, sub_ok, r, s, rec, ci, t_enter, e, occ, id,
+                                          best, bu, bv, cells, tests)) ||
+                e > r.tmax)
                 break;
-            t_enter = exit_t;
-            if (!dda_step(s))
+            t_enter = e;
+            if (!item_advance(fr, r, s, rr, axis, e))
                 break;
         }
     }
@@ -1533,13 +1621,19 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
     }
 }
 
-// `count` DDA steps, each after checking the cell's exit against tmax (the
-// serial walk's stop); t_enter receives the exit of the last cell left.
-__device__ __forceinline__ bool dda_skip2(Dda& s, uint32_t count, float tmax, float& t_enter)
+// `count` items of the walk, each after checking its exit against tmax (the
+// serial walk's stop); t_enter receives the exit of the last item left.
+__device__ __forceinline__ bool walk_skip(const TraceArgs& a, const DevFrame* __restrict__ fr,
+                                          const Ray2& r, Dda& s, uint32_t count, float& t_enter)
 {
+    const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
     for (uint32_t q = 0; q < count; ++q) {
-        const float e = tmin_f(tmin_f(s.tx, s.ty), s.tz);
-        if (e > tmax || !dda_step(s))
+        const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+        bool empty;
+        const int32_t rr = skip_radius(a.cellrec[ci], empty);
+        int axis;
+        const float e = item_exit(fr, r, s, rr, axis);
+        if (e > r.tmax || !item_advance(fr, r, s, rr, axis, e))
             return false;
         t_enter = e;
     }
@@ -1578,18 +1672,23 @@ __global__ __launch_bounds__(256) void k_trace2_split(TraceArgs a, RaySoA rays, 
     Ray2 r = {};
     Dda s = {};
     float t_enter = 0.0f;
-    bool alive = valid && ray2_setup(fr, rays, i, r, s, t_enter) && dda_skip2(s, j, r.tmax, t_enter);
+    bool alive = valid && ray2_setup(fr, rays, i, r, s, t_enter) && walk_skip(a, fr, r, s, j, t_enter);
     bool winner = false;
     while (__ballot(alive) != 0) {
         const uint32_t c0 = cells, t0 = tests;
         bool stop = false;
-        float exit_t = 0.0f;
+        int32_t rr = 0;
+        int axis = 0;
+        float e = 0.0f;
         if (alive) {
             ++cells;
-            exit_t = tmin_f(tmin_f(s.tx, s.ty), s.tz);
             const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            stop = top_cell2<ANY>(a, fr, tab, sub_ok, r, s, ci, t_enter, exit_t, occ, id, best,
-                                  bu, bv, cells, tests);
+            const uint2 rec = a.cellrec[ci];
+            bool empty;
+            rr = skip_radius(rec, empty);
+            e = item_exit(fr, r, s, rr, axis);
+            stop = !empty && top_cell2<ANY>(a, fr, tab, sub_ok, r, s, rec, ci, t_enter, e, occ,
+                                            id, best, bu, bv, cells, tests);
         }
         // The group's earliest cell that ends the walk ends the ray; cells
         // after it (higher lanes of this round) are not part of the serial walk.
@@ -1603,8 +1702,11 @@ __global__ __launch_bounds__(256) void k_trace2_split(TraceArgs a, RaySoA rays, 
             winner = alive && j == jw && (ANY ? occ : id >= 0);
             alive = false;
         } else if (alive) {
-            t_enter = exit_t;
-            alive = dda_skip2(s, lanes, r.tmax, t_enter);
+            alive = !(e > r.tmax) && item_advance(fr, r, s, rr, axis, e);   // NaN tmax: no stop
+            if (alive) {
+                t_enter = e;
+                alive = walk_skip(a, fr, r, s, lanes - 1u, t_enter);
+            }
         }
     }
     const bool group_hit = (__ballot(winner) & gmask) != 0;

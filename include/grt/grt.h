@@ -261,6 +261,9 @@ typedef struct {
 int grt_subgrid_info_get(grt_scene scene, grt_subgrid_info* out);
 /* cellsub_host: 2 words per top cell {first sub-cell, r0 | r1 << 8 | r2 << 16
  * (0: no sub-grid)}; sub-cell (x, y, z) of a cell is first + x + r0 (y + r1 z);
+ * an empty top cell reads {0, 0x80000000 | r}, r its skip radius (0..7): no
+ * cell within r cells of it on every axis holds a reference, and the walk
+ * crosses the r cells ahead of it on every axis in one step;
  * sub_offsets_host: cells + 1 words; sub_refs_host: refs words, ascending
  * triangle ids per sub-cell (any pointer may be NULL). */
 int grt_subgrid_download(grt_scene scene, uint32_t* cellsub_host, uint32_t* sub_offsets_host,

@@ -552,7 +552,11 @@ void orc_trace_any(const orc_frame* f, const uint32_t* offsets, const uint32_t* 
  * ascending triangle ids per sub-cell.  cellsub[2c] = first sub-cell of c in
  * the global sub-cell order (0 without a sub-grid), cellsub[2c+1] = r0 |
  * r1 << 8 | r2 << 16 (0: no sub-grid); sub-cell (x, y, z) of c is
- * cellsub[2c] + x + r0 (y + r1 z). */
+ * cellsub[2c] + x + r0 (y + r1 z).  An empty top cell has cellsub[2c+1] =
+ * 0x80000000 | r, its skip radius: the largest r <= ORC_SKIP_MAX such that no
+ * cell within r cells of it on every axis (Chebyshev distance; cells outside
+ * the grid count as empty) holds a reference. */
+#define ORC_SKIP_MAX 7
 typedef struct {
     float bmin[3], cell[3], inv[3];
     int32_t r[3];
@@ -560,7 +564,7 @@ typedef struct {
 
 static inline int sub_frame_of(const orc_frame* f, uint32_t c, uint32_t packed, sub_frame* s)
 {
-    if (packed == 0)
+    if (packed == 0 || (packed >> 31) != 0)
         return 0;
     const uint32_t nx = (uint32_t)f->n[0], ny = (uint32_t)f->n[1];
     const uint32_t cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
@@ -581,6 +585,68 @@ static inline int32_t sub_vox(const sub_frame* s, float val, int a)
     const float hi = (float)(s->r[a] - 1);
     x = x < hi ? x : hi;
     return (int32_t)x;
+}
+
+/* Skip radii of the empty cells: D = the Chebyshev distance to the nearest
+ * occupied cell, capped at C = ORC_SKIP_MAX + 1, as three separable passes --
+ * along x the distance to the nearest occupied cell of the row (a forward and
+ * a backward sweep), then along y and z min over |k| < C of max(|k|, previous
+ * pass at k) -- and r = D - 1. */
+static void grid2_skip_radii(const orc_frame* f, const uint32_t* offsets, uint32_t* cellsub)
+{
+    const int32_t C = ORC_SKIP_MAX + 1;
+    const int32_t n0 = f->n[0], n1 = f->n[1], n2 = f->n[2];
+    const size_t m = f->m;
+    uint8_t* d = (uint8_t*)malloc(m ? m : 1);
+    uint8_t* e = (uint8_t*)malloc(m ? m : 1);
+    for (int32_t z = 0; z < n2; ++z)
+        for (int32_t y = 0; y < n1; ++y) {
+            const size_t row = (size_t)n0 * ((size_t)y + (size_t)n1 * (size_t)z);
+            int32_t last = -C;                  /* nearest occupied at or before x */
+            for (int32_t x = 0; x < n0; ++x) {
+                if (offsets[row + x + 1] != offsets[row + x])
+                    last = x;
+                const int32_t dist = x - last;
+                d[row + x] = (uint8_t)(dist < C ? dist : C);
+            }
+            last = n0 + C;                      /* nearest occupied at or after x */
+            for (int32_t x = n0 - 1; x >= 0; --x) {
+                if (offsets[row + x + 1] != offsets[row + x])
+                    last = x;
+                const int32_t dist = last - x;
+                if (dist < d[row + x])
+                    d[row + x] = (uint8_t)dist;
+            }
+        }
+    /* along y (d -> e), then z (e -> d) */
+    for (int a = 1; a < 3; ++a) {
+        const uint8_t* src = a == 1 ? d : e;
+        uint8_t* dst = a == 1 ? e : d;
+        const size_t stride = a == 1 ? (size_t)n0 : (size_t)n0 * (size_t)n1;
+        const int32_t len = a == 1 ? n1 : n2;
+        for (size_t c = 0; c < m; ++c) {
+            const int32_t q = (int32_t)((c / stride) % (size_t)len);
+            int32_t best = src[c];
+            for (int32_t k = 1; k < C; ++k) {
+                if (q - k >= 0) {
+                    const int32_t v = src[c - (size_t)k * stride];
+                    const int32_t cand = v > k ? v : k;
+                    best = cand < best ? cand : best;
+                }
+                if (q + k < len) {
+                    const int32_t v = src[c + (size_t)k * stride];
+                    const int32_t cand = v > k ? v : k;
+                    best = cand < best ? cand : best;
+                }
+            }
+            dst[c] = (uint8_t)best;
+        }
+    }
+    for (size_t c = 0; c < m; ++c)
+        if (offsets[c + 1] == offsets[c])
+            cellsub[2 * c + 1] = 0x80000000u | (uint32_t)(d[c] - 1);
+    free(d);
+    free(e);
 }
 
 uint64_t orc_grid2_plan(const orc_frame* f, const uint32_t* offsets, uint32_t min_refs,
@@ -609,6 +675,7 @@ uint64_t orc_grid2_plan(const orc_frame* f, const uint32_t* offsets, uint32_t mi
         if (packed)
             total += m;
     }
+    grid2_skip_radii(f, offsets, cellsub);
     return total;
 }
 
@@ -748,13 +815,32 @@ static void grid2_trace_one(const orc_frame* f, const uint32_t* offsets, const u
             t_enter = tmax_f(t_enter, tmin_f(p, q));
         }
         const uint32_t nx = (uint32_t)f->n[0], ny = (uint32_t)f->n[1];
+        int neg[3];
+        for (int a = 0; a < 3; ++a)
+            neg[a] = signbit(d[a]) != 0;
+        int32_t* c = st.c;
+        /* The top walk goes item by item: a cell, or for an empty cell of skip
+         * radius rr > 0 the cells within rr ahead of it on every axis, left in
+         * one step.  An item's exit is its first boundary plane crossed, t =
+         * ((float)k cell_a + bmin_a - o_a) / d_a, in the DDA's tie order (x
+         * before y before z on ties).  After a skip the other two axes take
+         * the cells of the exit point (vox), kept within the skipped cells. */
         for (;;) {
             ++cells;
-            const float exit_t = tmin_f(tmin_f(st.tnext[0], st.tnext[1]), st.tnext[2]);
-            const uint32_t ci =
-                (uint32_t)st.c[0] + nx * ((uint32_t)st.c[1] + ny * (uint32_t)st.c[2]);
+            const uint32_t ci = (uint32_t)c[0] + nx * ((uint32_t)c[1] + ny * (uint32_t)c[2]);
+            const int empty = offsets[ci + 1] == offsets[ci];
+            const int32_t rr = empty ? (int32_t)(cellsub[2 * ci + 1] & 0xffu) : 0;
+            float tp[3];
+            for (int a = 0; a < 3; ++a) {
+                const int32_t k = neg[a] ? c[a] - rr : c[a] + rr + 1;
+                tp[a] = ((float)k * f->cell[a] + f->bmin[a] - o[a]) * inv[a];
+            }
+            const int ax = (tp[0] < tp[1] && tp[0] < tp[2]) ? 0 : (tp[1] < tp[2]) ? 1 : 2;
+            const float exit_t = tp[ax];
             sub_frame sf;
-            if (sub_frame_of(f, ci, cellsub[2 * ci + 1], &sf)) {
+            if (empty) {
+                /* nothing to test */
+            } else if (sub_frame_of(f, ci, cellsub[2 * ci + 1], &sf)) {
                 const uint32_t base = cellsub[2 * ci];
                 sub_dda q;
                 sub_dda_setup(&sf, o, d, inv, t_enter, &q);
@@ -818,8 +904,23 @@ static void grid2_trace_one(const orc_frame* f, const uint32_t* offsets, const u
             if (done || exit_t > tmax)
                 break;
             t_enter = exit_t;
-            if (!dda_step(&st))
+            const int32_t next = neg[ax] ? c[ax] - rr - 1 : c[ax] + rr + 1;
+            if (next < 0 || next >= f->n[ax])
                 break;
+            c[ax] = next;
+            if (rr > 0)
+                for (int a = 0; a < 3; ++a) {
+                    if (a == ax)
+                        continue;
+                    int32_t v = vox(f, o[a] + d[a] * exit_t, a);
+                    if (neg[a]) {
+                        v = v < c[a] ? v : c[a];
+                        c[a] = v > c[a] - rr ? v : c[a] - rr;
+                    } else {
+                        v = v > c[a] ? v : c[a];
+                        c[a] = v < c[a] + rr ? v : c[a] + rr;
+                    }
+                }
         }
     }
     if (any) {

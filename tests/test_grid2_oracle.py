@@ -1,7 +1,9 @@
 """CPU pins of the oracle's two-level grid (orc_grid2_plan / count / fill and
 orc_grid2_trace_closest / any): the sub-grid of every dense top cell from its
-geometry, the resolution rule from the mathematics it states, and the traces
-against float64 brute force over all triangles and the one-level oracle.
+geometry, the resolution rule from the mathematics it states, every empty
+cell's skip radius from a brute-force scan of the cubes around it, and the
+traces against float64 brute force over all triangles and the one-level
+oracle.
 The GPU suite then holds the HIP build and traversal to this oracle bit for
 bit (tests/test_gpu_grid2.py)."""
 import math
@@ -14,6 +16,13 @@ import pyoracle as orc
 from conftest import SEED_C1, rng
 
 LAM, LAM2, MIN_REFS, RMAX = 0.5, 4.0, 8, 16
+SKIP_MAX = 7
+EMPTY = np.uint32(0x80000000)
+
+
+def has_subgrid(cs):
+    """Top cells with a sub-grid (plan word non-zero, without the empty-cell flag)."""
+    return (cs[:, 1] != 0) & ((cs[:, 1] & EMPTY) == 0)
 
 
 @pytest.fixture(scope="module")
@@ -43,6 +52,9 @@ def test_resolution_is_the_smallest_per_axis(two_level):
     for c in range(int(fr["m"])):
         n = int(counts[c])
         p = int(cs[c, 1])
+        if n == 0:
+            assert p >> 31 == 1 and cs[c, 0] == 0, c
+            continue
         if n < MIN_REFS:
             assert p == 0 and cs[c, 0] == 0, c
             continue
@@ -73,7 +85,7 @@ def test_sub_cells_hold_the_triangles_that_overlap_them(two_level):
     cell = np.array(fr["cell"], np.float64)
     assert so[0] == 0 and np.all(np.diff(so.astype(np.int64)) >= 0) and so[-1] == len(sr)
     checked = 0
-    for c in np.nonzero(cs[:, 1])[0]:
+    for c in np.nonzero(has_subgrid(cs))[0]:
         p = int(cs[c, 1])
         r = np.array([p & 255, (p >> 8) & 255, (p >> 16) & 255])
         base = int(cs[c, 0])
@@ -108,6 +120,55 @@ def test_sub_cells_hold_the_triangles_that_overlap_them(two_level):
                         assert qx + r[0] * (qy + r[1] * qz) in got, (c, int(t), qx, qy, qz)
             checked += 1
     assert checked > 1000
+
+
+def _skip_radii_brute(occ):
+    """Largest r <= SKIP_MAX with the (2r+1)^3 cube around each cell empty
+    (cells outside the grid are empty), by scanning growing cubes."""
+    n2, n1, n0 = occ.shape
+    pad = SKIP_MAX + 1
+    big = np.zeros((n2 + 2 * pad, n1 + 2 * pad, n0 + 2 * pad), bool)
+    big[pad:pad + n2, pad:pad + n1, pad:pad + n0] = occ
+    r = np.full(occ.shape, -1, np.int64)
+    alive = ~occ
+    for k in range(SKIP_MAX + 1):
+        hit = np.zeros(occ.shape, bool)
+        for dz in range(-k, k + 1):
+            for dy in range(-k, k + 1):
+                for dx in range(-k, k + 1):
+                    if max(abs(dz), abs(dy), abs(dx)) != k:
+                        continue
+                    hit |= big[pad + dz:pad + dz + n2, pad + dy:pad + dy + n1,
+                               pad + dx:pad + dx + n0]
+        alive &= ~hit
+        r[alive] = k
+    return r
+
+
+@pytest.mark.parametrize("scene", ["spheres", "sparse"])
+def test_skip_radii_are_the_largest_empty_cubes(two_level, scene):
+    """Every empty top cell carries the largest r <= 7 such that no cell within
+    r cells of it on every axis holds a reference, checked against a scan of
+    the growing cubes around every cell; occupied cells carry no radius."""
+    if scene == "spheres":
+        v, fr, offs, refs, (cs, _, _) = two_level
+    else:
+        v, _ = grt.gen_spheres(SEED_C1 + 3, 3, 0.02, 0.05)
+        fr = orc.grid_frame(v, 4.0)
+        _, offs, refs = orc.grid_build(v, fr)
+        cs, _, _ = orc.grid2_build(v, fr, offs, refs, MIN_REFS, LAM2, RMAX)
+    n0, n1, n2 = (int(x) for x in fr["n"])
+    counts = np.diff(offs.astype(np.int64))
+    occ = (counts > 0).reshape(n2, n1, n0)
+    want = _skip_radii_brute(occ).ravel()
+    empty = counts == 0
+    assert np.all((cs[empty, 1] & EMPTY) != 0)
+    got = (cs[empty, 1] & 0xff).astype(np.int64)
+    assert np.array_equal(got, want[empty])
+    assert np.all((cs[~empty, 1] & EMPTY) == 0)
+    # radius 0 next to geometry; the cap in the sparse scene's open space
+    assert (got == 0).any() and got.max() >= (SKIP_MAX if scene == "sparse" else 4), \
+        np.bincount(got)
 
 
 def _rays(n, seed, spread=3.0):
