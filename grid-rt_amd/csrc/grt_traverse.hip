@@ -1438,19 +1438,24 @@ __device__ __forceinline__ bool top_cell2(const TraceArgs& a, const DevFrame* __
     }
     Sub2 q;
     sub_setup(fr, tab, r, s, rec, t_enter, q);
+    uint32_t kb = a.sub_offsets[q.si], ke = a.sub_offsets[q.si + 1];
     for (;;) {
         ++cells;
         const float sexit = tmin_f(tmin_f(q.tx, q.ty), q.tz);
-        const uint32_t kb = a.sub_offsets[q.si], ke = a.sub_offsets[q.si + 1];
-        if (trace2_cell<ANY>(a.sub_trir, kb, ke, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+        // the next sub-cell and its offsets, loaded before this one's tests
+        const uint32_t cb = kb, ce = ke;
+        const bool more = sub_step(r, q);
+        if (more) {
+            kb = a.sub_offsets[q.si];
+            ke = a.sub_offsets[q.si + 1];
+        }
+        if (trace2_cell<ANY>(a.sub_trir, cb, ce, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
                              r.tmax, sexit * kExitScale, id, best, bu, bv, tests)) {
             occ = ANY;
             return true;
         }
-        if (sexit > r.tmax)
-            return true;
-        if (!sub_step(r, q))
-            return false;
+        if (sexit > r.tmax || !more)
+            return sexit > r.tmax;
     }
 }
 
@@ -1590,21 +1595,28 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
     float t_enter;
     if (ray2_setup(fr, rays, i, r, s, t_enter)) {
         const uint32_t nx = (uint32_t)fr->n[0], ny = (uint32_t)fr->n[1];
+        uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
+        uint2 rec = a.cellrec[ci];
         for (;;) {
             ++cells;
-            const uint32_t ci = (uint32_t)s.cx + nx * ((uint32_t)s.cy + ny * (uint32_t)s.cz);
-            const uint2 rec = a.cellrec[ci];
             bool empty;
             const int32_t rr = skip_radius(rec, empty);
             int axis;
             const float e = item_exit(fr, r, s, rr, axis);
+            // the next item and its record, loaded before this cell's tests
+            Dda ns = s;
+            const bool more = !(e > r.tmax) && item_advance(fr, r, ns, rr, axis, e);
+            const uint32_t nci =
+                (uint32_t)ns.cx + nx * ((uint32_t)ns.cy + ny * (uint32_t)ns.cz);
+            const uint2 nrec = more ? a.cellrec[nci] : make_uint2(0u, 0u);
             if ((!empty && top_cell2<ANY>(a, fr, tab, sub_ok, r, s, rec, ci, t_enter, e, occ, id,
                                           best, bu, bv, cells, tests)) ||
-                e > r.tmax)
+                !more)
                 break;
             t_enter = e;
-            if (!item_advance(fr, r, s, rr, axis, e))
-                break;
+            s = ns;
+            ci = nci;
+            rec = nrec;
         }
     }
     if (ANY) {
