@@ -43,7 +43,7 @@ LAMBDA = 5.0
 LAMBDAS = (8.0, 32.0, 64.0, 128.0, 256.0, 512.0)
 # Two-level grids: (top lambda, sub-grid lambda, minimum references of a cell
 # that gets a sub-grid).
-TWO_LEVEL = ((8.0, 4.0, 8), (16.0, 4.0, 8))
+TWO_LEVEL = ((8.0, 4.0, 8), (16.0, 4.0, 8), (16.0, 4.0, 4))
 LIGHT = (1000.0, 1000.0, 1000.0)
 OFFSET = 1e-4
 EYE, CENTER, UP = (0.5, 0.6, -1.5), (0.5, 0.3, 0.5), (0.0, 1.0, 0.0)

@@ -20,9 +20,12 @@
 // fill pass scatters through LDS cursors, and every sub-cell's ids are then
 // sorted ascending over the whole level -- by a thread, a wave or, past 1024
 // ids, a block.
-//   k_g2_plan<false>  per tile: sub-cells and dense cells (tile sums)
+//   k_g2_plan<false>  per tile: sub-cells and dense cells (tile sums); the
+//                     x pass of the empty cells' skip radii
 //   k_g2_scan_tiles   exclusive scan of both; totals, capacity
-//   k_g2_plan<true>   per tile: cell records, the ordered dense-cell list
+//   k_g2_skip_line    y and z passes of the skip radii
+//   k_g2_plan<true>   per tile: cell records (skip radius of an empty cell),
+//                     the ordered dense-cell list
 //   k_g2_count        per dense cell: boxes, sub-cell counts, sub-references
 //   k_g2_scan_dense   exclusive scan of those; total, capacity
 //   k_g2_fill         per dense cell: sub-cell offsets, references
@@ -30,7 +33,6 @@
 //   k_g2_sort_wave    sub-cells of 17..1024 references, a wave each
 //   k_g2_sort_big     sub-cells of more than 1024 references
 //   k_gather_refs     48-byte record of every sub-reference
-//   k_g2_skip_x, k_g2_skip_line<1, 2>  skip radius of every empty top cell into its record
 #include "grt_device.h"
 #include "grt_launch.h"
 
@@ -75,6 +77,112 @@ __device__ __forceinline__ bool top_ok(const DevFrame* fr)
     return (fr->status & (kStatusFrameInvalid | kStatusCellsOver | kStatusRefsOver)) == 0;
 }
 
+// Skip radii: the Chebyshev distance D from each top cell to the nearest cell
+// holding a reference (cells outside the grid hold none), capped at C =
+// kSkipMax + 1, axis by axis -- D_x along the cell's x row (in the first plan
+// pass, from the occupancy bits of its tile), then D_xy = min_k max(|k|,
+// D_x(y + k)), then D = min_k max(|k|, D_xy(z + k)) -- and an empty cell's
+// record is {D - 1, D - 1} (the second plan pass).  The y and z passes stage
+// their tile plus a halo of kSkipMax rows per side in LDS; a cell's scan stops
+// once |k| reaches the best distance found.  The frame's dimensions are device
+// values, so those launches are fixed grids that loop over the tiles.
+constexpr uint32_t kSkipHalo = (uint32_t)kSkipMax;
+constexpr uint32_t kSkipRows = 64;      // rows per line-pass tile (64 lines wide)
+
+// D_x of the cell at bit p of an occupancy bit array (bit q: the cell p - q
+// positions before it; the array holds 7 bits either side), x its column in a
+// row of n0 cells: the nearest occupied cell of its row within kSkipMax.
+__device__ __forceinline__ uint32_t row_distance(const uint64_t* occ, uint32_t p, uint32_t x,
+                                                 uint32_t n0)
+{
+    constexpr uint32_t C = (uint32_t)kSkipMax + 1u;
+    const uint32_t lo = p - kSkipHalo, sh = lo & 63u;
+    const uint64_t w0 = occ[lo >> 6], w1 = occ[(lo >> 6) + 1u];
+    const uint32_t win = (uint32_t)((sh ? (w0 >> sh) | (w1 << (64u - sh)) : w0) & 0x7fffu);
+    if (win & (1u << kSkipHalo))
+        return 0u;
+    const uint32_t right = n0 - 1u - x;
+    uint32_t d = C;
+    // below: bit j is the cell 7 - j to the left, within the row
+    const uint32_t below = win & (x >= kSkipHalo ? 0x7fu : (0x7fu << (kSkipHalo - x)) & 0x7fu);
+    if (below)
+        d = kSkipHalo - (31u - (uint32_t)__clz(below));
+    // above: bit i is the cell i + 1 to the right, within the row
+    const uint32_t above =
+        (win >> (kSkipHalo + 1u)) & (right >= kSkipHalo ? 0x7fu : (1u << right) - 1u);
+    if (above) {
+        const uint32_t ka = (uint32_t)__ffs(above);
+        d = ka < d ? ka : d;
+    }
+    return d;
+}
+
+// y (AXIS 1: plane 0 -> plane 1 of skip) or z (AXIS 2: plane 1 -> plane 0)
+// pass over tiles of 64 lines x kSkipRows rows: the lines of a tile start at
+// 64 consecutive cells of coordinate 0 on the axis (x of one z plane for y;
+// x + n0 y for z).  Distances are stored as thermometer bytes, bit t set when
+// D <= t (D = 8: no bit), so that D_out <= t, i.e. some cell within t rows has
+// D_in <= t, is the OR over k <= t of bit t of the neighbours at +-k: per row
+// out = in | OR_k ((0xff << k) & (in(-k) | in(+k))), four lines per 32-bit
+// operation.  Rows outside the grid read 0 (no reference there).
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
+{
+    __shared__ uint32_t s_in[kSkipRows + 2 * kSkipHalo][16];   // 64 lines of bytes per row
+    const DevFrame* fr = b.fr;
+    if (!top_ok(fr))
+        return;
+    const uint32_t n0 = (uint32_t)fr->n[0], n1 = (uint32_t)fr->n[1], n2 = (uint32_t)fr->n[2];
+    const uint32_t step = AXIS == 1 ? n0 : n0 * n1;        // cell stride along the axis
+    const uint32_t len = AXIS == 1 ? n1 : n2;
+    const uint32_t span = AXIS == 1 ? n0 : n0 * n1;        // line starts per plane
+    const uint32_t planes = AXIS == 1 ? n2 : 1u;
+    // rows per tile: the axis when it is short, else kSkipRows
+    const uint32_t rows = len < kSkipRows ? len : kSkipRows;
+    const uint32_t gx = (span + 63u) / 64u, segs = (len + rows - 1u) / rows;
+    const uint64_t work = (uint64_t)gx * planes * segs;
+    const uint8_t* in = AXIS == 1 ? b.skip : b.skip + b.skip_stride;
+    uint8_t* out = AXIS == 1 ? b.skip + b.skip_stride : b.skip;
+    const uint32_t lane = threadIdx.x & 63u, row0 = threadIdx.x >> 6;
+    const uint32_t col = threadIdx.x & 15u, rsub = threadIdx.x >> 4;
+    uint8_t* s_bytes = reinterpret_cast<uint8_t*>(&s_in[0][0]);
+    for (uint64_t w = blockIdx.x; w < work; w += gridDim.x) {
+        const uint32_t seg = (uint32_t)(w % segs);
+        const uint64_t g = w / segs;
+        const uint32_t a0 = (uint32_t)(g % gx) * 64u;
+        const uint64_t plane = (g / gx) * (uint64_t)n0 * n1;
+        const int64_t q0 = (int64_t)seg * rows - (int64_t)kSkipHalo;
+        {
+            const uint32_t a = a0 + lane;
+            for (uint32_t r = row0; r < rows + 2u * kSkipHalo; r += 4u) {
+                const int64_t q = q0 + (int64_t)r;
+                s_bytes[64u * r + lane] = a < span && q >= 0 && q < (int64_t)len
+                                              ? in[(uint64_t)a + plane + (uint64_t)q * step]
+                                              : (uint8_t)0;
+            }
+        }
+        __syncthreads();
+        for (uint32_t r = rsub; r < rows; r += 16u) {
+            const uint32_t q = seg * rows + r;
+            if (q >= len)
+                break;
+            const uint32_t j = r + kSkipHalo;
+            uint32_t t = s_in[j][col];
+#pragma unroll
+            for (uint32_t k = 1; k <= kSkipHalo; ++k)
+                t |= (((0xffu << k) & 0xffu) * 0x01010101u) & (s_in[j - k][col] | s_in[j + k][col]);
+            const uint64_t c = plane + (uint64_t)q * step;
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) {
+                const uint32_t a = a0 + 4u * col + i;
+                if (a < span)
+                    out[c + a] = (uint8_t)(t >> (8u * i));
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // 1 (LIST = false): per 4096-cell tile, the sub-cells and dense cells it holds.
 // 3 (LIST = true): per tile, every cell's 8-byte record -- {0x80000000 | first
 // sub-cell, plan} for a cell with a sub-grid, else {refs begin, end} -- and
@@ -88,6 +196,7 @@ __global__ __launch_bounds__(256) void k_g2_plan(G2Build b)
 {
     __shared__ uint32_t s_tot[2][64];
     __shared__ uint32_t s_scan[8];
+    __shared__ uint64_t s_occ[kScanTile / 64u + 2u];   // occupancy, 64 cells either side
     const DevFrame* fr = b.fr;
     if (!top_ok(fr))
         return;
@@ -102,18 +211,43 @@ __global__ __launch_bounds__(256) void k_g2_plan(G2Build b)
     }
     const bool sub_ok = !LIST || b.g2->status == 0;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t packed[16], ms[16], nd = 0, msum = 0;
+    uint32_t packed[16], ms[16], nd = 0, msum = 0, empty = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) {
         const uint32_t c = base + 256u * k + threadIdx.x;
+        const uint32_t n = c < m_top ? b.offsets[c + 1] - b.offsets[c] : 0u;
         ms[k] = 0;
-        packed[k] = (c < m_top && sub_ok)
-                        ? g2_plan(fr, b.offsets[c + 1] - b.offsets[c], b.lambda2, b.min_refs, ms[k])
-                        : 0u;
+        packed[k] = sub_ok ? g2_plan(fr, n, b.lambda2, b.min_refs, ms[k]) : 0u;
+        empty |= (c < m_top && n == 0u ? 1u : 0u) << k;
         nd += packed[k] ? 1u : 0u;
         msum += ms[k];
+        if (!LIST) {
+            const uint64_t w = __ballot(n != 0u);   // cells base + 64 (4 k + wave) ...
+            if (lane == 0u)
+                s_occ[1u + 4u * k + wave] = w;
+        }
     }
     if (!LIST) {
+        // D_x of the tile's cells: the occupancy of 64 cells either side too
+        if (wave < 2u) {
+            const int64_t c = wave == 0u ? (int64_t)base - 64 + (int64_t)lane
+                                         : (int64_t)base + kScanTile + lane;
+            const bool occ = c >= 0 && c < (int64_t)m_top && b.offsets[c + 1] != b.offsets[c];
+            const uint64_t w = __ballot(occ);
+            if (lane == 0u)
+                s_occ[wave == 0u ? 0u : kScanTile / 64u + 1u] = w;
+        }
+        __syncthreads();
+        const uint32_t n0 = (uint32_t)fr->n[0], inc = 256u % n0;
+        uint32_t x = (base + threadIdx.x) % n0;
+        for (uint32_t k = 0; k < 16; ++k) {
+            const uint32_t c = base + 256u * k + threadIdx.x;
+            if (c < m_top)   // thermometer byte: bit t set when D_x <= t
+                b.skip[c] =
+                    (uint8_t)(0xffu << row_distance(s_occ, 256u * k + threadIdx.x + 64u, x, n0));
+            x += inc;
+            x -= x >= n0 ? n0 : 0u;
+        }
         uint32_t nd_total, m_total;
         block_excl_scan<256>(nd, s_scan, nd_total);
         block_excl_scan<256>(msum, s_scan, m_total);
@@ -153,6 +287,11 @@ __global__ __launch_bounds__(256) void k_g2_plan(G2Build b)
                 const uint32_t mpos = tm + s_tot[0][4u * k + wave] + ms[k];
                 b.cellrec[c] = make_uint2(0x80000000u | mpos, packed[k]);
                 b.dense[td + s_tot[1][4u * k + wave] + ed[k]] = c;
+            } else if ((empty >> k) & 1u) {
+                // skip radius D - 1, D the lowest bit of the thermometer byte (8: none)
+                const uint32_t t = b.skip[c];
+                const uint32_t r = (t ? (uint32_t)__ffs(t) - 1u : (uint32_t)kSkipMax + 1u) - 1u;
+                b.cellrec[c] = make_uint2(r, r);
             } else {
                 b.cellrec[c] = make_uint2(b.offsets[c], b.offsets[c + 1]);
             }
@@ -476,146 +615,21 @@ __global__ __launch_bounds__(1024) void k_g2_scan_dense(G2Build b, uint32_t sub_
 hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
                              uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s)
 {
+    if (b.skip_stride < top_cells_cap)
+        return hipErrorInvalidValue;
     const uint32_t nt = top_cells_cap / kScanTile + 1;
     hipLaunchKernelGGL(k_g2_plan<false>, dim3(nt), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_g2_scan_tiles, dim3(1), dim3(1024), 0, s, b, nt, sub_cells_cap,
                        dense_cap);
+    const uint32_t nsk = top_cells_cap / 1024u + 1u;
+    const dim3 gsk(nsk < 4096u ? nsk : 4096u);
+    hipLaunchKernelGGL(k_g2_skip_line<1>, gsk, dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_g2_skip_line<2>, gsk, dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_g2_plan<true>, dim3(nt), dim3(256), 0, s, b);
     const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
     hipLaunchKernelGGL(k_g2_count, dim3(nb), dim3(1024), 0, s, b);
     hipLaunchKernelGGL(k_g2_scan_dense, dim3(1), dim3(1024), 0, s, b, sub_refs_cap);
     return hipGetLastError();
-}
-
-// Skip radii: the Chebyshev distance D from each top cell to the nearest cell
-// holding a reference (cells outside the grid hold none), capped at C =
-// kSkipMax + 1, axis by axis -- D_x along the cell's x row, then D_xy =
-// min_k max(|k|, D_x(y + k)), then D = min_k max(|k|, D_xy(z + k)) -- so an
-// empty cell's radius is D - 1.  Each pass stages its tile plus a halo of
-// kSkipMax cells per side in LDS; a cell's scan stops once |k| reaches the
-// best distance found.  The frame's dimensions are device values, so the
-// launches are fixed grids that loop over the tiles.
-constexpr uint32_t kSkipTileX = 1024;   // cells per x-pass tile
-constexpr uint32_t kSkipHalo = (uint32_t)kSkipMax;
-constexpr uint32_t kSkipRows = 64;      // rows per line-pass tile (64 lines wide)
-
-// x pass: occupancy bits of a tile of consecutive cells and 64 cells either
-// side (one ballot per wave), then per cell the nearest occupied cell of its
-// row within kSkipMax from the 15-bit window around it (rows may change
-// inside the tile: the window is masked to the row).
-__global__ __launch_bounds__(256) void k_g2_skip_x(G2Build b)
-{
-    constexpr uint32_t W = kSkipTileX / 64u + 2u;   // words: the tile and one either side
-    __shared__ uint64_t s_occ[W + 1];
-    const DevFrame* fr = b.fr;
-    if (!top_ok(fr))
-        return;
-    const uint32_t m = fr->m, n0 = (uint32_t)fr->n[0];
-    constexpr uint32_t C = (uint32_t)kSkipMax + 1u;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t tiles = (m + kSkipTileX - 1u) / kSkipTileX;
-    for (uint32_t w = blockIdx.x; w < tiles; w += gridDim.x) {
-        const int64_t base = (int64_t)w * kSkipTileX - 64;
-        for (uint32_t q = wave; q < W; q += 4u) {
-            const int64_t c = base + 64 * (int64_t)q + lane;
-            const bool occ = c >= 0 && c < (int64_t)m && b.offsets[c + 1] != b.offsets[c];
-            const uint64_t bits = __ballot(occ);
-            if (lane == 0u)
-                s_occ[q] = bits;
-        }
-        if (threadIdx.x == 0u)
-            s_occ[W] = 0ull;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < kSkipTileX; i += 256u) {
-            const uint32_t c = w * kSkipTileX + i;
-            if (c >= m)
-                break;
-            // bits p - 7 .. p + 7 of the tile's bit array, p = the cell's bit
-            const uint32_t lo = i + 64u - kSkipHalo;
-            const uint32_t sh = lo & 63u;
-            const uint64_t w0 = s_occ[lo >> 6], w1 = s_occ[(lo >> 6) + 1u];
-            const uint32_t win = (uint32_t)((sh ? (w0 >> sh) | (w1 << (64u - sh)) : w0) & 0x7fffu);
-            const uint32_t x = c % n0, right = n0 - 1u - x;
-            uint32_t d = C;
-            if (win & (1u << kSkipHalo)) {
-                d = 0u;
-            } else {
-                // below: bit j is the cell k = 7 - j to the left, within the row
-                const uint32_t below =
-                    win & (x >= kSkipHalo ? 0x7fu : (0x7fu << (kSkipHalo - x)) & 0x7fu);
-                if (below)
-                    d = kSkipHalo - (31u - (uint32_t)__clz(below));
-                // above: bit i is the cell k = i + 1 to the right, within the row
-                const uint32_t above =
-                    (win >> (kSkipHalo + 1u)) & (right >= kSkipHalo ? 0x7fu : (1u << right) - 1u);
-                if (above) {
-                    const uint32_t ka = (uint32_t)__ffs(above);
-                    d = ka < d ? ka : d;
-                }
-            }
-            b.skip[c] = (uint8_t)d;
-        }
-        __syncthreads();
-    }
-}
-
-// y (AXIS 1) or z (AXIS 2) pass over tiles of 64 lines x kSkipRows rows: the
-// lines of a tile start at 64 consecutive cells of coordinate 0 on the axis
-// (x of one z plane for y; x + n0 y for z).  The z pass writes the radius of
-// every empty cell into its record as {D - 1, D - 1}.
-template <int AXIS>
-__global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
-{
-    __shared__ uint8_t s_in[kSkipRows + 2 * kSkipHalo][64];
-    const DevFrame* fr = b.fr;
-    if (!top_ok(fr))
-        return;
-    constexpr uint32_t C = (uint32_t)kSkipMax + 1u;
-    const uint32_t n0 = (uint32_t)fr->n[0], n1 = (uint32_t)fr->n[1], n2 = (uint32_t)fr->n[2];
-    const uint32_t step = AXIS == 1 ? n0 : n0 * n1;        // cell stride along the axis
-    const uint32_t len = AXIS == 1 ? n1 : n2;
-    const uint32_t span = AXIS == 1 ? n0 : n0 * n1;        // line starts per plane
-    const uint32_t planes = AXIS == 1 ? n2 : 1u;
-    // rows per tile: the axis when it is short, else kSkipRows
-    const uint32_t rows = len < kSkipRows ? ((len + 3u) & ~3u) : kSkipRows;
-    const uint32_t gx = (span + 63u) / 64u, segs = (len + rows - 1u) / rows;
-    const uint64_t work = (uint64_t)gx * planes * segs;
-    const uint8_t* in = AXIS == 1 ? b.skip : b.skip + b.skip_stride;
-    const uint32_t lane = threadIdx.x & 63u, row0 = threadIdx.x >> 6;
-    for (uint64_t w = blockIdx.x; w < work; w += gridDim.x) {
-        const uint32_t seg = (uint32_t)(w % segs);
-        const uint64_t g = w / segs;
-        const uint32_t a = (uint32_t)(g % gx) * 64u + lane;
-        const bool lane_ok = a < span;
-        const uint64_t start = (uint64_t)a + (uint64_t)(g / gx) * n0 * n1;
-        const int64_t q0 = (int64_t)seg * rows - (int64_t)kSkipHalo;
-        for (uint32_t r = row0; r < rows + 2u * kSkipHalo; r += 4u) {
-            const int64_t q = q0 + (int64_t)r;
-            s_in[r][lane] = lane_ok && q >= 0 && q < (int64_t)len
-                                ? in[start + (uint64_t)q * step]
-                                : (uint8_t)C;
-        }
-        __syncthreads();
-        for (uint32_t r = row0; r < rows; r += 4u) {
-            const uint32_t q = seg * rows + r;
-            if (!lane_ok || q >= len)
-                continue;
-            const uint32_t j = r + kSkipHalo;
-            uint32_t d = s_in[j][lane];
-            for (uint32_t k = 1; k < d; ++k) {
-                const uint32_t lo = s_in[j - k][lane], hi = s_in[j + k][lane];
-                const uint32_t v = lo < hi ? lo : hi;
-                const uint32_t cand = v > k ? v : k;
-                d = cand < d ? cand : d;
-            }
-            const uint64_t c = start + (uint64_t)q * step;
-            if (AXIS == 1)
-                b.skip[b.skip_stride + c] = (uint8_t)d;
-            else if (d > 0u)
-                b.cellrec[c] = make_uint2(d - 1u, d - 1u);
-        }
-        __syncthreads();
-    }
 }
 
 hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
@@ -625,13 +639,6 @@ hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t
     hipError_t e0 = launch_grid2_plan(b, top_cells_cap, sub_cells_cap, sub_refs_cap, dense_cap, s);
     if (e0 != hipSuccess)
         return e0;
-    if (b.skip_stride < top_cells_cap)
-        return hipErrorInvalidValue;
-    const uint32_t nsk = top_cells_cap / kSkipTileX + 1u;
-    const dim3 gsk(nsk < 4096u ? nsk : 4096u);
-    hipLaunchKernelGGL(k_g2_skip_x, gsk, dim3(256), 0, s, b);
-    hipLaunchKernelGGL(k_g2_skip_line<1>, gsk, dim3(256), 0, s, b);
-    hipLaunchKernelGGL(k_g2_skip_line<2>, gsk, dim3(256), 0, s, b);
     const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
     hipLaunchKernelGGL(k_g2_fill, dim3(nb), dim3(1024), 0, s, b);
     const uint32_t nt = sub_cells_cap / 256u + 1u;

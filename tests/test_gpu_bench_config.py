@@ -250,7 +250,7 @@ def test_bench_frame_four_stages(gpu_ctx, stadium, lam):
 
 # Two-level grids (top lambda, lambda2, minimum references of a subdivided
 # cell): the two the frame autotuner chooses among and two neighbours.
-TWO_LEVEL = [(8.0, 4.0, 8), (16.0, 4.0, 8), (8.0, 8.0, 8), (4.0, 4.0, 8)]
+TWO_LEVEL = [(8.0, 4.0, 8), (16.0, 4.0, 8), (16.0, 4.0, 4), (8.0, 8.0, 8), (4.0, 4.0, 8)]
 
 
 def _two_level_stage(ctx, sc, rays, fr, offs, refs, g2, ov, closest, tag, lst, count):
