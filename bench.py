@@ -85,6 +85,9 @@ def parse():
                         "batching and ambient-occlusion workloads on their own; kdtree: the "
                         "kd-tree comparison accelerator against the grid on the stadium frame")
     p.add_argument("--no-serve", action="store_true", help="frame mode without the latency run")
+    p.add_argument("--sequential", action="store_true",
+                   help="timed frames run every stage on one stream (default: the shadow and "
+                        "bounce branches run on two streams)")
     p.add_argument("--serve-requests", type=int, default=120)
     p.add_argument("--profile", action="store_true",
                    help="only the warmup and timed frames (with --lam and --sched: no "
@@ -186,16 +189,40 @@ class Frame:
         d = np.median([intervals(ev) for ev in evs], axis=0)
         return [d[0], d[1], d[2] + d[3], d[4] + d[5], d[6] + d[7], d[8] + d[9]]
 
+    def time_frames(self, frames=5, concurrent=False):
+        """Wall milliseconds per frame over `frames` frames after one warm-up."""
+        self.run(3000, concurrent=concurrent)
+        self.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            self.run(3001 + f, concurrent=concurrent)
+        self.torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / frames
+
     def transforms(self, frame):
         if frame not in self.xf:
             self.xf[frame] = self.grt.gen_cluster_transforms(SEED, frame, self.centers, 0.05)
         return self.xf[frame]
 
-    def run(self, frame, events=None):
-        """One frame.  Events (if given, N_EVENTS of them on the tracing
-        stream) bracket every launch group: transform, build, for each of the
-        four ray stages its generation kernel and its traversal kernel, and the
-        Phong shading of both levels into the RGBA8 frame."""
+    def concurrent_ok(self):
+        """Whether the shadow and bounce stages may run on two streams: the
+        persistent schedule of the one-level grid keeps its slot counter in
+        the scene, so two such launches at once would share it."""
+        return self.sub is not None or "p" not in self.sched
+
+    def run(self, frame, events=None, concurrent=False):
+        """One frame.  Events (if given, N_EVENTS of them, each recorded on
+        the stream of the launches it brackets) bracket every launch group:
+        transform, build, for each of the four ray stages its generation
+        kernel and its traversal kernel, and the Phong shading of both levels
+        into the RGBA8 frame.  Concurrent: the shadow rays are traced on the
+        context's stream while the bounce rays, their shadow rays and both
+        traces run on a second stream (the two branches depend only on the
+        primary hits); the streams join before shading.  Both ray lists of
+        the first two secondary stages are made before the branches split
+        (the compaction's block sums live in the context)."""
+        if concurrent:
+            return self.run_concurrent(frame, events)
         sc, W = self.scene, WIDTH
         ev = events if events is not None else [None] * self.N_EVENTS
 
@@ -244,6 +271,66 @@ class Frame:
         sc.shade_phong(self.prim, self.hits, self.rgb, self.occ, 0, self.phong)
         sc.shade_phong(self.refl, self.rhits, self.rgb, self.occ2, 1, self.phong, rgba=self.rgba)
         mark(11)
+
+    def run_concurrent(self, frame, events=None):
+        sc, W, torch = self.scene, WIDTH, self.torch
+        main = torch.cuda.current_stream()
+        if not hasattr(self, "side"):
+            self.side = torch.cuda.Stream()      # default priority
+        side = self.side
+        ev = events if events is not None else [None] * self.N_EVENTS
+
+        def mark(i, stream):
+            if ev[i] is not None:
+                ev[i].record(stream)
+
+        L, C, K = self.lists, self.counts, self.lists_kind
+        mark(0, main)
+        sc.transform_clusters(self.transforms(frame), TRIS_PER_CLUSTER, N_CLUSTER_TRIS)
+        mark(1, main)
+        sc.rebuild_grid_async(self.lam)
+        mark(2, main)
+        self.ctx.gen_primary(self.cam, self.prim)
+        self.mode(0)
+        mark(3, main)
+        sc.trace_closest(self.prim, self.hits, image_width=W)
+        mark(4, main)
+        sc.gen_shadow(self.prim, self.hits, LIGHT, OFFSET, self.shadow)
+        self.make_list(0, self.shadow, occluded=self.occ)
+        sc.gen_reflection(self.prim, self.hits, OFFSET, self.refl)
+        self.make_list(1, self.refl, miss_hits=self.rhits)
+        side.wait_stream(main)
+        self.mode(1)
+        mark(5, main)
+        if K[0] != "-":
+            sc.trace_any_list(self.shadow, self.occ, L[0], C[0])
+        else:
+            sc.trace_any(self.shadow, self.occ, image_width=W)
+        mark(6, main)
+        self.ctx.set_stream(side.cuda_stream)
+        try:
+            self.mode(2)
+            mark(7, side)
+            if K[1] != "-":
+                sc.trace_closest_list(self.refl, self.rhits, L[1], C[1])
+            else:
+                sc.trace_closest(self.refl, self.rhits, image_width=W)
+            mark(8, side)
+            sc.gen_shadow(self.refl, self.rhits, LIGHT, OFFSET, self.shadow2)
+            self.make_list(2, self.shadow2, occluded=self.occ2)
+            self.mode(3)
+            mark(9, side)
+            if K[2] != "-":
+                sc.trace_any_list(self.shadow2, self.occ2, L[2], C[2])
+            else:
+                sc.trace_any(self.shadow2, self.occ2, image_width=W)
+            mark(10, side)
+        finally:
+            self.ctx.set_stream(main.cuda_stream)
+        main.wait_stream(side)
+        sc.shade_phong(self.prim, self.hits, self.rgb, self.occ, 0, self.phong)
+        sc.shade_phong(self.refl, self.rhits, self.rgb, self.occ2, 1, self.phong, rgba=self.rgba)
+        mark(11, main)
 
     def make_list(self, k, rays, **out):
         kind = self.lists_kind[k]
@@ -517,25 +604,37 @@ def main():
     frame.scene.check_grid()
     torch.cuda.synchronize()
 
+    concurrent = not args.sequential and frame.concurrent_ok()
+    for w in range(2):
+        frame.run(1100 + w, concurrent=concurrent)
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(Frame.N_EVENTS)]
               for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        frame.run(s, events[s])
+        frame.run(s, events[s], concurrent=concurrent)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     frame.scene.check_grid()
     ms_per_step = wall * 1e3 / args.steps
     # Mean over the timed frames of each launch group (HIP events recorded on
-    # the stream the kernels run on: the context enqueues on torch's stream).
-    d = np.mean([intervals(ev) for ev in events], axis=0)
-    trace_ms = {"primary": d[3], "shadow": d[5], "bounce": d[7], "bounce_shadow": d[9]}
-    stage_ms = {"transform": d[0], "build": d[1], "raygen": d[2] + d[4] + d[6] + d[8],
-                **trace_ms, "shade": d[10]}
+    # the stream each launch runs on).  With the two branches concurrent, a
+    # traversal launch's interval is its own duration while sharing the GPU.
+    def span(ev, a, b):
+        return ev[a].elapsed_time(ev[b])
+    d = {k: float(np.mean([span(ev, a, b) for ev in events]))
+         for k, (a, b) in {"transform": (0, 1), "build": (1, 2), "gen0": (2, 3), "primary": (3, 4),
+                           "gen1": (4, 5), "shadow": (5, 6), "gen2": (6, 7), "bounce": (7, 8),
+                           "gen3": (8, 9), "bounce_shadow": (9, 10), "shade": (10, 11)}.items()}
+    if concurrent:
+        d["gen2"] = 0.0    # the reflection rays are made before the branches split
+    trace_ms = {k: d[k] for k in ("primary", "shadow", "bounce", "bounce_shadow")}
+    stage_ms = {"transform": d["transform"], "build": d["build"],
+                "raygen": d["gen0"] + d["gen1"] + d["gen2"] + d["gen3"],
+                **trace_ms, "shade": d["shade"]}
     if args.profile:
         print(json.dumps({"mode": "profile", "grid": tag, "schedule": sched, "coop": args.coop,
-                          "lists": frame.lists_kind,
+                          "lists": frame.lists_kind, "concurrent_branches": concurrent,
                           "ms_per_step": round(ms_per_step, 3),
                           "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}}))
         return
@@ -620,7 +719,12 @@ def main():
             "stage_counters": counters,
             "traversal": {"schedule_per_stage": sched, "refill": args.refill,
                           "block": args.block or 128, "grid": tag, "coop": args.coop,
-                          "secondary_ray_lists": frame.lists_kind},
+                          "secondary_ray_lists": frame.lists_kind,
+                          "concurrent_branches": concurrent},
+            "stage_ms_note": ("shadow and bounce branches run on two streams: a traversal "
+                              "stage's time is its launch duration while sharing the GPU, "
+                              "and the stages overlap" if concurrent else
+                              "stages run one after another on one stream"),
             "autotune_frame_ms_by_grid": tune_table,
             "parallelism": "single GPU",
         },
@@ -651,6 +755,8 @@ def main():
             out["config"]["one_level_comparison"] = {
                 "grid": tag1, "schedule_per_stage": e1["sched"], "lists": e1["lists"],
                 "frame_ms": round(sum(ms1), 3),
+                "frame_ms_wall": round(frame.time_frames(5, concurrent and
+                                                         frame.concurrent_ok()), 3),
                 "stage_ms": {n: round(v, 3) for n, v in zip(frame.stage_names, ms1)},
                 "primary_shadow_algorithmic_bytes": b1,
                 "roofline_primary_shadow": {"gbs": round(gbs1, 1),
@@ -658,7 +764,9 @@ def main():
                 "mean_cells_tests": {n: [c1[n]["mean_cells"], c1[n]["mean_tests"]]
                                      for n in c1},
                 "note": "stage times include ray generation and list compaction, as in "
-                        "the tuning table"}
+                        "the tuning table; frame_ms_wall: whole frames timed like the "
+                        "benchmarked ones (same branch concurrency when the schedule "
+                        "allows it)"}
             frame.set_grid(lam, sub)
             frame.sched, frame.lists_kind = sched, lists
     if not args.no_serve:

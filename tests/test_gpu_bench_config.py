@@ -367,3 +367,44 @@ def test_bench_two_level_frame(gpu_ctx, stadium, lam, lam2, min_refs):
                          f"bounce shadow {tag}", lst, count)
     finally:
         sc.close()
+
+
+@pytest.mark.parametrize("grid,sched,lists", [((16.0, (4.0, 8)), "ss24", "--c"),
+                                              ((8.0, (4.0, 8)), "4448", "ccc")])
+def test_bench_frame_branches_concurrent_same_as_sequential(gpu_ctx, grid, sched, lists):
+    """The benchmarked frame with its shadow and bounce branches on two streams
+    (the bounce rays, their shadow rays, list compaction and both traces on a
+    second stream) writes exactly what the one-stream frame writes: hits,
+    occlusion flags of both levels and the RGBA8 image, for two frames."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    lam, sub = grid
+    fr = bench.Frame(grt, torch, lam=lam, sched=sched, sub=sub)
+    fr.lists_kind = lists
+    assert fr.concurrent_ok()
+    try:
+        for f in (1, 2):
+            fr.run(f)
+            torch.cuda.synchronize()
+            want = {"hits": fr.hits.numpy(), "rhits": fr.rhits.numpy(),
+                    "occ": fr.occ.cpu().numpy(), "occ2": fr.occ2.cpu().numpy(),
+                    "rgba": fr.rgba.cpu().numpy()}
+            for t in (fr.occ, fr.occ2, fr.rgba):
+                t.fill_(0x5a)
+            fr.run(f, concurrent=True)
+            torch.cuda.synchronize()
+            got = {"hits": fr.hits.numpy(), "rhits": fr.rhits.numpy(),
+                   "occ": fr.occ.cpu().numpy(), "occ2": fr.occ2.cpu().numpy(),
+                   "rgba": fr.rgba.cpu().numpy()}
+            tag = f"[{lam}/{sub} {sched} {lists} frame {f}]"
+            for k in ("hits", "rhits"):
+                for fld in ("id", "t", "u", "v"):
+                    assert_same(got[k][fld], want[k][fld], f"{k}.{fld} {tag}")
+            for k in ("occ", "occ2", "rgba"):
+                assert_same(got[k], want[k], f"{k} {tag}")
+            assert (want["occ2"] != 0).any() and (want["hits"]["id"] >= 0).mean() > 0.5
+    finally:
+        fr.scene.close()
+        fr.ctx.close()
