@@ -188,6 +188,22 @@ static int enqueue_build_passes(grt_scene_t* sc, uint32_t refs_cap, bool with_li
     return enqueue_scan(sc, refs_cap, with_lists, false);
 }
 
+static int enqueue_scatter_sort_only(grt_scene_t* sc)
+{
+    hipStream_t s = sc->ctx->stream;
+    DevFrame* fr = sc->frame.as<DevFrame>();
+    int rc;
+    if ((rc = hip_check(launch_grid_scatter(fr, sc->verts.as<float>(), sc->n_tris,
+                                            sc->counts.as<uint32_t>(), sc->offsets.as<uint32_t>(),
+                                            sc->refs.as<uint32_t>(), sc->big_list.as<uint32_t>(),
+                                            sc->small_counters.as<uint32_t>(), s),
+                        "grid scatter")) != GRT_OK)
+        return rc;
+    return hip_check(launch_grid_sort(fr, sc->offsets.as<uint32_t>(), sc->refs.as<uint32_t>(),
+                                      sort_lists(sc), s),
+                     "grid sort");
+}
+
 static int enqueue_scatter_sort(grt_scene_t* sc)
 {
     hipStream_t s = sc->ctx->stream;
@@ -311,13 +327,18 @@ static int g2_reserve_sub(grt_scene_t* sc, uint32_t cells, uint32_t refs)
 
 // Enqueue the second level after a top build on the same stream (no host
 // synchronisation); skipped when its buffers do not cover the top capacity.
+// Whether the second level is on and its buffers cover the top capacity.
+static bool g2_ready(const grt_scene_t* sc)
+{
+    return sc->g2_lambda > 0.0f && sc->g2_top_cap >= sc->cells_cap &&
+           sc->g2_box_cap >= sc->refs_cap && sc->sub_offsets.ptr && sc->sub_refs.ptr &&
+           sc->g2dense.ptr && sc->g2big.ptr && sc->g2long.ptr;
+}
+
 static int g2_enqueue(grt_scene_t* sc)
 {
     sc->g2_built = false;
-    if (!(sc->g2_lambda > 0.0f))
-        return GRT_OK;
-    if (sc->g2_top_cap < sc->cells_cap || sc->g2_box_cap < sc->refs_cap || !sc->sub_offsets.ptr ||
-        !sc->sub_refs.ptr || !sc->g2dense.ptr || !sc->g2big.ptr || !sc->g2long.ptr)
+    if (!g2_ready(sc))
         return GRT_OK;
     const G2Build b = g2_args(sc, true);
     const int rc = hip_check(launch_grid2_build(b, sc->cells_cap, sc->g2_cells_cap,
@@ -436,6 +457,13 @@ int grt_shutdown(grt_context ctx)
         grt_scene_destroy(sc);
     ctx->instance_scratch.release(ctx);
     ctx->compact_scratch.release(ctx);
+    if (ctx->aux_stream) {
+        (void)hipStreamSynchronize(ctx->aux_stream);
+        (void)hipStreamDestroy(ctx->aux_stream);
+    }
+    for (hipEvent_t ev : ctx->aux_ev)
+        if (ev)
+            (void)hipEventDestroy(ev);
     if (ctx->own_stream)
         (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -807,6 +835,98 @@ int grt_grid_reserve(grt_scene sc, uint32_t cells, uint32_t refs)
     return rc;
 }
 
+static bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+static int aux_stream_ready(grt_context_t* ctx)
+{
+    if (ctx->aux_stream)
+        return GRT_OK;
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess)
+        return hip_check(e, "hipStreamCreate");
+    for (hipEvent_t& ev : ctx->aux_ev) {
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+            for (hipEvent_t& done : ctx->aux_ev) {
+                if (done)
+                    (void)hipEventDestroy(done);
+                done = nullptr;
+            }
+            (void)hipStreamDestroy(st);
+            return hip_check(e, "hipEventCreate");
+        }
+    }
+    ctx->aux_stream = st;
+    return GRT_OK;
+}
+
+// The rest of an asynchronous rebuild of both levels, after the top scan, on
+// two streams.  What the second level plans from the top grid's offsets
+// alone (cell plans, skip radii, cell records) runs on the auxiliary stream
+// while the main stream scatters and sorts the top references; the top
+// records are gathered on the auxiliary stream while the main stream builds
+// the sub-grids from the sorted references.  The main stream waits for the
+// auxiliary one at the end, so callers see one stream-ordered rebuild.
+static int rebuild_two_streams_enqueue(grt_scene_t* sc);
+
+static int rebuild_two_streams(grt_scene_t* sc)
+{
+    const int rc = rebuild_two_streams_enqueue(sc);
+    if (rc != GRT_OK) {
+        // keep the callers' single-stream view even after a failed enqueue
+        grt_context_t* ctx = sc->ctx;
+        if (hipEventRecord(ctx->aux_ev[3], ctx->aux_stream) == hipSuccess)
+            (void)hipStreamWaitEvent(ctx->stream, ctx->aux_ev[3], 0);
+    }
+    return rc;
+}
+
+static int rebuild_two_streams_enqueue(grt_scene_t* sc)
+{
+    grt_context_t* ctx = sc->ctx;
+    hipStream_t s = ctx->stream, aux = ctx->aux_stream;
+    hipEvent_t* ev = ctx->aux_ev;
+    sc->g2_built = false;
+    const G2Build b = g2_args(sc, true);
+    int rc;
+    // offsets ready -> the plan on aux
+    if ((rc = hip_check(hipEventRecord(ev[0], s), "hipEventRecord")) != GRT_OK ||
+        (rc = hip_check(hipStreamWaitEvent(aux, ev[0], 0), "hipStreamWaitEvent")) != GRT_OK ||
+        (rc = hip_check(launch_grid2_plan_cells(b, sc->cells_cap, sc->g2_cells_cap,
+                                                sc->g2_dense_cap, aux),
+                        "sub-grid plan")) != GRT_OK ||
+        (rc = hip_check(hipEventRecord(ev[1], aux), "hipEventRecord")) != GRT_OK)
+        return rc;
+    // top references scattered and sorted on the main stream
+    if ((rc = enqueue_scatter_sort_only(sc)) != GRT_OK)
+        return rc;
+    // sorted references -> the top records on aux
+    if ((rc = hip_check(hipEventRecord(ev[2], s), "hipEventRecord")) != GRT_OK ||
+        (rc = hip_check(hipStreamWaitEvent(aux, ev[2], 0), "hipStreamWaitEvent")) != GRT_OK ||
+        (rc = hip_check(launch_gather_refs(sc->frame.as<DevFrame>(), sc->refs.as<uint32_t>(),
+                                           sc->tri.as<float4>(), sc->trir.as<float4>(),
+                                           sc->refs_cap, aux),
+                        "grid records")) != GRT_OK ||
+        (rc = hip_check(hipEventRecord(ev[3], aux), "hipEventRecord")) != GRT_OK)
+        return rc;
+    // the sub-grids from the plan and the sorted references on the main stream
+    if ((rc = hip_check(hipStreamWaitEvent(s, ev[1], 0), "hipStreamWaitEvent")) != GRT_OK ||
+        (rc = hip_check(launch_grid2_plan_refs(b, sc->g2_refs_cap, sc->g2_dense_cap, s),
+                        "sub-grid plan")) != GRT_OK ||
+        (rc = hip_check(launch_grid2_fill(b, sc->g2_cells_cap, sc->g2_refs_cap, sc->g2_dense_cap,
+                                          sc->tri.as<float4>(), sc->sub_trir.as<float4>(), s),
+                        "sub-grid build")) != GRT_OK)
+        return rc;
+    if ((rc = hip_check(hipStreamWaitEvent(s, ev[3], 0), "hipStreamWaitEvent")) != GRT_OK)
+        return rc;
+    sc->g2_built = true;
+    return GRT_OK;
+}
+
 int grt_grid_rebuild_async(grt_scene sc, float lambda)
 {
     if (!sc)
@@ -827,11 +947,13 @@ int grt_grid_rebuild_async(grt_scene sc, float lambda)
         return rc;
     if ((rc = enqueue_build_passes(sc, sc->refs_cap, true)) != GRT_OK)
         return rc;
-    if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
-        return rc;
     sc->lambda = lambda;
     sc->grid_valid = true;
     sc->info_stale = true;
+    if (g2_ready(sc) && !stream_capturing(s) && aux_stream_ready(sc->ctx) == GRT_OK)
+        return rebuild_two_streams(sc);
+    if ((rc = enqueue_scatter_sort(sc)) != GRT_OK)
+        return rc;
     return g2_enqueue(sc);
 }
 

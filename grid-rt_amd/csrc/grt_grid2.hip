@@ -612,8 +612,8 @@ __global__ __launch_bounds__(1024) void k_g2_scan_dense(G2Build b, uint32_t sub_
     }
 }
 
-hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
-                             uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s)
+hipError_t launch_grid2_plan_cells(const G2Build& b, uint32_t top_cells_cap,
+                                   uint32_t sub_cells_cap, uint32_t dense_cap, hipStream_t s)
 {
     if (b.skip_stride < top_cells_cap)
         return hipErrorInvalidValue;
@@ -626,19 +626,29 @@ hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t 
     hipLaunchKernelGGL(k_g2_skip_line<1>, gsk, dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_g2_skip_line<2>, gsk, dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_g2_plan<true>, dim3(nt), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid2_plan_refs(const G2Build& b, uint32_t sub_refs_cap, uint32_t dense_cap,
+                                  hipStream_t s)
+{
     const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
     hipLaunchKernelGGL(k_g2_count, dim3(nb), dim3(1024), 0, s, b);
     hipLaunchKernelGGL(k_g2_scan_dense, dim3(1), dim3(1024), 0, s, b, sub_refs_cap);
     return hipGetLastError();
 }
 
-hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
-                              uint32_t sub_refs_cap, uint32_t dense_cap, const float4* tri,
-                              float4* sub_trir, hipStream_t s)
+hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
+                             uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s)
 {
-    hipError_t e0 = launch_grid2_plan(b, top_cells_cap, sub_cells_cap, sub_refs_cap, dense_cap, s);
-    if (e0 != hipSuccess)
-        return e0;
+    const hipError_t e = launch_grid2_plan_cells(b, top_cells_cap, sub_cells_cap, dense_cap, s);
+    return e != hipSuccess ? e : launch_grid2_plan_refs(b, sub_refs_cap, dense_cap, s);
+}
+
+hipError_t launch_grid2_fill(const G2Build& b, uint32_t sub_cells_cap, uint32_t sub_refs_cap,
+                             uint32_t dense_cap, const float4* tri, float4* sub_trir,
+                             hipStream_t s)
+{
     const uint32_t nb = dense_cap < 2048u ? (dense_cap ? dense_cap : 1u) : 2048u;
     hipLaunchKernelGGL(k_g2_fill, dim3(nb), dim3(1024), 0, s, b);
     const uint32_t nt = sub_cells_cap / 256u + 1u;
@@ -651,6 +661,17 @@ hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t
     if (e != hipSuccess)
         return e;
     return launch_gather_refs(b.g2, b.sub_refs, tri, sub_trir, sub_refs_cap, s);
+}
+
+hipError_t launch_grid2_build(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
+                              uint32_t sub_refs_cap, uint32_t dense_cap, const float4* tri,
+                              float4* sub_trir, hipStream_t s)
+{
+    const hipError_t e = launch_grid2_plan(b, top_cells_cap, sub_cells_cap, sub_refs_cap,
+                                           dense_cap, s);
+    return e != hipSuccess ? e
+                           : launch_grid2_fill(b, sub_cells_cap, sub_refs_cap, dense_cap, tri,
+                                               sub_trir, s);
 }
 
 }  // namespace grt

@@ -64,6 +64,10 @@ struct grt_context_t {
     std::set<grt_scene_t*> scenes;
     grt::DevMem instance_scratch;   // object-space rays and candidate hits of instance tracing
     grt::DevMem compact_scratch;    // block sums of ray-list compaction
+    // A second stream (default priority) and events for the parts of an
+    // asynchronous rebuild that run beside the main stream; created on first use.
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t aux_ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 struct grt_scene_t {

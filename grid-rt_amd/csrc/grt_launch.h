@@ -121,6 +121,17 @@ struct G2Build {
 // kScanTile + 1 entries each.
 hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
                              uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s);
+// The plan in two parts: what needs only the top grid's offsets (cell plans,
+// tile sums, skip radii, cell records, the dense list) and what needs its
+// references (per dense cell: boxes, sub-cell counts; the dense scan).
+hipError_t launch_grid2_plan_cells(const G2Build& b, uint32_t top_cells_cap,
+                                   uint32_t sub_cells_cap, uint32_t dense_cap, hipStream_t s);
+hipError_t launch_grid2_plan_refs(const G2Build& b, uint32_t sub_refs_cap, uint32_t dense_cap,
+                                  hipStream_t s);
+// After the plan: sub-cell offsets and references, their sorts and records.
+hipError_t launch_grid2_fill(const G2Build& b, uint32_t sub_cells_cap, uint32_t sub_refs_cap,
+                             uint32_t dense_cap, const float4* tri, float4* sub_trir,
+                             hipStream_t s);
 // The whole second level after the top build on the same stream: the plan,
 // the skip radii of the empty top cells, then sub-cell offsets, references
 // and their 48-byte records.  Over a
