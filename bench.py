@@ -205,10 +205,11 @@ class Frame:
         return self.xf[frame]
 
     def concurrent_ok(self):
-        """Whether the shadow and bounce stages may run on two streams: the
-        persistent schedule of the one-level grid keeps its slot counter in
-        the scene, so two such launches at once would share it."""
-        return self.sub is not None or "p" not in self.sched
+        """Whether the shadow and bounce stages may run on two streams (the
+        persistent schedules keep their slot counter per stream, the ray-list
+        compaction its block sums per context: the frame compacts before the
+        branches split)."""
+        return True
 
     def run(self, frame, events=None, concurrent=False):
         """One frame.  Events (if given, N_EVENTS of them, each recorded on

@@ -443,6 +443,11 @@ int grt_init(int device, grt_context* out)
         return hip_check(e, "hipStreamCreate");
     }
     ctx->stream = ctx->own_stream;
+    if ((rc = ctx->stream_counters[ctx->stream].reserve(ctx, 64, false)) != GRT_OK) {
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return GRT_OK;
 }
@@ -457,6 +462,9 @@ int grt_shutdown(grt_context ctx)
         grt_scene_destroy(sc);
     ctx->instance_scratch.release(ctx);
     ctx->compact_scratch.release(ctx);
+    for (auto& kv : ctx->stream_counters)
+        kv.second.release(ctx);
+    ctx->stream_counters.clear();
     if (ctx->aux_stream) {
         (void)hipStreamSynchronize(ctx->aux_stream);
         (void)hipStreamDestroy(ctx->aux_stream);
@@ -475,7 +483,9 @@ int grt_set_stream(grt_context ctx, void* hip_stream)
     if (!ctx)
         return set_error(GRT_INVALID_PARAM, "grt_set_stream: NULL context");
     ctx->stream = (hipStream_t)hip_stream;
-    return GRT_OK;
+    // the stream's persistent-launch slot counter, allocated here (not under
+    // a capture that a later trace may be part of)
+    return ctx->stream_counters[ctx->stream].reserve(ctx, 64, false);
 }
 
 int grt_synchronize(grt_context ctx)
@@ -1223,6 +1233,14 @@ static uint32_t trace2_lanes(const grt_context_t* ctx)
     return ctx->trace_mode == GRT_TRACE_SPLIT ? ctx->split_lanes : 1u;
 }
 
+// The slot counter of a persistent launch on the context's current stream.
+static uint32_t* stream_counter(grt_context_t* ctx, int& rc)
+{
+    grt::DevMem& m = ctx->stream_counters[ctx->stream];   // allocated by grt_set_stream
+    rc = m.reserve(ctx, 64, false);
+    return rc == GRT_OK ? m.as<uint32_t>() : nullptr;
+}
+
 // One closest-hit launch of the context's schedule.
 static int dispatch_closest(grt_scene sc, const TraceArgs& a, const grt_rays* rays,
                             const grt_hits* hits)
@@ -1233,11 +1251,15 @@ static int dispatch_closest(grt_scene sc, const TraceArgs& a, const grt_rays* ra
         return hip_check(launch_trace2(a, to_soa(rays), h, nullptr, trace2_lanes(ctx),
                                        ctx->stream),
                          "trace_closest");
-    if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
-        return hip_check(launch_trace_persistent(a, to_soa(rays), h, nullptr, false,
-                                                 sc->small_counters.as<uint32_t>() + 2,
+    if (ctx->trace_mode == GRT_TRACE_PERSISTENT) {
+        int rc = GRT_OK;
+        uint32_t* counter = stream_counter(ctx, rc);
+        if (rc != GRT_OK)
+            return rc;
+        return hip_check(launch_trace_persistent(a, to_soa(rays), h, nullptr, false, counter,
                                                  ctx->block_closest, ctx->refill, ctx->stream),
                          "trace_closest");
+    }
     if (ctx->trace_mode == GRT_TRACE_SPLIT)
         return hip_check(launch_trace_split(a, to_soa(rays), h, nullptr, false, ctx->split_lanes,
                                             ctx->block_closest, ctx->stream),
@@ -1255,11 +1277,16 @@ static int dispatch_any(grt_scene sc, const TraceArgs& a, const grt_rays* rays,
         return hip_check(launch_trace2(a, to_soa(rays), HitSoA{}, occluded_dev, trace2_lanes(ctx),
                                        ctx->stream),
                          "trace_any");
-    if (ctx->trace_mode == GRT_TRACE_PERSISTENT)
+    if (ctx->trace_mode == GRT_TRACE_PERSISTENT) {
+        int rc = GRT_OK;
+        uint32_t* counter = stream_counter(ctx, rc);
+        if (rc != GRT_OK)
+            return rc;
         return hip_check(launch_trace_persistent(a, to_soa(rays), HitSoA{}, occluded_dev, true,
-                                                 sc->small_counters.as<uint32_t>() + 2,
-                                                 ctx->block_any, ctx->refill, ctx->stream),
+                                                 counter, ctx->block_any, ctx->refill,
+                                                 ctx->stream),
                          "trace_any");
+    }
     if (ctx->trace_mode == GRT_TRACE_SPLIT)
         return hip_check(launch_trace_split(a, to_soa(rays), HitSoA{}, occluded_dev, true,
                                             ctx->split_lanes, ctx->block_any, ctx->stream),

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <set>
 #include <vector>
 
@@ -68,6 +69,9 @@ struct grt_context_t {
     // asynchronous rebuild that run beside the main stream; created on first use.
     hipStream_t aux_stream = nullptr;
     hipEvent_t aux_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Per stream the context has traced on: the slot counter of persistent
+    // launches (launches on different streams may overlap).
+    std::map<hipStream_t, grt::DevMem> stream_counters;
 };
 
 struct grt_scene_t {

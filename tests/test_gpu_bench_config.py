@@ -265,7 +265,8 @@ def _two_level_stage(ctx, sc, rays, fr, offs, refs, g2, ov, closest, tag, lst, c
     ores = fn(fr, offs, refs, g2, ov, o, counters=True)
     for mode, param, listed in (("simple", 16, False), ("persistent", 16, False),
                                 ("pooled", 16, False), ("split", 4, False), ("split", 16, False),
-                                ("simple", 16, True), ("pooled", 16, True), ("split", 8, True)):
+                                ("simple", 16, True), ("pooled", 16, True), ("split", 8, True),
+                                ("persistent", 32, True)):
         ctx.set_trace_mode(mode, param)
         cnt = torch.full((2 * n,), -7, dtype=torch.int32, device="cuda:0")
         t = f"{tag} [{mode} {param}{' listed' if listed else ''}]"
@@ -370,12 +371,15 @@ def test_bench_two_level_frame(gpu_ctx, stadium, lam, lam2, min_refs):
 
 
 @pytest.mark.parametrize("grid,sched,lists", [((16.0, (4.0, 8)), "ss24", "--c"),
-                                              ((8.0, (4.0, 8)), "4448", "ccc")])
+                                              ((8.0, (4.0, 8)), "4448", "ccc"),
+                                              ((64.0, None), "pp2p", "-c-")])
 def test_bench_frame_branches_concurrent_same_as_sequential(gpu_ctx, grid, sched, lists):
     """The benchmarked frame with its shadow and bounce branches on two streams
     (the bounce rays, their shadow rays, list compaction and both traces on a
     second stream) writes exactly what the one-stream frame writes: hits,
-    occlusion flags of both levels and the RGBA8 image, for two frames."""
+    occlusion flags of both levels and the RGBA8 image, for two frames; on the
+    one-level grid both branches run the persistent schedule at once (each
+    stream has its own slot counter)."""
     import os
     import sys
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))

@@ -47,10 +47,11 @@ def check_levels(sc, v, lam, lam2, min_refs, tag=""):
     return fr, ooffs, orefs, (ocs, oso, osr)
 
 
-# Trace modes: one ray per lane (simple; pooled launches the same kernel on a
-# two-level grid), and 2, 8 and 16 lanes per ray walking the ray's top cells
-# in turn (the split schedule).
-MODES = [("simple", 16), ("pooled", 16), ("split", 2), ("split", 8), ("split", 16)]
+# Trace modes: one ray per lane (simple; pooled and persistent launch the same
+# kernel on a two-level grid), and 2, 8 and 16 lanes per ray walking the ray's
+# top items in turn (the split schedule).
+MODES = [("simple", 16), ("pooled", 16), ("persistent", 16), ("split", 2), ("split", 8),
+         ("split", 16)]
 
 
 def check_trace2(sc, fr, offs, refs, g2, v, rays, image_width=0, one_level=True, modes=MODES):
