@@ -220,3 +220,47 @@ def test_two_level_any_matches_brute_force(two_level):
     assert 100 < o["occluded"].sum() < 5900
     one = orc.trace_any(fr, offs, refs, v, rays)["occluded"]
     assert math.isclose(float((o["occluded"] == one).mean()), 1.0)
+
+
+def test_rays_lying_in_a_subgrid_cell_plane(two_level):
+    """Rays with a zero direction component whose origin lies exactly on the
+    lower boundary plane of a cell with a sub-grid (binary32 `cx cell + bmin`):
+    every plane t on that axis is 0 x inf = NaN or infinite, so the walk must
+    neither cross nor drop the axis.  Aimed through triangles straddling the
+    plane, their closest hits equal the one-level oracle's and float64 brute
+    force."""
+    v, fr, offs, refs, (cs, so, sr) = two_level
+    f32 = np.float32
+    v9 = v.reshape(-1, 9)
+    nx, ny = int(fr["n"][0]), int(fr["n"][1])
+    cell, bmin = np.asarray(fr["cell"], f32), np.asarray(fr["bmin"], f32)
+    ox, oy, oz, dz = [], [], [], []
+    for c in np.nonzero(has_subgrid(cs))[0]:
+        cx = int(c) % nx
+        if cx == 0:
+            continue
+        plane = f32(f32(cx) * cell[0]) + bmin[0]
+        for t in refs[offs[c]:offs[c + 1]]:
+            tri = v9[t].reshape(3, 3)
+            if not (tri[:, 0].min() < plane < tri[:, 0].max()):
+                continue
+            ox.append(plane)
+            oy.append(tri[:, 1].mean())
+            oz.append(tri[:, 2].min() - f32(1.0))
+            dz.append(f32(1.0))
+        if len(ox) > 400:
+            break
+    n = len(ox)
+    assert n > 50
+    rays = {"ox": np.array(ox, f32), "oy": np.array(oy, f32), "oz": np.array(oz, f32),
+            "dx": np.zeros(n, f32), "dy": np.zeros(n, f32), "dz": np.array(dz, f32),
+            "tmin": np.zeros(n, f32), "tmax": np.full(n, np.inf, f32)}
+    o = orc.grid2_trace_closest(fr, offs, refs, (cs, so, sr), v, rays)
+    o1 = orc.trace_closest(fr, offs, refs, v, rays)
+    b = orc.brute_closest_f64(v, rays)
+    assert (o1["id"] >= 0).sum() > n // 4
+    assert np.array_equal(o["id"], o1["id"]) and np.array_equal(o["t"].view(np.uint32),
+                                                              o1["t"].view(np.uint32))
+    with np.errstate(invalid="ignore"):
+        near_tie = np.abs(b["t2"] - b["t1"]) <= 1e-6 * np.abs(b["t1"])
+    assert ((o["id"] != b["id"]) & ~near_tie).sum() == 0
