@@ -876,10 +876,11 @@ static int aux_stream_ready(grt_context_t* ctx)
 
 // The rest of an asynchronous rebuild of both levels, after the top scan, on
 // two streams.  What the second level plans from the top grid's offsets
-// alone (cell plans, skip radii, cell records) runs on the auxiliary stream
-// while the main stream scatters and sorts the top references; the top
-// records are gathered on the auxiliary stream while the main stream builds
-// the sub-grids from the sorted references.  The main stream waits for the
+// alone (cell plans, the occupied cells' records, the dense list) runs on the
+// auxiliary stream while the main stream scatters and sorts the top
+// references; the skip radii and empty cells' records, then the top records,
+// follow on the auxiliary stream while the main stream builds the sub-grids
+// from the plan and the sorted references.  The main stream waits for the
 // auxiliary one at the end, so callers see one stream-ordered rebuild.
 static int rebuild_two_streams_enqueue(grt_scene_t* sc);
 
@@ -909,7 +910,9 @@ static int rebuild_two_streams_enqueue(grt_scene_t* sc)
         (rc = hip_check(launch_grid2_plan_cells(b, sc->cells_cap, sc->g2_cells_cap,
                                                 sc->g2_dense_cap, aux),
                         "sub-grid plan")) != GRT_OK ||
-        (rc = hip_check(hipEventRecord(ev[1], aux), "hipEventRecord")) != GRT_OK)
+        (rc = hip_check(hipEventRecord(ev[1], aux), "hipEventRecord")) != GRT_OK ||
+        (rc = hip_check(launch_grid2_skip_records(b, sc->cells_cap, aux), "skip radii")) !=
+            GRT_OK)
         return rc;
     // top references scattered and sorted on the main stream
     if ((rc = enqueue_scatter_sort_only(sc)) != GRT_OK)

@@ -23,9 +23,10 @@
 //   k_g2_plan<false>  per tile: sub-cells and dense cells (tile sums); the
 //                     x pass of the empty cells' skip radii
 //   k_g2_scan_tiles   exclusive scan of both; totals, capacity
-//   k_g2_skip_line    y and z passes of the skip radii
-//   k_g2_plan<true>   per tile: cell records (skip radius of an empty cell),
-//                     the ordered dense-cell list
+//   k_g2_plan<true>   per tile: records of the occupied cells, the ordered
+//                     dense-cell list
+//   k_g2_skip_line    y and z passes of the skip radii; the z pass writes the
+//                     empty cells' records
 //   k_g2_count        per dense cell: boxes, sub-cell counts, sub-references
 //   k_g2_scan_dense   exclusive scan of those; total, capacity
 //   k_g2_fill         per dense cell: sub-cell offsets, references
@@ -117,14 +118,17 @@ __device__ __forceinline__ uint32_t row_distance(const uint64_t* occ, uint32_t p
     return d;
 }
 
-// y (AXIS 1: plane 0 -> plane 1 of skip) or z (AXIS 2: plane 1 -> plane 0)
-// pass over tiles of 64 lines x kSkipRows rows: the lines of a tile start at
+// y (AXIS 1: plane 0 -> plane 1 of skip) or z (AXIS 2: plane 1 -> the records
+// of the empty cells) pass over tiles of 64 lines x kSkipRows rows: the lines of a tile start at
 // 64 consecutive cells of coordinate 0 on the axis (x of one z plane for y;
 // x + n0 y for z).  Distances are stored as thermometer bytes, bit t set when
 // D <= t (D = 8: no bit), so that D_out <= t, i.e. some cell within t rows has
 // D_in <= t, is the OR over k <= t of bit t of the neighbours at +-k: per row
 // out = in | OR_k ((0xff << k) & (in(-k) | in(+k))), four lines per 32-bit
-// operation.  Rows outside the grid read 0 (no reference there).
+// operation.  Rows outside the grid read 0 (no reference there).  Bit 0 of the
+// final byte is the cell's own occupancy (D = 0), so the z pass writes {r, r},
+// r = D - 1, for exactly the cells without references, and the records of the
+// other cells come from k_g2_plan<true> alone.
 template <int AXIS>
 __global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
 {
@@ -142,7 +146,7 @@ __global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
     const uint32_t gx = (span + 63u) / 64u, segs = (len + rows - 1u) / rows;
     const uint64_t work = (uint64_t)gx * planes * segs;
     const uint8_t* in = AXIS == 1 ? b.skip : b.skip + b.skip_stride;
-    uint8_t* out = AXIS == 1 ? b.skip + b.skip_stride : b.skip;
+    uint8_t* out = b.skip + b.skip_stride;
     const uint32_t lane = threadIdx.x & 63u, row0 = threadIdx.x >> 6;
     const uint32_t col = threadIdx.x & 15u, rsub = threadIdx.x >> 4;
     uint8_t* s_bytes = reinterpret_cast<uint8_t*>(&s_in[0][0]);
@@ -175,8 +179,16 @@ __global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
 #pragma unroll
             for (uint32_t i = 0; i < 4u; ++i) {
                 const uint32_t a = a0 + 4u * col + i;
-                if (a < span)
-                    out[c + a] = (uint8_t)(t >> (8u * i));
+                const uint32_t v = (t >> (8u * i)) & 0xffu;
+                if (a >= span)
+                    continue;
+                if (AXIS == 1) {
+                    out[c + a] = (uint8_t)v;
+                } else if (!(v & 1u)) {
+                    // skip radius D - 1, D the lowest bit of the byte (8: none)
+                    const uint32_t r = (v ? (uint32_t)__ffs(v) - 1u : (uint32_t)kSkipMax + 1u) - 1u;
+                    b.cellrec[c + a] = make_uint2(r, r);
+                }
             }
         }
         __syncthreads();
@@ -184,9 +196,9 @@ __global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
 }
 
 // 1 (LIST = false): per 4096-cell tile, the sub-cells and dense cells it holds.
-// 3 (LIST = true): per tile, every cell's 8-byte record -- {0x80000000 | first
-// sub-cell, plan} for a cell with a sub-grid, else {refs begin, end} -- and
-// the tile's dense cells into the global dense list, in cell order.  Slot k of
+// 3 (LIST = true): per tile, the 8-byte record of every cell with references
+// -- {0x80000000 | first sub-cell, plan} for a cell with a sub-grid, else
+// {refs begin, end} -- and the tile's dense cells into the global dense list, in cell order.  Slot k of
 // thread j is cell base + 256 k + j, so loads and record stores are coalesced;
 // in-tile positions come from a wave scan per slot (sub-cells in bits 0..23,
 // the dense flag in bits 24..31: a wave's sums stay below 2^18 and 64) and a
@@ -287,12 +299,7 @@ __global__ __launch_bounds__(256) void k_g2_plan(G2Build b)
                 const uint32_t mpos = tm + s_tot[0][4u * k + wave] + ms[k];
                 b.cellrec[c] = make_uint2(0x80000000u | mpos, packed[k]);
                 b.dense[td + s_tot[1][4u * k + wave] + ed[k]] = c;
-            } else if ((empty >> k) & 1u) {
-                // skip radius D - 1, D the lowest bit of the thermometer byte (8: none)
-                const uint32_t t = b.skip[c];
-                const uint32_t r = (t ? (uint32_t)__ffs(t) - 1u : (uint32_t)kSkipMax + 1u) - 1u;
-                b.cellrec[c] = make_uint2(r, r);
-            } else {
+            } else if (!((empty >> k) & 1u)) {
                 b.cellrec[c] = make_uint2(b.offsets[c], b.offsets[c + 1]);
             }
         }
@@ -621,11 +628,18 @@ hipError_t launch_grid2_plan_cells(const G2Build& b, uint32_t top_cells_cap,
     hipLaunchKernelGGL(k_g2_plan<false>, dim3(nt), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_g2_scan_tiles, dim3(1), dim3(1024), 0, s, b, nt, sub_cells_cap,
                        dense_cap);
+    hipLaunchKernelGGL(k_g2_plan<true>, dim3(nt), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid2_skip_records(const G2Build& b, uint32_t top_cells_cap, hipStream_t s)
+{
+    if (b.skip_stride < top_cells_cap)
+        return hipErrorInvalidValue;
     const uint32_t nsk = top_cells_cap / 1024u + 1u;
     const dim3 gsk(nsk < 4096u ? nsk : 4096u);
     hipLaunchKernelGGL(k_g2_skip_line<1>, gsk, dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_g2_skip_line<2>, gsk, dim3(256), 0, s, b);
-    hipLaunchKernelGGL(k_g2_plan<true>, dim3(nt), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
@@ -641,7 +655,9 @@ hipError_t launch_grid2_plan_refs(const G2Build& b, uint32_t sub_refs_cap, uint3
 hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
                              uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s)
 {
-    const hipError_t e = launch_grid2_plan_cells(b, top_cells_cap, sub_cells_cap, dense_cap, s);
+    hipError_t e = launch_grid2_plan_cells(b, top_cells_cap, sub_cells_cap, dense_cap, s);
+    if (e == hipSuccess)
+        e = launch_grid2_skip_records(b, top_cells_cap, s);
     return e != hipSuccess ? e : launch_grid2_plan_refs(b, sub_refs_cap, dense_cap, s);
 }
 

@@ -121,11 +121,14 @@ struct G2Build {
 // kScanTile + 1 entries each.
 hipError_t launch_grid2_plan(const G2Build& b, uint32_t top_cells_cap, uint32_t sub_cells_cap,
                              uint32_t sub_refs_cap, uint32_t dense_cap, hipStream_t s);
-// The plan in two parts: what needs only the top grid's offsets (cell plans,
-// tile sums, skip radii, cell records, the dense list) and what needs its
+// The plan in three parts: what needs only the top grid's offsets (cell
+// plans, tile sums, the records of the occupied cells, the dense list), the
+// skip radii and records of the empty cells (after the plan's first pass;
+// nothing of the second level reads them), and what needs the top grid's
 // references (per dense cell: boxes, sub-cell counts; the dense scan).
 hipError_t launch_grid2_plan_cells(const G2Build& b, uint32_t top_cells_cap,
                                    uint32_t sub_cells_cap, uint32_t dense_cap, hipStream_t s);
+hipError_t launch_grid2_skip_records(const G2Build& b, uint32_t top_cells_cap, hipStream_t s);
 hipError_t launch_grid2_plan_refs(const G2Build& b, uint32_t sub_refs_cap, uint32_t dense_cap,
                                   hipStream_t s);
 // After the plan: sub-cell offsets and references, their sorts and records.
