@@ -1301,21 +1301,34 @@ __device__ __forceinline__ bool trace2_cell(const float4* __restrict__ rec, uint
 {
     if (b >= e)
         return false;
-    float4 A = rec[3ull * b], B = rec[3ull * b + 1], C = rec[3ull * b + 2];
-    for (uint32_t k = b; k < e; ++k) {
-        const float4 cA = A, cB = B, cC = C;
-        if (k + 1 < e) {
-            const uint64_t q = 3ull * (k + 1);
-            A = rec[q];
-            B = rec[q + 1];
-            C = rec[q + 2];
-        }
+    // Two record sets take turns: the next record loads while the current
+    // one is tested, unconditionally (the index stays within [b, e), so the
+    // last record may load twice) and without copies between iterations.
+    const uint32_t last = e - 1u;
+    float4 A0 = rec[3ull * b], B0 = rec[3ull * b + 1], C0 = rec[3ull * b + 2];
+    for (uint32_t k = b; k < e; k += 2) {
+        const uint64_t q1 = 3ull * (k + 1u < last ? k + 1u : last);
+        const float4 A1 = rec[q1], B1 = rec[q1 + 1], C1 = rec[q1 + 2];
         ++tests;
         if (ANY) {
-            if (any_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax))
+            if (any_test(A0, B0, C0, o0, o1, o2, d0, d1, d2, tmin, tmax))
                 return true;
         } else {
-            closest_test(cA, cB, cC, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu, bv);
+            closest_test(A0, B0, C0, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu, bv);
+        }
+        const uint64_t q2 = 3ull * (k + 2u < last ? k + 2u : last);
+        A0 = rec[q2];
+        B0 = rec[q2 + 1];
+        C0 = rec[q2 + 2];
+        if (k + 1u < e) {
+            ++tests;
+            if (ANY) {
+                if (any_test(A1, B1, C1, o0, o1, o2, d0, d1, d2, tmin, tmax))
+                    return true;
+            } else {
+                closest_test(A1, B1, C1, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu,
+                             bv);
+            }
         }
     }
     return !ANY && id >= 0;
