@@ -1456,12 +1456,11 @@ __device__ __forceinline__ bool top_cell2(const TraceArgs& a, const DevFrame* __
         ++cells;
         const float sexit = tmin_f(tmin_f(q.tx, q.ty), q.tz);
         // the next sub-cell and its offsets, loaded before this one's tests
+        // (unconditionally: a step out of the sub-grid leaves q.si in place)
         const uint32_t cb = kb, ce = ke;
         const bool more = sub_step(r, q);
-        if (more) {
-            kb = a.sub_offsets[q.si];
-            ke = a.sub_offsets[q.si + 1];
-        }
+        kb = a.sub_offsets[q.si];
+        ke = a.sub_offsets[q.si + 1];
         if (trace2_cell<ANY>(a.sub_trir, cb, ce, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
                              r.tmax, sexit * kExitScale, id, best, bu, bv, tests)) {
             occ = ANY;
@@ -1621,7 +1620,8 @@ __global__ __launch_bounds__(128) void k_trace2(TraceArgs a, RaySoA rays, HitSoA
             const bool more = !(e > r.tmax) && item_advance(fr, r, ns, rr, axis, e);
             const uint32_t nci =
                 (uint32_t)ns.cx + nx * ((uint32_t)ns.cy + ny * (uint32_t)ns.cz);
-            const uint2 nrec = more ? a.cellrec[nci] : make_uint2(0u, 0u);
+            // ns is inside the grid whether or not the walk goes on
+            const uint2 nrec = a.cellrec[nci];
             if ((!empty && top_cell2<ANY>(a, fr, tab, sub_ok, r, s, rec, ci, t_enter, e, occ, id,
                                           best, bu, bv, cells, tests)) ||
                 !more)
