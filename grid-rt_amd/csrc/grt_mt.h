@@ -111,4 +111,74 @@ __device__ __forceinline__ bool any_test(const float4 A, const float4 B, const f
     return mt_exact(m, t, u, v) && t > tmin && t <= tmax;
 }
 
+// ---------------------------------------------------------------------------
+// A list of records
+// ---------------------------------------------------------------------------
+
+// Read-only view in the constant address space: with a wave-uniform index the
+// compiler reads it through the scalar unit (loads only).
+typedef const __attribute__((address_space(4))) float* KF4;
+
+__device__ __forceinline__ float4 kload4(KF4 p, uint64_t i)
+{
+    return make_float4(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]);
+}
+
+__device__ __forceinline__ float4 rload(const float4* __restrict__ p, uint64_t i)
+{
+    return p[i];
+}
+
+__device__ __forceinline__ float4 rload(KF4 p, uint64_t i)
+{
+    return kload4(p, i);
+}
+
+// Tests the 48-byte records [b, e) of rec (a per-lane pointer, or the
+// constant-space view for wave-uniform b, e): closest (hits bounded by
+// `bound`) or any.  Counts the tests, an occluder included.  True when an
+// any-hit ray found its occluder, or a closest-hit ray holds a hit.  Two
+// record sets take turns: the next record loads while the current one is
+// tested, unconditionally (the index stays within [b, e), so the last record
+// may load twice), and nothing is copied between iterations -- a load under a
+// per-lane condition would make every test wait for it at the join.
+template <bool ANY, typename P>
+__device__ __forceinline__ bool test_records(P rec, uint32_t b, uint32_t e, float o0, float o1,
+                                             float o2, float d0, float d1, float d2, float tmin,
+                                             float tmax, float bound, int32_t& id, float& best,
+                                             float& bu, float& bv, uint32_t& tests)
+{
+    if (b >= e)
+        return false;
+    const uint32_t last = e - 1u;
+    float4 A0 = rload(rec, 3ull * b), B0 = rload(rec, 3ull * b + 1),
+           C0 = rload(rec, 3ull * b + 2);
+    for (uint32_t k = b; k < e; k += 2) {
+        const uint64_t q1 = 3ull * (k + 1u < last ? k + 1u : last);
+        const float4 A1 = rload(rec, q1), B1 = rload(rec, q1 + 1), C1 = rload(rec, q1 + 2);
+        ++tests;
+        if (ANY) {
+            if (any_test(A0, B0, C0, o0, o1, o2, d0, d1, d2, tmin, tmax))
+                return true;
+        } else {
+            closest_test(A0, B0, C0, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu, bv);
+        }
+        const uint64_t q2 = 3ull * (k + 2u < last ? k + 2u : last);
+        A0 = rload(rec, q2);
+        B0 = rload(rec, q2 + 1);
+        C0 = rload(rec, q2 + 2);
+        if (k + 1u < e) {
+            ++tests;
+            if (ANY) {
+                if (any_test(A1, B1, C1, o0, o1, o2, d0, d1, d2, tmin, tmax))
+                    return true;
+            } else {
+                closest_test(A1, B1, C1, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu,
+                             bv);
+            }
+        }
+    }
+    return !ANY && id >= 0;
+}
+
 }  // namespace grt

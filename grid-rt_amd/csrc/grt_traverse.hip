@@ -131,14 +131,6 @@ __device__ __forceinline__ bool dda_step(Dda& s)
     return true;
 }
 
-// Read-only view in the constant address space: with a wave-uniform index the
-// compiler reads it through the scalar unit (loads only).
-typedef const __attribute__((address_space(4))) float* KF4;
-
-__device__ __forceinline__ float4 kload4(KF4 p, uint64_t i)
-{
-    return make_float4(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]);
-}
 
 // ---------------------------------------------------------------------------
 // Wave-cooperative cell tests
@@ -512,34 +504,12 @@ __device__ __forceinline__ void trace_closest_lane(const DevFrame* __restrict__ 
             const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
             if (__ballot(b != b0 || e != e0) == 0) {
                 // Every lane here is in this cell: wave-uniform records.
-                float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
-                       C = kload4(ktr, 3ull * b0 + 2);
-                for (uint32_t k = b0; k < e0; ++k) {
-                    const float4 cA = A, cB = B, cC = C;
-                    if (k + 1 < e0) {
-                        const uint64_t q = 3ull * (k + 1);
-                        A = kload4(ktr, q);
-                        B = kload4(ktr, q + 1);
-                        C = kload4(ktr, q + 2);
-                    }
-                    closest_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax,
-                                 r.bound, r.id, r.best, r.bu, r.bv);
-                }
+                test_records<false>(ktr, b0, e0, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                                    r.tmax, r.bound, r.id, r.best, r.bu, r.bv, r.tests);
             } else {
-                float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
-                for (uint32_t k = b; k < e; ++k) {
-                    const float4 cA = A, cB = B, cC = C;
-                    if (k + 1 < e) {
-                        const uint64_t q = 3ull * (k + 1);
-                        A = trir[q];
-                        B = trir[q + 1];
-                        C = trir[q + 2];
-                    }
-                    closest_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax,
-                                 r.bound, r.id, r.best, r.bu, r.bv);
-                }
+                test_records<false>(trir, b, e, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                                    r.tmax, r.bound, r.id, r.best, r.bu, r.bv, r.tests);
             }
-            r.tests += e - b;
             r.k = e;
         }
         if (busy) {
@@ -619,43 +589,17 @@ __device__ __forceinline__ void trace_any_lane(const DevFrame* __restrict__ fr,
             const uint32_t b = r.k, e = r.e;
             const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
             const uint32_t e0 = __builtin_amdgcn_readfirstlane(e);
-            uint32_t k = b;
-            if (__ballot(b != b0 || e != e0) == 0) {
-                float4 A = kload4(ktr, 3ull * b0), B = kload4(ktr, 3ull * b0 + 1),
-                       C = kload4(ktr, 3ull * b0 + 2);
-                for (k = b0; k < e0; ++k) {
-                    const float4 cA = A, cB = B, cC = C;
-                    if (k + 1 < e0) {
-                        const uint64_t q = 3ull * (k + 1);
-                        A = kload4(ktr, q);
-                        B = kload4(ktr, q + 1);
-                        C = kload4(ktr, q + 2);
-                    }
-                    if (any_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
-                                 r.tmax)) {
-                        r.id = 1;
-                        break;
-                    }
-                }
-            } else {
-                float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
-                for (; k < e; ++k) {
-                    const float4 cA = A, cB = B, cC = C;
-                    if (k + 1 < e) {
-                        const uint64_t q = 3ull * (k + 1);
-                        A = trir[q];
-                        B = trir[q + 1];
-                        C = trir[q + 2];
-                    }
-                    if (any_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
-                                 r.tmax)) {
-                        r.id = 1;
-                        break;
-                    }
-                }
-            }
+            int32_t nid = -1;
+            float nt = 0.0f, nu = 0.0f, nv = 0.0f;
             // references tested: up to and including an occluder
-            r.tests += (r.id >= 0 ? k + 1 : e) - b;
+            const bool occ =
+                __ballot(b != b0 || e != e0) == 0
+                    ? test_records<true>(ktr, b0, e0, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                                         r.tmax, 0.0f, nid, nt, nu, nv, r.tests)
+                    : test_records<true>(trir, b, e, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                                         r.tmax, 0.0f, nid, nt, nu, nv, r.tests);
+            if (occ)
+                r.id = 1;
             r.k = e;
         }
         if (busy) {
@@ -753,29 +697,17 @@ __device__ __forceinline__ bool dda_skip(Dda& s, uint32_t count, float tmax)
 template <bool ANY>
 __device__ __forceinline__ void lane_cell(const float4* __restrict__ trir, LaneRay& r)
 {
-    const uint32_t b = r.k, e = r.e;
-    float4 A = trir[3ull * b], B = trir[3ull * b + 1], C = trir[3ull * b + 2];
-    uint32_t k = b;
-    for (; k < e; ++k) {
-        const float4 cA = A, cB = B, cC = C;
-        if (k + 1 < e) {
-            const uint64_t q = 3ull * (k + 1);
-            A = trir[q];
-            B = trir[q + 1];
-            C = trir[q + 2];
-        }
-        if (ANY) {
-            if (any_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax)) {
-                r.id = 1;
-                break;
-            }
-        } else {
-            closest_test(cA, cB, cC, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax, r.bound,
-                         r.id, r.best, r.bu, r.bv);
-        }
+    if (ANY) {
+        int32_t nid = -1;
+        float nt = 0.0f, nu = 0.0f, nv = 0.0f;
+        if (test_records<true>(trir, r.k, r.e, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin,
+                               r.tmax, 0.0f, nid, nt, nu, nv, r.tests))
+            r.id = 1;
+    } else {
+        test_records<false>(trir, r.k, r.e, r.o0, r.o1, r.o2, r.d0, r.d1, r.d2, r.tmin, r.tmax,
+                            r.bound, r.id, r.best, r.bu, r.bv, r.tests);
     }
-    r.tests += ((ANY && r.id >= 0) ? k + 1 : e) - b;
-    r.k = e;
+    r.k = r.e;
 }
 
 // Sum over the `lanes`-lane group (every lane of the wave calls it).
@@ -1299,39 +1231,8 @@ __device__ __forceinline__ bool trace2_cell(const float4* __restrict__ rec, uint
                                             float bound, int32_t& id, float& best, float& bu,
                                             float& bv, uint32_t& tests)
 {
-    if (b >= e)
-        return false;
-    // Two record sets take turns: the next record loads while the current
-    // one is tested, unconditionally (the index stays within [b, e), so the
-    // last record may load twice) and without copies between iterations.
-    const uint32_t last = e - 1u;
-    float4 A0 = rec[3ull * b], B0 = rec[3ull * b + 1], C0 = rec[3ull * b + 2];
-    for (uint32_t k = b; k < e; k += 2) {
-        const uint64_t q1 = 3ull * (k + 1u < last ? k + 1u : last);
-        const float4 A1 = rec[q1], B1 = rec[q1 + 1], C1 = rec[q1 + 2];
-        ++tests;
-        if (ANY) {
-            if (any_test(A0, B0, C0, o0, o1, o2, d0, d1, d2, tmin, tmax))
-                return true;
-        } else {
-            closest_test(A0, B0, C0, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu, bv);
-        }
-        const uint64_t q2 = 3ull * (k + 2u < last ? k + 2u : last);
-        A0 = rec[q2];
-        B0 = rec[q2 + 1];
-        C0 = rec[q2 + 2];
-        if (k + 1u < e) {
-            ++tests;
-            if (ANY) {
-                if (any_test(A1, B1, C1, o0, o1, o2, d0, d1, d2, tmin, tmax))
-                    return true;
-            } else {
-                closest_test(A1, B1, C1, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu,
-                             bv);
-            }
-        }
-    }
-    return !ANY && id >= 0;
+    return test_records<ANY>(rec, b, e, o0, o1, o2, d0, d1, d2, tmin, tmax, bound, id, best, bu,
+                             bv, tests);
 }
 
 __device__ __forceinline__ int32_t sub_vox(float val, float bmin, float inv, int32_t r)
