@@ -1262,10 +1262,11 @@ __device__ __forceinline__ void stage_sub_tab(const TraceArgs& a, float* tab)
 }
 
 // DDA state inside a top cell's sub-grid: current sub-cell (and its index si
-// in the sub-cell arrays), next boundary crossings and their increments.
+// in the sub-cell arrays, which a step along y or z changes by sy or sz; by
+// +-1 along x), next boundary crossings and their increments.
 struct Sub2 {
-    uint32_t base, si;
-    int32_t r0, r1, qx, qy, qz, ox, oy, oz;
+    uint32_t si, sy, sz;
+    int32_t qx, qy, qz, ox, oy, oz;
     float tx, ty, tz, dtx, dty, dtz;
 };
 
@@ -1274,20 +1275,20 @@ __device__ __forceinline__ void sub_setup(const DevFrame* __restrict__ fr, const
                                           const Ray2& r, const Dda& s, uint2 rec, float t_enter,
                                           Sub2& q)
 {
-    q.base = rec.x & 0x7fffffffu;
-    q.r0 = (int32_t)(rec.y & 255u);
-    q.r1 = (int32_t)((rec.y >> 8) & 255u);
+    const uint32_t base = rec.x & 0x7fffffffu;
+    const int32_t r0 = (int32_t)(rec.y & 255u);
+    const int32_t r1 = (int32_t)((rec.y >> 8) & 255u);
     const int32_t r2 = (int32_t)((rec.y >> 16) & 255u);
     const float sb0 = (float)s.cx * fr->cell[0] + fr->bmin[0];
     const float sb1 = (float)s.cy * fr->cell[1] + fr->bmin[1];
     const float sb2 = (float)s.cz * fr->cell[2] + fr->bmin[2];
-    const float sc0 = tab[q.r0 - 1], sc1 = tab[16 + q.r1 - 1], sc2 = tab[32 + r2 - 1];
-    const float si0 = tab[48 + q.r0 - 1], si1 = tab[64 + q.r1 - 1], si2 = tab[80 + r2 - 1];
-    q.qx = sub_vox(r.o0 + r.d0 * t_enter, sb0, si0, q.r0);
-    q.qy = sub_vox(r.o1 + r.d1 * t_enter, sb1, si1, q.r1);
+    const float sc0 = tab[r0 - 1], sc1 = tab[16 + r1 - 1], sc2 = tab[32 + r2 - 1];
+    const float si0 = tab[48 + r0 - 1], si1 = tab[64 + r1 - 1], si2 = tab[80 + r2 - 1];
+    q.qx = sub_vox(r.o0 + r.d0 * t_enter, sb0, si0, r0);
+    q.qy = sub_vox(r.o1 + r.d1 * t_enter, sb1, si1, r1);
     q.qz = sub_vox(r.o2 + r.d2 * t_enter, sb2, si2, r2);
-    q.ox = r.n0 ? -1 : q.r0;
-    q.oy = r.n1 ? -1 : q.r1;
+    q.ox = r.n0 ? -1 : r0;
+    q.oy = r.n1 ? -1 : r1;
     q.oz = r.n2 ? -1 : r2;
     q.tx = ((float)(q.qx + (r.n0 ? 0 : 1)) * sc0 + sb0 - r.o0) * r.i0;
     q.ty = ((float)(q.qy + (r.n1 ? 0 : 1)) * sc1 + sb1 - r.o1) * r.i1;
@@ -1295,31 +1296,35 @@ __device__ __forceinline__ void sub_setup(const DevFrame* __restrict__ fr, const
     q.dtx = fabsf(sc0 * r.i0);
     q.dty = fabsf(sc1 * r.i1);
     q.dtz = fabsf(sc2 * r.i2);
-    q.si = q.base + (uint32_t)q.qx +
-           (uint32_t)q.r0 * ((uint32_t)q.qy + (uint32_t)q.r1 * (uint32_t)q.qz);
+    q.si = base + (uint32_t)q.qx +
+           (uint32_t)r0 * ((uint32_t)q.qy + (uint32_t)r1 * (uint32_t)q.qz);
+    q.sy = r.n1 ? 0u - (uint32_t)r0 : (uint32_t)r0;
+    q.sz = r.n2 ? 0u - (uint32_t)(r0 * r1) : (uint32_t)(r0 * r1);
 }
 
 // One sub-grid step (the top DDA's tie order); false when it leaves the grid.
 __device__ __forceinline__ bool sub_step(const Ray2& r, Sub2& q)
 {
     if (q.tx < q.ty && q.tx < q.tz) {
-        q.qx += r.n0 ? -1 : 1;
+        const int32_t d = r.n0 ? -1 : 1;
+        q.qx += d;
         if (q.qx == q.ox)
             return false;
         q.tx += q.dtx;
+        q.si += (uint32_t)d;
     } else if (q.ty < q.tz) {
         q.qy += r.n1 ? -1 : 1;
         if (q.qy == q.oy)
             return false;
         q.ty += q.dty;
+        q.si += q.sy;
     } else {
         q.qz += r.n2 ? -1 : 1;
         if (q.qz == q.oz)
             return false;
         q.tz += q.dtz;
+        q.si += q.sz;
     }
-    q.si = q.base + (uint32_t)q.qx +
-           (uint32_t)q.r0 * ((uint32_t)q.qy + (uint32_t)q.r1 * (uint32_t)q.qz);
     return true;
 }
 
