@@ -550,6 +550,9 @@ def kdtree_config(lam: float = 256.0, reps: int = 5, width: int = 1920, height: 
     lam1, lam2, min_refs = two_level
     sc.set_subgrid(lam2, min_refs)
     sc.build_grid(lam1)
+    # an untimed rebuild first, like the one-level grid's above (the first
+    # asynchronous two-level rebuild also sets up the context's second stream)
+    sc.rebuild_grid_async(lam1)
     ev_b[0].record()
     sc.rebuild_grid_async(lam1)
     ev_b[1].record()
