@@ -119,9 +119,9 @@ __device__ __forceinline__ uint32_t row_distance(const uint64_t* occ, uint32_t p
 }
 
 // y (AXIS 1: plane 0 -> plane 1 of skip) or z (AXIS 2: plane 1 -> the records
-// of the empty cells) pass over tiles of 64 lines x kSkipRows rows: the lines of a tile start at
-// 64 consecutive cells of coordinate 0 on the axis (x of one z plane for y;
-// x + n0 y for z).  Distances are stored as thermometer bytes, bit t set when
+// of the empty cells) pass over tiles of 64 lines x kSkipRows rows: the lines
+// of a tile start at 64 consecutive cells of coordinate 0 on the axis (x of one
+// z plane for y; x + n0 y for z).  Distances are stored as thermometer bytes, bit t set when
 // D <= t (D = 8: no bit), so that D_out <= t, i.e. some cell within t rows has
 // D_in <= t, is the OR over k <= t of bit t of the neighbours at +-k: per row
 // out = in | OR_k ((0xff << k) & (in(-k) | in(+k))), four lines per 32-bit
@@ -198,11 +198,12 @@ __global__ __launch_bounds__(256) void k_g2_skip_line(G2Build b)
 // 1 (LIST = false): per 4096-cell tile, the sub-cells and dense cells it holds.
 // 3 (LIST = true): per tile, the 8-byte record of every cell with references
 // -- {0x80000000 | first sub-cell, plan} for a cell with a sub-grid, else
-// {refs begin, end} -- and the tile's dense cells into the global dense list, in cell order.  Slot k of
-// thread j is cell base + 256 k + j, so loads and record stores are coalesced;
-// in-tile positions come from a wave scan per slot (sub-cells in bits 0..23,
-// the dense flag in bits 24..31: a wave's sums stay below 2^18 and 64) and a
-// scan of the 64 (slot, wave) totals in slot-major order.
+// {refs begin, end} -- and the tile's dense cells into the global dense list,
+// in cell order.  Slot k of thread j is cell base + 256 k + j, so loads and
+// record stores are coalesced; in-tile positions come from a wave scan per slot
+// (sub-cells in bits 0..23, the dense flag in bits 24..31: a wave's sums stay
+// below 2^18 and 64) and a scan of the 64 (slot, wave) totals in slot-major
+// order.
 template <bool LIST>
 __global__ __launch_bounds__(256) void k_g2_plan(G2Build b)
 {
