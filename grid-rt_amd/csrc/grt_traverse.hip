@@ -1302,29 +1302,28 @@ __device__ __forceinline__ void sub_setup(const DevFrame* __restrict__ fr, const
     q.sz = r.n2 ? 0u - (uint32_t)(r0 * r1) : (uint32_t)(r0 * r1);
 }
 
-// One sub-grid step (the top DDA's tie order); false when it leaves the grid.
+// One sub-grid step (the top DDA's tie order); false when it leaves the grid
+// (q is then left as it was).  Branch-free: the crossed axis selects which
+// coordinate, crossing and index change.
 __device__ __forceinline__ bool sub_step(const Ray2& r, Sub2& q)
 {
-    if (q.tx < q.ty && q.tx < q.tz) {
-        const int32_t d = r.n0 ? -1 : 1;
-        q.qx += d;
-        if (q.qx == q.ox)
-            return false;
-        q.tx += q.dtx;
-        q.si += (uint32_t)d;
-    } else if (q.ty < q.tz) {
-        q.qy += r.n1 ? -1 : 1;
-        if (q.qy == q.oy)
-            return false;
-        q.ty += q.dty;
-        q.si += q.sy;
-    } else {
-        q.qz += r.n2 ? -1 : 1;
-        if (q.qz == q.oz)
-            return false;
-        q.tz += q.dtz;
-        q.si += q.sz;
-    }
+    const bool ax = q.tx < q.ty && q.tx < q.tz;
+    const bool ay = !ax && q.ty < q.tz;
+    const bool az = !ax && !ay;
+    const int32_t dx = r.n0 ? -1 : 1;
+    const int32_t qx = ax ? q.qx + dx : q.qx;
+    const int32_t qy = ay ? q.qy + (r.n1 ? -1 : 1) : q.qy;
+    const int32_t qz = az ? q.qz + (r.n2 ? -1 : 1) : q.qz;
+    if ((ax && qx == q.ox) || (ay && qy == q.oy) || (az && qz == q.oz))
+        return false;
+    q.qx = qx;
+    q.qy = qy;
+    q.qz = qz;
+    const float tx = q.tx + q.dtx, ty = q.ty + q.dty, tz = q.tz + q.dtz;
+    q.tx = ax ? tx : q.tx;
+    q.ty = ay ? ty : q.ty;
+    q.tz = az ? tz : q.tz;
+    q.si += ax ? (uint32_t)dx : (ay ? q.sy : q.sz);
     return true;
 }
 
