@@ -205,3 +205,58 @@ def test_grid2_lists_and_capacity(gpu_ctx, c1_scene):
                     "one-level cells after turning the level off")
     finally:
         sc.close()
+
+
+def test_grid2_rebuild_and_trace_replay_in_a_graph(gpu_ctx, c1_scene):
+    """A two-level rebuild and a trace captured in a launch graph (under
+    capture the rebuild stays on the captured stream instead of forking onto
+    the context's second stream) give the eager results on every replay, with
+    eager work between the replays."""
+    v, nrm = c1_scene
+    stream = grt.stream_create()
+    torch.cuda.synchronize()
+    gpu_ctx.set_stream(stream)
+    sc = _built(gpu_ctx, v, nrm, 2.0, 4.0, 8)
+    try:
+        rays = orc.primary_rays(grt.camera_dict(c1_camera(200, 150)))
+        n = len(rays["ox"])
+        rb = grt.RayBatch.from_numpy(rays)
+        hb = grt.HitBatch.empty(n)
+        torch.cuda.synchronize()
+
+        def work():
+            sc.rebuild_grid_async(2.0)
+            sc.trace_closest(rb, hb, image_width=200)
+
+        work()
+        gpu_ctx.synchronize()
+        eager = hb.numpy()
+        g = grt.Graph(gpu_ctx)
+        g.begin()
+        work()
+        g.end()
+        try:
+            for rep in range(3):
+                for t in hb.f.values():
+                    t.fill_(-3)
+                torch.cuda.synchronize()
+                g.launch()
+                gpu_ctx.synchronize()
+                got = hb.numpy()
+                for k in ("id", "t", "u", "v"):
+                    assert_same(got[k], eager[k], f"replay {rep} hit.{k}")
+                sc.trace_closest(rb, grt.HitBatch.empty(n), image_width=200)   # eager work
+                gpu_ctx.synchronize()
+        finally:
+            g.close()
+        sc.check_grid()
+        assert sc.subgrid_info()["built"]
+        fr, offs, refs, g2 = check_levels(sc, v, 2.0, 4.0, 8, "after the replays")
+        o = orc.grid2_trace_closest(fr, offs, refs, g2, v, rays)
+        for k in ("id", "t", "u", "v"):
+            assert_same(eager[k], o[k], f"eager hit.{k}")
+    finally:
+        sc.close()
+        gpu_ctx.synchronize()
+        gpu_ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        grt.stream_destroy(stream)
